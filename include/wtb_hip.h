@@ -1,0 +1,163 @@
+/*
+ * wtb_hip.h -- C-ABI of the MI355X (gfx950) exciton / Bethe-Salpeter path.
+ *
+ * The functions replace the hot calls of the reference's bsesolver
+ * (src/module_bse.F90:2023-2594).  Host arrays use the reference's Fortran
+ * layouts (column-major, 1-based in Fortran, complex(8) as interleaved double
+ * pairs), so a Fortran caller binds them with ISO_C_BINDING and passes its
+ * own arrays unchanged.  Device state is owned by the opaque context and
+ * stays resident between calls: the BSE matrix never crosses PCIe.
+ *
+ * Every call returns 0 on success or a negative WTB_ERR_* code; the message
+ * of the last failure is available from wtb_last_error().  One context per
+ * host thread; calls on a context are serialised on its HIP stream.
+ */
+#ifndef WTB_HIP_H
+#define WTB_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define WTB_OK               0
+#define WTB_ERR_ARG         -1   /* bad argument or shape                   */
+#define WTB_ERR_HIP         -2   /* HIP runtime / kernel launch failure      */
+#define WTB_ERR_NOCONV      -3   /* eigensolver did not converge             */
+#define WTB_ERR_NOCP        -4   /* band selection impossible (E_F / nv / nc) */
+#define WTB_ERR_NOMEM       -5   /* device allocation failed                 */
+#define WTB_ERR_STATE       -6   /* call out of order (e.g. solve before build) */
+#define WTB_ERR_UNSUPPORTED -7   /* option not available on this path        */
+
+/* model flags */
+#define WTB_FLAG_COMPAT_F32   1  /* round k.R, R_alpha, E_F, eta, 1/sqrt2 to float32 as the reference */
+#define WTB_FLAG_GAUGE_LAPACK 2  /* eigenvector phases of ZHEEV('V','L') (up to a real sign) */
+
+/* Coulomb potentials (COULOMB_POT card, module_bse.F90:49-72) */
+#define WTB_V2DK  1
+#define WTB_V3D   2
+#define WTB_V3DL  3
+#define WTB_V2DT  4
+#define WTB_V2DT2 5
+#define WTB_V2DOH 6
+#define WTB_V2DRK 7
+#define WTB_V0DT  8
+#define WTB_V1DT  9              /* not available: WTB_ERR_UNSUPPORTED */
+
+/* precision of the BSE kernel store and the Krylov matvecs */
+#define WTB_FP64   0             /* complex FP64 store, FP64 MFMA matvec                     */
+#define WTB_MIXED  1             /* complex FP32 store, split-bf16 MFMA matvec with FP32
+                                    accumulation, FP64 orthogonalisation / Rayleigh-Ritz /
+                                    final residual refinement                               */
+#define WTB_FP32   2             /* complex FP32 store, FP32 MFMA matvec (no refinement)     */
+
+/* kernel overlap convention */
+#define WTB_CONV_REFERENCE 0     /* V <c_i|c_j><v_i|v_j>, module_bse.F90:77-81 */
+#define WTB_CONV_TEXTBOOK  1     /* V <c_i|c_j><v_j|v_i>                        */
+
+/* optical components for wtb_oscillator */
+#define WTB_X  0
+#define WTB_Y  1
+#define WTB_Z  2
+#define WTB_SP 3
+#define WTB_SM 4
+
+typedef struct wtb_ctx wtb_ctx;
+
+/* Coulomb parameters: input cards COULOMB_POT, R_0, EDIEL_T/EDIEL/EDIEL_B,
+ * EDIEL_Z, W_COUL, LC, KTOL (module_input.F90:55-122). */
+typedef struct {
+    int kind;
+    double r0;
+    double ediel[3];   /* EDIEL_T, EDIEL, EDIEL_B */
+    double ez, w, lc, ktol;
+} wtb_coulomb;
+
+/* Filled by wtb_bse_solve. */
+typedef struct {
+    int iterations;          /* outer block iterations                       */
+    int nconv;               /* converged pairs among the nev requested       */
+    long matvec_columns;     /* columns pushed through H*X (all precisions)   */
+    double max_rel_residual; /* max_j ||H x_j - l_j x_j|| / |l_j| (FP64)       */
+    double t_solve_ms;       /* eigensolver wall time on the stream           */
+    double t_refine_ms;      /* FP64 refinement part of t_solve_ms            */
+    double t_matvec_ms;      /* time inside H*X kernels                       */
+    double t_rr_ms;          /* time in Rayleigh-Ritz / orthogonalisation     */
+} wtb_report;
+
+/* Context. device: HIP ordinal.  Replaces the implicit module state of
+ * module_bse.F90:9-15 (hbse, Ws, HE). */
+int wtb_create(wtb_ctx **ctx, int device);
+int wtb_destroy(wtb_ctx *ctx);
+const char *wtb_last_error(const wtb_ctx *ctx);
+
+/* TB model, replaces hamiltonian_input_read (module_hamiltonian.F90:35-194).
+ *   rvec  : int    rvec(nvec,3)           lattice coordinates of R
+ *   deg   : int    ffactor(nvec)          Wigner-Seitz degeneracies
+ *   hop_re, hop_im : double hopmatrices(nvec,norb,norb), ihopmatrices(...)
+ *   rlat  : double rlat(3,3), rlat(i,:) = a_i  (Angstrom)
+ *   efermi, scissor : eV;  flags : WTB_FLAG_* */
+int wtb_set_tb_model(wtb_ctx *ctx, int norb, int nvec, const int *rvec, const int *deg,
+                     const double *hop_re, const double *hop_im, const double *rlat,
+                     double efermi, double scissor, int flags);
+
+/* k-points, replaces monhkhorst_pack's output (module_kmesh.F90:338-386).
+ *   kpt : double kpt(nk,3) Cartesian (1/Angstrom);  ngrid : the MP grid
+ *   (its product normalises the Coulomb kernel, module_coulomb.F90:316). */
+int wtb_set_kpoints(wtb_ctx *ctx, int nk, const double *kpt, const int *ngrid);
+
+/* Batched H(k) build + Hermitian eigensolve for every k, band selection and
+ * scissor: replaces the eigsys loop of bsesolver (module_bse.F90:2230-2245,
+ * module_hamiltonian.F90:200-320).  Outputs (each may be NULL):
+ *   eig  : double eigv(nk, nc+nv)
+ *   vec  : complex(8) vector(nk, nc+nv, norb)
+ *   nocp : int nocpk(nk) */
+int wtb_hk_eig(wtb_ctx *ctx, int nv, int nc, double *eig, double *vec, int *nocp);
+
+/* Full spectrum of every H(k): energies(nk,norb) (scissor applied) and
+ * eigenvectors(nk,norb,norb) with ev(k,o,n) = component o of band n. */
+int wtb_get_bands(wtb_ctx *ctx, double *energies, double *vectors);
+
+/* BSE kernel assembly on the device: replaces matrizelbse + the assembly
+ * loop (module_bse.F90:24-83,2392-2415).  precision: WTB_FP64/MIXED/FP32;
+ * convention: WTB_CONV_*.  The N x N matrix (N = nk*nc*nv) stays on the GPU. */
+int wtb_bse_build(wtb_ctx *ctx, const wtb_coulomb *coulomb, int precision, int convention);
+
+/* Lowest nev eigenpairs of the BSE matrix: replaces slepc_solver(HX,EX,HE,NX,NE)
+ * (module_slepc_solver.F90:12-97) with nev explicit.  tol: relative residual
+ * (SLEPc default 1e-8); max_it: outer iterations (200).
+ *   E : double EX(nev), ascending;  X : complex(8) HE(N,nev) or NULL;
+ *   rep : may be NULL. */
+int wtb_bse_solve(wtb_ctx *ctx, int nev, double tol, int max_it, double *E, double *X,
+                  wtb_report *rep);
+
+/* Optical dipoles of every transition: replaces the optsp/hlm loop
+ * (module_bse.F90:2317-2331, module_optics.F90:12-167).
+ *   d : complex(8) d(N,5) with columns hrx, hry, hrz, hrsp, hrsm, or NULL. */
+int wtb_optics(wtb_ctx *ctx, double cshift, double *d);
+
+/* Exciton optical activity: replaces dielbsev (comp_a == comp_b) and
+ * dielbsep (module_bse.F90:171-245).  act : double act(N); rows beyond nev
+ * are zero.  Needs wtb_bse_solve and wtb_optics first. */
+int wtb_oscillator(wtb_ctx *ctx, int nev, int comp_a, int comp_b, double *act);
+
+/* Test and integration helpers.
+ *   wtb_bse_get_block : copy H(i0:i0+ni-1, j0:j0+nj-1) (0-based) as complex(8)
+ *                       column-major ni x nj.
+ *   wtb_bse_matvec    : Y = H X for host X, Y complex(8) (N, ncol), with the
+ *                       kernel of the given precision (WTB_FP64 / WTB_MIXED /
+ *                       WTB_FP32).
+ *   wtb_bse_diag      : the transition energies E_c - E_v (N). */
+int wtb_bse_get_block(wtb_ctx *ctx, long i0, long ni, long j0, long nj, double *out);
+int wtb_bse_matvec(wtb_ctx *ctx, int ncol, const double *X, double *Y, int precision);
+int wtb_bse_diag(wtb_ctx *ctx, double *diag);
+
+/* Drop-in for slepc_solver(HX,EX,HE,NX,NE) (module_slepc_solver.F90:12-97):
+ * uploads the caller's full host matrix HX(NX,NX), returns the NE
+ * smallest-magnitude eigenpairs.  EX(NE) ascending in |E|; HE(NX,NE). */
+int wtb_slepc_solver_compat(const double *HX, double *EX, double *HE, int NX, int NE);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

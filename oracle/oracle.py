@@ -1,0 +1,237 @@
+"""ctypes binding of the CPU FP64 oracle (``oracle/libwtb_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY: imported by ``tests/``, ``__graft_entry__.smoke()``
+and the CPU-baseline leg of ``bench.py``, never by the product package.
+Array conventions are those documented in ``wtb_oracle.h`` (row-major,
+complex128 interleaved).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "libwtb_oracle.so")
+_lib = None
+
+COMPAT_F32 = 1
+GAUGE_LAPACK = 2
+COULOMB_KINDS = {"V2DK": 1, "V3D": 2, "V3DL": 3, "V2DT": 4, "V2DT2": 5, "V2DOH": 6,
+                 "V2DRK": 7, "V0DT": 8, "V1DT": 9}
+
+
+class _Coulomb(C.Structure):
+    _fields_ = [("kind", C.c_int), ("r0", C.c_double), ("ediel", C.c_double * 3),
+                ("ez", C.c_double), ("w", C.c_double), ("lc", C.c_double), ("ktol", C.c_double)]
+
+
+class _Model(C.Structure):
+    _fields_ = [("norb", C.c_int), ("nvec", C.c_int), ("rvec", C.c_void_p), ("deg", C.c_void_p),
+                ("hop", C.c_void_p), ("rlat", C.c_double * 9), ("efermi", C.c_double),
+                ("scissor", C.c_double), ("flags", C.c_int)]
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        d, i, p = C.c_double, C.c_int, C.c_void_p
+        _lib.or_pi_f32.restype = d
+        _lib.or_cic.restype = d
+        _lib.or_vcell2d.restype = d
+        _lib.or_vcell3d.restype = d
+        _lib.or_coulomb_v.restype = d
+        _lib.or_coulomb_v.argtypes = [p, p, p, p, p]
+        _lib.or_heev_jacobi.restype = i
+        _lib.or_hk_eig.restype = i
+        _lib.or_bse_matrix.restype = i
+        _lib.or_dipoles.restype = i
+        _lib.or_optdiel.argtypes = [i, d, i, p, d, p, p, d, p, p]
+        _lib.or_spectrum.argtypes = [i, p, p, i, p, d, d, d, i, d, i, p]
+    return _lib
+
+
+def _p(a):
+    return C.c_void_p(a.ctypes.data)
+
+
+class Model:
+    """Keeps the numpy buffers of a TB model alive for the C struct."""
+
+    def __init__(self, tb, flags=COMPAT_F32 | GAUGE_LAPACK):
+        self.rvec = np.ascontiguousarray(tb.rvec, np.int32)
+        self.deg = np.ascontiguousarray(tb.deg, np.int32)
+        self.hop = np.ascontiguousarray(tb.hop, np.complex128)
+        self.s = _Model(tb.norb, tb.nvec, _p(self.rvec).value, _p(self.deg).value,
+                        _p(self.hop).value, (C.c_double * 9)(*np.asarray(tb.rlat, float).ravel()),
+                        float(tb.efermi), float(tb.scissor), int(flags))
+        self.norb = tb.norb
+        self.rlat = np.ascontiguousarray(tb.rlat, np.float64)
+
+    @property
+    def ref(self):
+        return C.byref(self.s)
+
+
+def coulomb(kind="V2DRK", r0=1.0, ediel=(1.0, 1.0, 1.0), ez=1.0, w=0.0, lc=1.0, ktol=0.001):
+    return _Coulomb(COULOMB_KINDS[kind] if isinstance(kind, str) else int(kind), float(r0),
+                    (C.c_double * 3)(*ediel), float(ez), float(w), float(lc), float(ktol))
+
+
+def pi_f32():
+    return lib().or_pi_f32()
+
+
+def cic():
+    return lib().or_cic()
+
+
+def mp_mesh(n1, n2, n3, rlat, shift=(0.0, 0.0, 0.0)):
+    out = np.zeros((n1 * n2 * n3, 3))
+    sh = np.asarray(shift, float)
+    lib().or_mp_mesh(n1, n2, n3, _p(sh), _p(np.ascontiguousarray(rlat, float)), _p(out))
+    return out
+
+
+def recvec(rlat):
+    out = np.zeros((3, 3))
+    lib().or_recvec(_p(np.ascontiguousarray(rlat, float)), _p(out))
+    return out
+
+
+def hk(model: Model, k):
+    out = np.zeros((model.norb, model.norb), np.complex128)
+    lib().or_hk(model.ref, _p(np.ascontiguousarray(k, float)), _p(out))
+    return out
+
+
+def dhk(model: Model, k):
+    n = model.norb
+    out = np.zeros((3, n, n), np.complex128)
+    lib().or_dhk(model.ref, _p(np.ascontiguousarray(k, float)), _p(out[0]), _p(out[1]), _p(out[2]))
+    return out
+
+
+def heev_jacobi(a):
+    a = np.array(a, np.complex128, order="C")
+    n = a.shape[0]
+    w = np.zeros(n)
+    z = np.zeros((n, n), np.complex128)
+    st = lib().or_heev_jacobi(n, _p(a), _p(w), _p(z))
+    if st < 0:
+        raise RuntimeError("oracle Jacobi did not converge")
+    return w, z
+
+
+def householder_q(h):
+    h = np.ascontiguousarray(h, np.complex128)
+    q = np.zeros_like(h)
+    lib().or_householder_q(h.shape[0], _p(h), _p(q))
+    return q
+
+
+def gauge_lapack(h, z):
+    h = np.ascontiguousarray(h, np.complex128)
+    z = np.array(z, np.complex128, order="C")
+    lib().or_gauge_lapack(h.shape[0], _p(h), _p(z))
+    return z
+
+
+def hk_eig(model: Model, kpt, nv, nc, full=False):
+    kpt = np.ascontiguousarray(kpt, float)
+    nk, n, nb = kpt.shape[0], model.norb, nc + nv
+    eig = np.zeros((nk, nb))
+    vec = np.zeros((nk, nb, n), np.complex128)
+    nocp = np.zeros(nk, np.int32)
+    eall = np.zeros((nk, n)) if full else None
+    zall = np.zeros((nk, n, n), np.complex128) if full else None
+    st = lib().or_hk_eig(model.ref, nk, _p(kpt), nv, nc, _p(eig), _p(vec), _p(nocp),
+                         _p(eall) if full else None, _p(zall) if full else None)
+    res = dict(eig=eig, vec=vec, nocp=nocp, status=st)
+    if full:
+        res.update(eall=eall, zall=zall)
+    return res
+
+
+def coulomb_v(c, k1, k2, ngrid, rlat):
+    return lib().or_coulomb_v(C.byref(c), _p(np.ascontiguousarray(k1, float)),
+                              _p(np.ascontiguousarray(k2, float)),
+                              _p(np.ascontiguousarray(ngrid, np.int32)),
+                              _p(np.ascontiguousarray(rlat, float)))
+
+
+def bse_matrix(kpt, eig, vec, c, ngrid, rlat, nc, nv, textbook=False):
+    nk, nb, norb = vec.shape
+    N = nk * nc * nv
+    h = np.zeros((N, N), np.complex128)
+    st = lib().or_bse_matrix(nk, nc, nv, norb, _p(np.ascontiguousarray(kpt, float)),
+                             _p(np.ascontiguousarray(eig, float)),
+                             _p(np.ascontiguousarray(vec, np.complex128)), C.byref(c),
+                             _p(np.ascontiguousarray(ngrid, np.int32)),
+                             _p(np.ascontiguousarray(rlat, float)), int(textbook), _p(h))
+    if st:
+        raise RuntimeError(f"oracle bse_matrix status {st}")
+    return h
+
+
+def dipoles(model: Model, kpt, eig, vec, nc, nv, cshift):
+    nk = kpt.shape[0]
+    N = nk * nc * nv
+    d = np.zeros((5, N), np.complex128)
+    lib().or_dipoles(model.ref, nk, _p(np.ascontiguousarray(kpt, float)), nc, nv,
+                     _p(np.ascontiguousarray(eig, float)),
+                     _p(np.ascontiguousarray(vec, np.complex128)), C.c_double(cshift), _p(d))
+    return d
+
+
+def oscillator(x, d1, d2):
+    """x: (N, nev) exciton amplitudes (column n = state n)."""
+    x = np.asfortranarray(x, np.complex128)
+    n, nev = x.shape
+    act = np.zeros(n)
+    lib().or_oscillator(n, nev, _p(x), _p(np.ascontiguousarray(d1, np.complex128)),
+                        _p(np.ascontiguousarray(d2, np.complex128)), _p(act))
+    return act
+
+
+def bse_rows(kpt, eig, vec, c, ngrid, rlat, nc, nv, i0, ni):
+    nk, nb, norb = vec.shape
+    N = nk * nc * nv
+    h = np.zeros((ni, N), np.complex128)
+    lib().or_bse_rows(nk, nc, nv, norb, _p(np.ascontiguousarray(kpt, float)),
+                      _p(np.ascontiguousarray(eig, float)), _p(np.ascontiguousarray(vec, np.complex128)),
+                      C.byref(c), _p(np.ascontiguousarray(ngrid, np.int32)),
+                      _p(np.ascontiguousarray(rlat, float)), C.c_long(i0), C.c_long(ni), _p(h))
+    return h
+
+
+def matvec_rows(h, x):
+    h = np.ascontiguousarray(h, np.complex128)
+    x = np.ascontiguousarray(x, np.complex128)
+    y = np.zeros((h.shape[0], x.shape[1]), np.complex128)
+    lib().or_matvec_rows(C.c_long(h.shape[0]), C.c_long(h.shape[1]), x.shape[1], _p(h), _p(x), _p(y))
+    return y
+
+
+def num_threads():
+    return lib().or_num_threads()
+
+
+def spectrum(ex, fosc, diagonal, ngrid, vc, e0, e1, npts, sme, renorm=True):
+    ex = np.ascontiguousarray(ex, float)
+    fosc = np.ascontiguousarray(fosc, float)
+    out = np.zeros((npts, 3))
+    lib().or_spectrum(len(ex), _p(ex), _p(fosc), int(diagonal),
+                      _p(np.ascontiguousarray(ngrid, np.int32)), vc, e0, e1, npts, sme,
+                      int(renorm), _p(out))
+    return out

@@ -1,0 +1,203 @@
+"""Parity of the HIP path (through the C-ABI) against the CPU oracle."""
+import sys
+
+import numpy as np
+import pytest
+
+from conftest import cubic_model, hex_model
+from oracle import oracle as O
+from wtb_amd import hip
+from wtb_amd import tbmodel as T
+
+pytestmark = pytest.mark.gpu
+
+COUL = dict(kind="V2DRK", r0=40.0, ediel=(1.0, 1.0, 1.0), ez=1.0, w=0.0, lc=1.0, ktol=0.001)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def setup(model, mesh, nv, nc, flags=hip.FLAG_COMPAT_F32 | hip.FLAG_GAUGE_LAPACK):
+    k = T.mp_mesh(*mesh, model.rlat)
+    ctx = hip.Context(0)
+    ctx.set_tb_model(model, flags)
+    ctx.set_kpoints(k, mesh)
+    eig, vec, nocp = ctx.hk_eig(nv, nc)
+    ref = O.hk_eig(O.Model(model, flags=flags), k, nv, nc, full=True)
+    return ctx, k, eig, vec, nocp, ref
+
+
+def same_up_to_sign(a, b, tol):
+    """columns (last axis is the component) equal up to a real sign"""
+    ov = np.sum(a.conj() * b, axis=-1)
+    s = np.where(ov.real < 0, -1.0, 1.0)
+    return np.abs(a * s[..., None] - b).max() < tol
+
+
+@pytest.mark.parametrize("case", ["honeycomb-12", "hex22-8", "cubic16-4"])
+def test_hk_eig(case):
+    say("case", case)
+    if case == "honeycomb-12":
+        m, mesh, nv, nc = T.honeycomb_two_band(), (12, 12, 1), 1, 1
+    elif case == "hex22-8":
+        m, mesh, nv, nc = hex_model((8, 8, 1)), (8, 8, 1), 2, 2
+    else:
+        m, mesh, nv, nc = cubic_model((4, 4, 4)), (4, 4, 4), 3, 3
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, nv, nc)
+    assert np.array_equal(nocp, ref["nocp"])
+    assert np.abs(eig - ref["eig"]).max() < 1e-10
+    e_all, v_all = ctx.get_bands()
+    assert np.abs(e_all - ref["eall"]).max() < 1e-10
+    # LAPACK gauge on both sides: vectors agree up to a real sign
+    assert same_up_to_sign(vec, ref["vec"], 1e-8)
+    zg = np.transpose(v_all, (0, 2, 1))          # [k][band][orb]
+    zr = np.transpose(ref["zall"], (0, 2, 1))
+    assert same_up_to_sign(zg, zr, 1e-8)
+    ctx.close()
+
+
+def test_hk_eig_errors():
+    say("case hk errors")
+    m = hex_model((4, 4, 1))
+    ctx = hip.Context(0)
+    ctx.set_tb_model(m)
+    ctx.set_kpoints(T.mp_mesh(4, 4, 1, m.rlat), (4, 4, 1))
+    with pytest.raises(hip.WTBError) as e:
+        ctx.hk_eig(12, 2)
+    assert e.value.code == -4
+    m.efermi = 1e3
+    ctx.set_tb_model(m)
+    with pytest.raises(hip.WTBError) as e:
+        ctx.hk_eig(1, 1)
+    assert e.value.code == -4
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [hip.FP64, hip.MIXED])
+@pytest.mark.parametrize("conv", [hip.CONV_REFERENCE, hip.CONV_TEXTBOOK])
+def test_bse_matrix(precision, conv):
+    say("case bse matrix", precision, conv)
+    mesh = (8, 8, 1)
+    m = hex_model(mesh)
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, 2, 2)
+    ctx.bse_build(COUL, precision, conv)
+    H = ctx.get_block(0, ctx.N, 0, ctx.N)
+    Hr = O.bse_matrix(k, ref["eig"], ref["vec"], O.coulomb(**COUL), mesh, m.rlat, 2, 2,
+                      textbook=conv == hip.CONV_TEXTBOOK)
+    tol = 1e-12 if precision == hip.FP64 else 2e-6 * np.abs(Hr).max()
+    assert np.abs(H - Hr).max() < tol
+    assert np.allclose(ctx.diag(), np.diag(Hr).real, atol=1e-12)
+    ctx.close()
+
+
+@pytest.mark.parametrize("kind", ["V2DK", "V3D", "V2DT", "V2DT2", "V2DOH", "V0DT"])
+def test_bse_matrix_potentials(kind):
+    say("case potential", kind)
+    mesh = (6, 6, 1)
+    m = hex_model(mesh)
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, 2, 2)
+    c = dict(COUL, kind=kind, lc=8.0, w=0.5, ediel=(1.5, 4.0, 2.0), ez=2.0)
+    ctx.bse_build(c, hip.FP64)
+    H = ctx.get_block(0, ctx.N, 0, ctx.N)
+    Hr = O.bse_matrix(k, ref["eig"], ref["vec"], O.coulomb(**c), mesh, m.rlat, 2, 2)
+    assert np.abs(H - Hr).max() < 1e-11 * max(1.0, np.abs(Hr).max())
+    ctx.close()
+
+
+def test_bse_matrix_cubic_v3dl():
+    say("case cubic V3DL")
+    mesh = (4, 4, 4)
+    m = cubic_model(mesh)
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, 3, 3)
+    c = dict(COUL, kind="V3DL", ediel=(1.0, 6.0, 1.0))
+    ctx.bse_build(c, hip.FP64)
+    H = ctx.get_block(0, ctx.N, 0, ctx.N)
+    Hr = O.bse_matrix(k, ref["eig"], ref["vec"], O.coulomb(**c), mesh, m.rlat, 3, 3)
+    assert np.abs(H - Hr).max() < 1e-12
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [hip.FP64, hip.MIXED])
+@pytest.mark.parametrize("ncol", [1, 5, 16, 40, 130])
+def test_matvec(precision, ncol, rng):
+    say("case matvec", precision, ncol)
+    mesh = (10, 10, 1)
+    m = hex_model(mesh)
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, 2, 2)
+    ctx.bse_build(COUL, precision)
+    H = ctx.get_block(0, ctx.N, 0, ctx.N)
+    X = rng.standard_normal((ctx.N, ncol)) + 1j * rng.standard_normal((ctx.N, ncol))
+    Y = ctx.matvec(X, precision)
+    Yr = H @ X
+    rel = np.abs(Y - Yr).max() / np.abs(Yr).max()
+    assert rel < (1e-13 if precision == hip.FP64 else 1e-5)
+    if precision == hip.MIXED:
+        Y64 = ctx.matvec(X, hip.FP64)     # FP64 arithmetic on the FP32 store
+        assert np.abs(Y64 - Yr).max() / np.abs(Yr).max() < 1e-13
+    ctx.close()
+
+
+def test_solve_dense_small():
+    say("case dense honeycomb-12")
+    m = T.honeycomb_two_band()
+    mesh = (12, 12, 1)
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, 1, 1)
+    c = dict(COUL, r0=10.0)
+    ctx.bse_build(c, hip.FP64)
+    E, X, rep = ctx.bse_solve(144)
+    Hr = O.bse_matrix(k, ref["eig"], ref["vec"], O.coulomb(**c), mesh, m.rlat, 1, 1)
+    w = np.linalg.eigvalsh(Hr)
+    assert np.abs(E - w).max() < 1e-9
+    assert np.abs(Hr @ X - X * E).max() < 1e-8
+    ctx.close()
+
+
+@pytest.mark.parametrize("precision", [hip.FP64, hip.MIXED])
+def test_solve_lobpcg(precision):
+    say("case lobpcg", precision)
+    mesh = (20, 20, 1)
+    m = hex_model(mesh)
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, 2, 2)
+    ctx.bse_build(COUL, precision)
+    E, X, rep = ctx.bse_solve(32, tol=1e-8)
+    say("report", rep)
+    Hr = O.bse_matrix(k, ref["eig"], ref["vec"], O.coulomb(**COUL), mesh, m.rlat, 2, 2)
+    w = np.linalg.eigvalsh(Hr)[:32]
+    err = np.abs(E - w).max()
+    say("max |dE| eV", err)
+    if precision == hip.FP64:
+        assert err < 1e-9
+        r = np.linalg.norm(Hr @ X - X * E, axis=0) / np.abs(E)
+        assert r.max() < 1e-8
+    else:
+        assert err < 1e-4          # 0.1 meV
+    assert np.allclose(X.conj().T @ X, np.eye(32), atol=1e-8)
+    ctx.close()
+
+
+def test_optics_and_oscillator():
+    say("case optics")
+    mesh = (8, 8, 1)
+    m = hex_model(mesh)
+    flags = hip.FLAG_COMPAT_F32 | hip.FLAG_GAUGE_LAPACK
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, 2, 2, flags)
+    d = ctx.optics(0.05)
+    dr = O.dipoles(O.Model(m, flags=flags), k, ref["eig"], ref["vec"], 2, 2, 0.05)
+    # both sides use the LAPACK gauge up to a real sign per band; the product
+    # c^* v changes sign with either, so compare magnitudes and the sign-fixed values
+    assert np.abs(np.abs(d.T) - np.abs(dr)).max() < 1e-9
+    ctx.bse_build(COUL, hip.FP64)
+    E, X, rep = ctx.bse_solve(16)
+    # oracle oscillator on the same GPU eigenvectors and the oracle dipoles (gauge-aligned)
+    sgn_c = np.sign(np.sum(vec.conj() * ref["vec"], axis=-1).real)       # (nk, nb)
+    s = (sgn_c[:, 2:, None] * sgn_c[:, None, :2]).reshape(-1)               # per transition
+    assert np.abs(d[:, 0] - s * dr[0]).max() < 1e-9
+    for a, b in (("x", "x"), ("y", "y"), ("x", "y"), ("sp", "sp")):
+        act = ctx.oscillator(16, a, b)
+        ia, ib = hip.COMPONENTS[a], hip.COMPONENTS[b]
+        ar = O.oscillator(X, d[:, ia], d[:, ib])
+        assert np.abs(act - ar).max() < 1e-9 * max(1, np.abs(ar).max())
+        assert np.all(act[16:] == 0)
+    ctx.close()

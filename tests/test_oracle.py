@@ -1,0 +1,341 @@
+"""CPU oracle checks: every oracle function is pinned by an independent route
+(closed forms, LAPACK through scipy, invariants, brute force)."""
+import math
+
+import numpy as np
+import pytest
+import scipy.linalg.lapack as lapack
+
+from conftest import hex_model
+from oracle import oracle as O
+from wtb_amd import tbmodel as T
+
+
+def rand_herm(rng, n):
+    a = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    return a + a.conj().T
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 5, 16, 22, 31, 32])
+def test_jacobi_matches_lapack(rng, n):
+    a = rand_herm(rng, n)
+    w, z = O.heev_jacobi(a)
+    assert np.allclose(w, np.linalg.eigvalsh(a), atol=1e-12 * max(1, np.abs(w).max()))
+    assert np.allclose(a @ z, z * w, atol=1e-11)
+    assert np.allclose(z.conj().T @ z, np.eye(n), atol=1e-12)
+
+
+@pytest.mark.parametrize("n", [2, 3, 7, 22])
+def test_householder_q_reduces_to_real_tridiagonal(rng, n):
+    a = rand_herm(rng, n)
+    q = O.householder_q(a)
+    t = q.conj().T @ a @ q
+    assert np.allclose(q.conj().T @ q, np.eye(n), atol=1e-12)
+    assert np.abs(np.triu(t, 2)).max(initial=0) < 1e-12
+    assert np.abs(np.diag(t, -1).imag).max(initial=0) < 1e-12
+    # the same reduction as LAPACK zhetrd('L'): identical tridiagonal
+    _, d, e, _, info = lapack.zhetrd(a, lower=1)
+    assert info == 0
+    assert np.allclose(np.diag(t).real, d, atol=1e-11)
+    assert np.allclose(np.diag(t, -1).real, e, atol=1e-11)
+
+
+@pytest.mark.parametrize("n", [2, 5, 22])
+def test_gauge_matches_zheev_up_to_real_sign(rng, n):
+    a = rand_herm(rng, n)
+    w, z = O.heev_jacobi(a)
+    z = O.gauge_lapack(a, z)
+    w2, v2, info = lapack.zheev(a, compute_v=1, lower=1)
+    assert info == 0
+    ov = np.sum(z.conj() * v2, axis=0)
+    assert np.allclose(np.abs(ov.real), 1.0, atol=1e-10)
+    assert np.abs(ov.imag).max() < 1e-10
+
+
+def test_constants():
+    assert O.pi_f32() == float(np.float32(np.pi))
+    assert O.cic() == float(-(np.float32(0.0904756) * np.float32(1000.0)))
+    assert O.cic() == -90.47559356689453
+
+
+def test_recvec_and_mesh():
+    m = T.honeycomb_two_band()
+    b = O.recvec(m.rlat)
+    assert np.allclose(b @ m.rlat.T, 2 * O.pi_f32() * np.eye(3), atol=1e-12)
+    k = O.mp_mesh(4, 3, 2, m.rlat, shift=(0.1, 0.0, 0.0))
+    assert k.shape == (24, 3)
+    # i outer, l inner: the second point steps along b3
+    assert np.allclose(k[1] - k[0], b[2] / 2)
+    assert np.allclose(k[2] - k[0], b[1] / 3)
+    assert np.allclose(k[6] - k[0], b[0] / 4)
+    assert np.allclose(k[0], 0.1 * b[0])
+    assert np.allclose(O.mp_mesh(24, 24, 1, m.rlat), T.mp_mesh(24, 24, 1, m.rlat))
+
+
+def test_honeycomb_closed_form():
+    m = T.honeycomb_two_band()
+    k = O.mp_mesh(24, 24, 1, m.rlat)
+    r = O.hk_eig(O.Model(m, flags=0), k, 1, 1)
+    a1, a2 = m.rlat[0], m.rlat[1]
+    f = 1 + np.exp(-1j * k @ a1) + np.exp(-1j * k @ a2)
+    e = np.sqrt(4.0 + 2.8 ** 2 * np.abs(f) ** 2)
+    assert r["status"] == 0
+    assert np.all(r["nocp"] == 1)
+    assert np.abs(r["eig"][:, 1] - e).max() < 1e-12
+    assert np.abs(r["eig"][:, 0] + e).max() < 1e-12
+    # gap 4 eV at K = (b1 - b2)/3 lies on the 24x24 mesh
+    assert abs(np.min(r["eig"][:, 1] - r["eig"][:, 0]) - 4.0) < 1e-9
+
+
+def test_f32_phase_compat_changes_hk_at_float_level():
+    m = hex_model((8, 8, 1))
+    k = O.mp_mesh(8, 8, 1, m.rlat)[13]
+    h64 = O.hk(O.Model(m, flags=0), k)
+    h32 = O.hk(O.Model(m, flags=O.COMPAT_F32), k)
+    d = np.abs(h64 - h32).max()
+    assert 0 < d < 1e-4
+    # the float32 phase is exactly what the compat build evaluates
+    rc = m.rvec @ m.rlat
+    ph = np.float32(k @ rc.T).astype(np.float64)
+    href = np.einsum("r,rmn->mn", np.exp(1j * ph) / m.deg, m.hop) - float(np.float32(m.efermi)) * np.eye(m.norb)
+    assert np.abs(href - h32).max() < 1e-12
+
+
+def test_eigsys_scissor_and_band_selection():
+    m = hex_model((6, 6, 1))
+    m.scissor = 0.25
+    k = O.mp_mesh(6, 6, 1, m.rlat)
+    r = O.hk_eig(O.Model(m, flags=0), k, 2, 2, full=True)
+    assert np.all(r["nocp"] == 11)
+    w = np.linalg.eigvalsh(m.hk(k))
+    w[:, 11:] += 0.25
+    assert np.abs(r["eall"] - w).max() < 1e-10
+    assert np.abs(r["eig"] - w[:, 9:13]).max() < 1e-10
+    # selected vectors are eigenvectors of H(k) for the selected bands
+    h = m.hk(k)
+    for kk in (0, 7, 35):
+        for j in range(4):
+            v = r["vec"][kk, j]
+            e = r["eig"][kk, j] - (0.25 if j >= 2 else 0)
+            assert np.abs(h[kk] @ v - e * v).max() < 1e-10
+
+
+def test_band_selection_errors():
+    m = hex_model((4, 4, 1))
+    k = O.mp_mesh(4, 4, 1, m.rlat)
+    assert O.hk_eig(O.Model(m), k, 12, 1)["status"] == -4   # nv > nocp
+    m2 = m.copy()
+    m2.efermi = 1e3                                         # no band above E_F
+    assert O.hk_eig(O.Model(m2), k, 1, 1)["status"] == -4
+
+
+# ----------------------------- Coulomb ------------------------------------
+
+def _v(kind, q, rlat, ngrid=(10, 10, 1), **kw):
+    c = O.coulomb(kind, **kw)
+    return O.coulomb_v(c, np.array([q, 0.0, 0.0]), np.zeros(3), np.array(ngrid, np.int32), rlat)
+
+
+def test_v2drk_isotropic_closed_form():
+    m = T.honeycomb_two_band()
+    area = np.linalg.norm(np.cross(m.rlat[0], m.rlat[1]))
+    nk = 100
+    for q in (0.002, 0.05, 0.7):
+        for r0, eps, w in ((10.0, 1.0, 0.0), (40.0, 2.5, 0.3)):
+            v = _v("V2DRK", q, m.rlat, r0=r0, ediel=(eps, eps, eps), ez=eps, w=w, lc=7.0)
+            # p_t = p_b = 0 when all permittivities agree: eps_w = eps + r0 q e^{-q w}
+            ref = O.cic() * math.exp(-q * w) / (nk * area * q * (eps + r0 * q * math.exp(-q * w)))
+            assert abs(v - ref) <= 1e-12 * abs(ref)
+    assert _v("V2DRK", 5e-4, m.rlat, r0=10.0) == 0.0
+
+
+def test_v3d_and_v3dl_closed_form():
+    rlat = np.eye(3) * 4.0
+    for q in (0.01, 0.3):
+        ref = O.cic() / (10 * 10 * 1 * 64.0 * q * q)
+        assert abs(_v("V3D", q, rlat) - ref) < 1e-12 * abs(ref)
+        assert abs(_v("V3DL", q, rlat, ediel=(1.0, 6.0, 1.0)) - ref / 6.0) < 1e-12 * abs(ref)
+    assert _v("V3DL", 1e-4, rlat, ediel=(1.0, 6.0, 1.0)) == 0.0
+
+
+def test_v0dt_regularisation_is_the_q_to_zero_limit():
+    rlat = np.eye(3) * 4.0
+    v0 = _v("V0DT", 1e-4, rlat)
+    vq = _v("V0DT", 2e-3, rlat)      # (1 - cos(cr q))/q^2 -> cr^2/2
+    assert abs(vq - v0) < 1e-5 * abs(v0)
+
+
+def test_v2dk_and_v2dt2_limits():
+    m = T.honeycomb_two_band()
+    area = np.linalg.norm(np.cross(m.rlat[0], m.rlat[1]))
+    # V2DK with EDIEL=1 has r0 = 0: pure 2D Coulomb cic/(N A q ed)
+    q = 0.2
+    ref = O.cic() / (100 * area * q * 1.0)
+    assert abs(_v("V2DK", q, m.rlat) - ref) < 1e-12 * abs(ref)
+    # V2DT2 -> (1 - e^{-q lc/2})/q^2 / (N V) for in-plane q
+    vol = area * 20.0
+    lc = 8.0
+    ref = O.cic() / (100 * vol) / q ** 2 * (1 - math.exp(-0.5 * q * lc))
+    assert abs(_v("V2DT2", q, m.rlat, lc=lc) - ref) < 1e-12 * abs(ref)
+
+
+# ----------------------------- assembly -----------------------------------
+
+def _small_problem(mesh=(6, 6, 1), nv=2, nc=2, flags=O.COMPAT_F32 | O.GAUGE_LAPACK):
+    m = hex_model(mesh)
+    k = O.mp_mesh(*mesh, m.rlat)
+    r = O.hk_eig(O.Model(m, flags=flags), k, nv, nc)
+    assert r["status"] == 0
+    return m, k, r
+
+
+COUL = dict(kind="V2DRK", r0=40.0, ediel=(1.0, 1.0, 1.0), ez=1.0, w=0.0, lc=1.0, ktol=0.001)
+
+
+def _block_form(k, r, c, ngrid, rlat, nv, nc, textbook=False):
+    """Independent route: V(k,k') (M^c kron M^v) with numpy einsum."""
+    vec, eig = r["vec"], r["eig"]
+    nk = k.shape[0]
+    V = np.array([[O.coulomb_v(c, k[i], k[j], np.array(ngrid, np.int32), rlat) for j in range(nk)]
+                  for i in range(nk)])
+    uc, uv = vec[:, nv:], vec[:, :nv]
+    Mc = np.einsum("kco,pdo->kcpd", uc.conj(), uc)
+    Mv = np.einsum("kvo,pwo->kvpw", uv.conj(), uv)
+    if textbook:
+        Mv = Mv.conj()
+    H = np.einsum("kp,kcpd,kvpw->kcvpdw", V, Mc, Mv).reshape(nk * nc * nv, -1)
+    de = (eig[:, nv:, None] - eig[:, None, :nv]).reshape(-1)
+    H[np.diag_indices_from(H)] = de + np.diag(V).repeat(nc * nv)
+    return H
+
+
+@pytest.mark.parametrize("textbook", [False, True])
+def test_bse_matrix_block_form_and_hermitian(textbook):
+    m, k, r = _small_problem()
+    c = O.coulomb(**COUL)
+    H = O.bse_matrix(k, r["eig"], r["vec"], c, (6, 6, 1), m.rlat, 2, 2, textbook=textbook)
+    assert np.abs(H - H.conj().T).max() == 0.0
+    Hb = _block_form(k, r, c, (6, 6, 1), m.rlat, 2, 2, textbook)
+    assert np.abs(H - Hb).max() < 1e-14
+
+
+def test_bse_noninteracting_limit():
+    m, k, r = _small_problem()
+    c = O.coulomb(**dict(COUL, kind="V2DRK"))
+    H = O.bse_matrix(k, r["eig"], r["vec"], c, (6, 6, 1), m.rlat, 2, 2)
+    D = np.diag(H).real
+    c0 = O.coulomb(**dict(COUL, ktol=1e9))     # every q below KTOL -> V = 0
+    H0 = O.bse_matrix(k, r["eig"], r["vec"], c0, (6, 6, 1), m.rlat, 2, 2)
+    assert np.abs(H0 - np.diag(D)).max() == 0.0
+    assert np.allclose(np.linalg.eigvalsh(H0), np.sort(D))
+
+
+def test_bse_energies_gauge_invariant_both_conventions(rng):
+    m, k, r = _small_problem()
+    c = O.coulomb(**COUL)
+    ph = np.exp(1j * rng.uniform(0, 2 * np.pi, r["vec"].shape[:2]))
+    vec2 = r["vec"] * ph[:, :, None]
+    for tb in (False, True):
+        w1 = np.linalg.eigvalsh(O.bse_matrix(k, r["eig"], r["vec"], c, (6, 6, 1), m.rlat, 2, 2, tb))
+        w2 = np.linalg.eigvalsh(O.bse_matrix(k, r["eig"], vec2, c, (6, 6, 1), m.rlat, 2, 2, tb))
+        assert np.abs(w1 - w2).max() < 1e-12
+    wr = np.linalg.eigvalsh(O.bse_matrix(k, r["eig"], r["vec"], c, (6, 6, 1), m.rlat, 2, 2, False))
+    wt = np.linalg.eigvalsh(O.bse_matrix(k, r["eig"], r["vec"], c, (6, 6, 1), m.rlat, 2, 2, True))
+    assert np.abs(wr - wt).max() > 1e-6      # the two conventions are different physics
+
+
+# ------------------------------ optics ------------------------------------
+
+def test_dhk_matches_finite_difference():
+    m = hex_model((6, 6, 1))
+    M = O.Model(m, flags=0)
+    k = np.array([0.13, -0.21, 0.0])
+    d = O.dhk(M, k)
+    h = 1e-5
+    for a in range(2):
+        e = np.zeros(3)
+        e[a] = h
+        fd = (O.hk(M, k + e) - O.hk(M, k - e)) / (2 * h)
+        assert np.abs(d[a] - fd).max() < 1e-6 * np.abs(d[a]).max()
+
+
+def test_dipoles_definition_and_circular():
+    m, k, r = _small_problem(flags=0)
+    M = O.Model(m, flags=0)
+    d = O.dipoles(M, k, r["eig"], r["vec"], 2, 2, 0.05)
+    kk, c, v = 5, 1, 0
+    i = kk * 4 + c * 2 + v
+    dh = O.dhk(M, k[kk])
+    de = r["eig"][kk, 2 + c] - r["eig"][kk, v]
+    fac = (1 / (de + 0.05j)).real
+    for a in range(3):
+        ref = r["vec"][kk, 2 + c].conj() @ dh[a] @ r["vec"][kk, v] * fac
+        assert abs(d[a, i] - ref) < 1e-12
+    assert np.allclose(d[3], (d[0] + 1j * d[1]) / math.sqrt(2))
+    assert np.allclose(d[4], (d[0] - 1j * d[1]) / math.sqrt(2))
+
+
+def test_oscillator_phase_rule(rng):
+    """|sum_j X_j d_j|^2 has no conjugate on X: a conduction phase e^{i t}
+    multiplies term j by e^{-2 i t} (reference convention)."""
+    n, nev = 40, 3
+    x = rng.standard_normal((n, nev)) + 1j * rng.standard_normal((n, nev))
+    d = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    a = O.oscillator(x, d, d)
+    assert np.allclose(a[:nev], np.abs(x.T @ d) ** 2)
+    assert np.all(a[nev:] == 0)
+    # real sign flips of a basis state leave it unchanged
+    s = np.where(rng.uniform(size=n) < 0.5, -1.0, 1.0)
+    assert np.allclose(O.oscillator(x * s[:, None], d * s, d * s), a)
+    t = 0.3
+    x2, d2 = x.copy(), d.copy()
+    x2[0] *= np.exp(-1j * t)
+    d2[0] *= np.exp(-1j * t)
+    b = O.oscillator(x2, d2, d2)
+    ref = np.abs(x.T @ d + (np.exp(-2j * t) - 1) * x[0] * d[0]) ** 2
+    assert np.allclose(b[:nev], ref)
+
+
+# ----------------------------- spectrum -----------------------------------
+
+def test_single_line_lorentzian():
+    rlat = np.eye(3) * 3.0
+    vc = 27.0
+    out = O.spectrum(np.array([2.0]), np.array([1.5]), True, np.array([4, 4, 1], np.int32), vc,
+                     1.0, 3.0, 201, 0.05, renorm=False)
+    caux = float(np.float32(180.90)) / (16 * vc)
+    w = out[:, 0]
+    assert np.allclose(w, np.linspace(1.0, 3.0, 201))
+    assert np.allclose(out[:, 2], caux * 1.5 * 0.05 / ((w - 2.0) ** 2 + 0.05 ** 2))
+    assert np.allclose(out[:, 1], 1.0 + caux * 1.5 * (2.0 - w) / ((w - 2.0) ** 2 + 0.05 ** 2))
+
+
+def test_avgsme_telescopes_and_renorm():
+    ex = np.array([1.0, 1.5, 2.5])
+    f = np.array([1.0, 0.0, 2.0])
+    out = O.spectrum(ex, f, False, np.array([2, 2, 1], np.int32), 10.0, 0.0, 4.0, 41, 0.1, renorm=False)
+    gam = 0.1 + 10 * (2.5 - 1.0) / 3
+    caux = float(np.float32(180.90)) / (4 * 10.0)
+    w = out[:, 0]
+    ref = caux * sum(fi * gam / ((w - ei) ** 2 + gam ** 2) for ei, fi in zip(ex, f))
+    assert np.allclose(out[:, 2], ref)
+    assert np.allclose(out[:, 1], caux * sum(fi * (ei - w) / ((w - ei) ** 2 + gam ** 2) for ei, fi in zip(ex, f)))
+    rn = O.spectrum(ex, f, False, np.array([2, 2, 1], np.int32), 10.0, 0.0, 4.0, 41, 0.1, renorm=True)
+    cut = 1.0 - float(np.float32(0.15))
+    ib = np.max(np.nonzero(w <= cut)[0])
+    assert np.all(rn[:ib + 1, 2] == 0)
+    assert np.allclose(rn[ib + 1:, 2], np.maximum(out[ib + 1:, 2] - out[ib + 1, 2], 0))
+
+
+# --------------------------- TB file I/O -----------------------------------
+
+def test_tb_file_roundtrip(tmp_path):
+    m = hex_model((4, 4, 1))
+    p = tmp_path / "tb.dat"
+    T.write_tb_file(m, str(p))
+    m2 = T.read_tb_file(str(p))
+    assert m2.systype == m.systype and m2.norb == m.norb and m2.nvec == m.nvec
+    assert np.array_equal(m2.rvec, m.rvec) and np.array_equal(m2.deg, m.deg)
+    assert np.abs(m2.hop - m.hop).max() < 1e-15
+    assert abs(m2.efermi - m.efermi) < 1e-15
+    assert m.hermiticity_error() < 1e-15

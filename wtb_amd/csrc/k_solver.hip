@@ -1,0 +1,604 @@
+// Lowest eigenpairs of the Hermitian BSE matrix on the GPU (replaces
+// slepc_solver, src/module_slepc_solver.F90:12-97: Krylov-Schur for the nev
+// smallest-magnitude pairs, tol 1e-8, max_it 200).
+//
+// Method: block LOBPCG with soft locking and the exact-diagonal
+// preconditioner W = (diag(H) - l)^-1 R.  The BSE matrix is strongly
+// diagonally dominant (diag = E_c - E_v), so the block converges in a handful
+// of iterations and the cost is dominated by the H*X products (k_matvec).
+// The Rayleigh-Ritz step is done in FP64 on the basis S = [X W P] without
+// explicit orthogonalisation: G = S^H S is diagonalised (SVQB, small
+// directions dropped), the projected problem T^H (S^H H S) T is solved with a
+// GPU Jacobi eigensolver, and the Ritz block is formed with tall-skinny GEMMs.
+// Mixed precision: the products run on the FP32 store; a final FP64
+// refinement applies H in FP64 arithmetic to the converged block, redoes the
+// Rayleigh-Ritz step and reports FP64 residuals.
+#include "wtb_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+
+namespace wtb {
+
+float elapsed_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+namespace {
+
+__device__ __forceinline__ cd cmul(cd a, cd b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ cd cconj(cd a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ cd csub(cd a, cd b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ cd cscale(double s, cd a) { return make_double2(s * a.x, s * a.y); }
+
+__device__ __forceinline__ void rr_pair(int m, int t, int i, int &p, int &q) {
+    int mm = m - 1, x, y;
+    if (i == 0) { x = t % mm; y = mm; }
+    else { x = (t + i) % mm; y = (t - i + mm) % mm; }
+    p = min(x, y);
+    q = max(x, y);
+}
+
+constexpr int kJT = 1024;
+
+__device__ double block_sum(double v, double *red) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) red[wave] = v;
+    __syncthreads();
+    double s = 0.0;
+    const int nw = (blockDim.x + 63) / 64;
+    for (int w = 0; w < nw; ++w) s += red[w];
+    return s;
+}
+
+// Parallel cyclic Jacobi on one workgroup.  A (m x m, column-major, lda) is
+// Hermitian and is diagonalised in place; V (m x m, ldv) accumulates the
+// rotations.  Round-robin pairing: each step applies m/2 disjoint rotations,
+// updating every 2x2 block (a, b) as J_a^H A_ab J_b in one pass.
+__global__ void __launch_bounds__(kJT)
+jacobi_kernel(int m, cd *__restrict__ A, int lda, cd *__restrict__ V, int ldv, double *__restrict__ W,
+              int *__restrict__ perm, int max_sweeps) {
+    __shared__ double rc[kJT], rs[kJT];
+    __shared__ cd rph[kJT];
+    __shared__ int rp[kJT], rq[kJT];
+    __shared__ double red[kJT / 64];
+    const int tid = threadIdx.x;
+    const int me = (m + 1) & ~1, np = me / 2;
+    for (int idx = tid; idx < m * m; idx += kJT) {
+        int r = idx % m, s = idx / m;
+        V[r + (size_t)s * ldv] = make_double2(r == s ? 1.0 : 0.0, 0.0);
+    }
+    double fro = 0.0;
+    for (int idx = tid; idx < m * m; idx += kJT) {
+        cd a = A[idx % m + (size_t)(idx / m) * lda];
+        fro += a.x * a.x + a.y * a.y;
+    }
+    fro = block_sum(fro, red);
+    for (int sweep = 0; sweep < max_sweeps; ++sweep) {
+        double off = 0.0;
+        for (int idx = tid; idx < m * m; idx += kJT) {
+            int r = idx % m, s = idx / m;
+            if (r != s) {
+                cd a = A[r + (size_t)s * lda];
+                off += a.x * a.x + a.y * a.y;
+            }
+        }
+        off = block_sum(off, red);
+        if (off <= 1e-28 * fro || off < 1e-280) break;
+        for (int t = 0; t < me - 1; ++t) {
+            for (int i = tid; i < np; i += kJT) {
+                int p, q;
+                rr_pair(me, t, i, p, q);
+                double c = 1.0, s = 0.0;
+                cd e = make_double2(1.0, 0.0);
+                if (q < m) {
+                    cd b = A[p + (size_t)q * lda];
+                    double ab = sqrt(b.x * b.x + b.y * b.y);
+                    if (ab > 0.0) {
+                        double theta = (A[q + (size_t)q * lda].x - A[p + (size_t)p * lda].x) / (2.0 * ab);
+                        double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
+                        c = 1.0 / sqrt(tt * tt + 1.0);
+                        s = tt * c;
+                        e = make_double2(b.x / ab, b.y / ab);
+                    }
+                }
+                rp[i] = p; rq[i] = q; rc[i] = c; rs[i] = s; rph[i] = e;
+            }
+            __syncthreads();
+            // 2x2 block updates of A (upper block triangle, mirrored)
+            const int nblk = np * (np + 1) / 2;
+            for (int task = tid; task < nblk; task += kJT) {
+                // map task -> (a, b) with a <= b
+                int b = (int)((sqrt(8.0 * task + 1.0) - 1.0) / 2.0);
+                while (b * (b + 1) / 2 > task) --b;
+                while ((b + 1) * (b + 2) / 2 <= task) ++b;
+                int a = task - b * (b + 1) / 2;
+                int pa = rp[a], qa = rq[a], pb = rp[b], qb = rq[b];
+                bool va = qa < m, vb = qb < m;
+                // gather (missing dummy index -> zero row/col)
+                cd x00 = A[pa + (size_t)pb * lda];
+                cd x10 = va ? A[qa + (size_t)pb * lda] : make_double2(0, 0);
+                cd x01 = vb ? A[pa + (size_t)qb * lda] : make_double2(0, 0);
+                cd x11 = (va && vb) ? A[qa + (size_t)qb * lda] : make_double2(0, 0);
+                // right multiply by J_b = [[c, s], [-s conj(e), c conj(e)]]
+                double cb = rc[b], sb = rs[b];
+                cd eb = cconj(rph[b]);
+                cd y00 = csub(cscale(cb, x00), cscale(sb, cmul(eb, x01)));
+                cd y01 = cadd(cscale(sb, x00), cscale(cb, cmul(eb, x01)));
+                cd y10 = csub(cscale(cb, x10), cscale(sb, cmul(eb, x11)));
+                cd y11 = cadd(cscale(sb, x10), cscale(cb, cmul(eb, x11)));
+                // left multiply by J_a^H = [[c, -s e], [s, c e]]
+                double ca = rc[a], sa = rs[a];
+                cd ea = rph[a];
+                cd z00 = csub(cscale(ca, y00), cscale(sa, cmul(ea, y10)));
+                cd z01 = csub(cscale(ca, y01), cscale(sa, cmul(ea, y11)));
+                cd z10 = cadd(cscale(sa, y00), cscale(ca, cmul(ea, y10)));
+                cd z11 = cadd(cscale(sa, y01), cscale(ca, cmul(ea, y11)));
+                if (a == b) {
+                    z00.y = 0.0; z11.y = 0.0;
+                    z01 = make_double2(0, 0); z10 = make_double2(0, 0);
+                    A[pa + (size_t)pa * lda] = z00;
+                    if (va) {
+                        A[qa + (size_t)qa * lda] = z11;
+                        A[pa + (size_t)qa * lda] = z01;
+                        A[qa + (size_t)pa * lda] = z10;
+                    }
+                } else {
+                    A[pa + (size_t)pb * lda] = z00;
+                    A[pb + (size_t)pa * lda] = cconj(z00);
+                    if (va) { A[qa + (size_t)pb * lda] = z10; A[pb + (size_t)qa * lda] = cconj(z10); }
+                    if (vb) { A[pa + (size_t)qb * lda] = z01; A[qb + (size_t)pa * lda] = cconj(z01); }
+                    if (va && vb) { A[qa + (size_t)qb * lda] = z11; A[qb + (size_t)qa * lda] = cconj(z11); }
+                }
+            }
+            // V <- V J (column pairs)
+            for (int task = tid; task < m * np; task += kJT) {
+                int r = task % m, a = task / m;
+                int p = rp[a], q = rq[a];
+                if (q >= m) continue;
+                double c = rc[a], s = rs[a];
+                cd ec = cconj(rph[a]);
+                cd x = V[r + (size_t)p * ldv], y = V[r + (size_t)q * ldv];
+                cd ey = cmul(ec, y);
+                V[r + (size_t)p * ldv] = csub(cscale(c, x), cscale(s, ey));
+                V[r + (size_t)q * ldv] = cadd(cscale(s, x), cscale(c, ey));
+            }
+            __syncthreads();
+        }
+    }
+    // ascending order
+    for (int j = tid; j < m; j += kJT) {
+        double w = A[j + (size_t)j * lda].x;
+        int rank = 0;
+        for (int i = 0; i < m; ++i) {
+            double u = A[i + (size_t)i * lda].x;
+            rank += (u < w) || (u == w && i < j);
+        }
+        perm[rank] = j;
+        W[rank] = w;
+    }
+}
+
+__global__ void gather_cols(int m, int ncol, const int *__restrict__ perm, const cd *__restrict__ V, int ldv,
+                            cd *__restrict__ Z, int ldz) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    if (r < m && j < ncol) Z[r + (size_t)j * ldz] = V[r + (size_t)perm[j] * ldv];
+}
+
+__global__ void hermitize(int m, cd *A, int lda) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    int s = blockIdx.y;
+    if (r >= m || s >= m || r > s) return;
+    cd a = A[r + (size_t)s * lda], b = A[s + (size_t)r * lda];
+    cd h = make_double2(0.5 * (a.x + b.x), 0.5 * (a.y - b.y));
+    if (r == s) h.y = 0.0;
+    A[r + (size_t)s * lda] = h;
+    A[s + (size_t)r * lda] = cconj(h);
+}
+
+__global__ void set_units(long N, int b, const int *__restrict__ idx, cd *__restrict__ X) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    if (i >= N) return;
+    X[i + (size_t)j * N] = make_double2(i == idx[j] ? 1.0 : 0.0, 0.0);
+}
+
+// norm2[j] += sum_i |AX_ij - lam_j X_ij|^2   (lam may be null: plain column norms of AX)
+__global__ void resid_norms(long N, const cd *__restrict__ X, const cd *__restrict__ AX,
+                            const double *__restrict__ lam, double *__restrict__ norm2) {
+    __shared__ double red[4];
+    const int j = blockIdx.y;
+    double s = 0.0;
+    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
+        cd a = AX[i + (size_t)j * N];
+        if (lam) {
+            cd x = X[i + (size_t)j * N];
+            a.x -= lam[j] * x.x;
+            a.y -= lam[j] * x.y;
+        }
+        s += a.x * a.x + a.y * a.y;
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&norm2[j], red[0] + red[1] + red[2] + red[3]);
+}
+
+// W[:, a] = (AX[:, j] - lam_j X[:, j]) / (d - lam_j), j = act[a]; scaled by sc[a].
+__global__ void precondition(long N, const cd *__restrict__ X, const cd *__restrict__ AX,
+                             const double *__restrict__ lam, const double *__restrict__ diag,
+                             const int *__restrict__ act, const double *__restrict__ sc, cd *__restrict__ W) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int a = blockIdx.y;
+    if (i >= N) return;
+    int j = act[a];
+    double l = lam[j];
+    cd x = X[i + (size_t)j * N], ax = AX[i + (size_t)j * N];
+    double den = diag[i] - l;
+    const double floor_ = 1e-2;
+    if (fabs(den) < floor_) den = den < 0 ? -floor_ : floor_;
+    double f = sc[a] / den;
+    W[i + (size_t)a * N] = make_double2(f * (ax.x - l * x.x), f * (ax.y - l * x.y));
+}
+
+__global__ void gather_scale(long N, const int *__restrict__ act, const double *__restrict__ sc,
+                             const cd *__restrict__ P, const cd *__restrict__ AP, cd *__restrict__ Po,
+                             cd *__restrict__ APo) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int a = blockIdx.y;
+    if (i >= N) return;
+    int j = act[a];
+    double f = sc ? sc[a] : 1.0;
+    cd p = P[i + (size_t)j * N], ap = AP[i + (size_t)j * N];
+    Po[i + (size_t)a * N] = cscale(f, p);
+    APo[i + (size_t)a * N] = cscale(f, ap);
+}
+
+__global__ void scale_cols(long N, const double *__restrict__ sc, cd *__restrict__ W) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int a = blockIdx.y;
+    if (i >= N) return;
+    W[i + (size_t)a * N] = cscale(sc[a], W[i + (size_t)a * N]);
+}
+
+// T = U[:, off:off+r] diag(d^-1/2)
+__global__ void svqb_t(int m, int r, int off, const cd *__restrict__ U, int ldu, const double *__restrict__ d,
+                       cd *__restrict__ T, int ldt) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    if (i >= m || j >= r) return;
+    T[i + (size_t)j * ldt] = cscale(1.0 / sqrt(d[off + j]), U[i + (size_t)(off + j) * ldu]);
+}
+
+const rocblas_double_complex kOne = {1.0, 0.0};
+const rocblas_double_complex kZero = {0.0, 0.0};
+
+inline const rocblas_double_complex *rb(const cd *p) { return reinterpret_cast<const rocblas_double_complex *>(p); }
+inline rocblas_double_complex *rb(cd *p) { return reinterpret_cast<rocblas_double_complex *>(p); }
+
+// C (m x n) = A^H B, A: K x m, B: K x n (column-major, ld = K)
+void gram(Ctx &c, int m, int n, long K, const cd *A, int lda, const cd *B, int ldb, cd *C, int ldc) {
+    WTB_BLAS_CHECK(rocblas_zgemm(c.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, n,
+                                 (rocblas_int)K, &kOne, rb(A), lda, rb(B), ldb, &kZero, rb(C), ldc));
+}
+
+// C (M x n) = A (M x k) B (k x n)
+void gemm_nn(Ctx &c, long M, int n, int k, const cd *A, long lda, const cd *B, int ldb, cd *C, long ldc) {
+    WTB_BLAS_CHECK(rocblas_zgemm(c.blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, n, k,
+                                 &kOne, rb(A), (rocblas_int)lda, rb(B), ldb, &kZero, rb(C), (rocblas_int)ldc));
+}
+
+}  // namespace
+
+void small_heev(Ctx &c, int m, cd *A, int lda, double *W, cd *Z, int ldz) {
+    if (m <= 0) return;
+    if (m > 2 * kJT) throw Error{WTB_ERR_UNSUPPORTED, "dense eigenproblem larger than 2048"};
+    DBuf<cd> &V = c.ws_c;
+    DBuf<int> &perm = c.ws_i;
+    V.alloc((size_t)m * m);
+    perm.alloc(m);
+    hipLaunchKernelGGL(jacobi_kernel, dim3(1), dim3(kJT), 0, c.stream, m, A, lda, V.p, m, W, perm.p, 60);
+    WTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(gather_cols, dim3((m + 255) / 256, m), dim3(256), 0, c.stream, m, m, perm.p, V.p, m, Z, ldz);
+    WTB_LAUNCH_CHECK();
+}
+
+namespace {
+
+struct Timer {
+    hipStream_t st;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+    void begin() {
+        hipEvent_t a, b;
+        WTB_HIP_CHECK(hipEventCreate(&a));
+        WTB_HIP_CHECK(hipEventCreate(&b));
+        WTB_HIP_CHECK(hipEventRecord(a, st));
+        ev.push_back({a, b});
+    }
+    void end() { WTB_HIP_CHECK(hipEventRecord(ev.back().second, st)); }
+    double total() {
+        double t = 0;
+        for (auto &p : ev) t += elapsed_ms(p.first, p.second);
+        return t;
+    }
+    ~Timer() {
+        for (auto &p : ev) {
+            (void)hipEventDestroy(p.first);
+            (void)hipEventDestroy(p.second);
+        }
+    }
+};
+
+struct Lobpcg {
+    Ctx &c;
+    long N;
+    int b, nev, m3;
+    DBuf<cd> S, AS, T1, P, AP;      // N x cols
+    DBuf<cd> G, A, U, T, Tmp, Cm;   // small m3 x m3
+    DBuf<double> D, lam, lamr, norm2, sc;
+    DBuf<int> idx, act;
+    Timer tmv, trr;
+    std::vector<double> h_lam;
+
+    Lobpcg(Ctx &cx, int nev_, int b_) : c(cx), N(cx.bse.N), b(b_), nev(nev_), m3(3 * b_) {
+        tmv.st = c.stream;
+        trr.st = c.stream;
+        S.alloc((size_t)N * m3);
+        AS.alloc((size_t)N * m3);
+        T1.alloc((size_t)N * 2 * b);
+        P.alloc((size_t)N * b);
+        AP.alloc((size_t)N * b);
+        G.alloc((size_t)m3 * m3);
+        A.alloc((size_t)m3 * m3);
+        U.alloc((size_t)m3 * m3);
+        T.alloc((size_t)m3 * m3);
+        Tmp.alloc((size_t)m3 * m3);
+        Cm.alloc((size_t)m3 * m3);
+        D.alloc(m3);
+        lam.alloc(m3);
+        lamr.alloc(m3);
+        norm2.alloc(m3);
+        sc.alloc(m3);
+        idx.alloc(m3);
+        act.alloc(m3);
+    }
+
+    dim3 gN(int cols) const { return dim3((unsigned)((N + 255) / 256), (unsigned)cols); }
+
+    void apply(int precision, int ncol, const cd *X, cd *Y) {
+        tmv.begin();
+        matvec(c, precision, ncol, X, Y);
+        tmv.end();
+    }
+
+    // column norms^2 of (AX - lam X) or of AX when lam == nullptr
+    std::vector<double> norms(int ncol, const cd *X, const cd *AX, const double *lamd) {
+        WTB_HIP_CHECK(hipMemsetAsync(norm2.p, 0, sizeof(double) * ncol, c.stream));
+        unsigned gx = (unsigned)std::min<long>((N + 255) / 256, 1024);
+        hipLaunchKernelGGL(resid_norms, dim3(gx, ncol), dim3(256), 0, c.stream, N, X, AX, lamd, norm2.p);
+        WTB_LAUNCH_CHECK();
+        std::vector<double> h(ncol);
+        WTB_HIP_CHECK(hipMemcpyAsync(h.data(), norm2.p, sizeof(double) * ncol, hipMemcpyDeviceToHost, c.stream));
+        WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+        return h;
+    }
+
+    // Rayleigh-Ritz on S[:, :m] (first b columns become the new X, the
+    // remainder combination the new P when withP).
+    void rayleigh_ritz(int m, bool withP) {
+        trr.begin();
+        gram(c, m, m, N, S.p, (int)N, S.p, (int)N, G.p, m3);
+        gram(c, m, m, N, S.p, (int)N, AS.p, (int)N, A.p, m3);
+        hipLaunchKernelGGL(hermitize, dim3((m + 63) / 64, m), dim3(64), 0, c.stream, m, G.p, m3);
+        hipLaunchKernelGGL(hermitize, dim3((m + 63) / 64, m), dim3(64), 0, c.stream, m, A.p, m3);
+        WTB_LAUNCH_CHECK();
+        small_heev(c, m, G.p, m3, D.p, U.p, m3);
+        std::vector<double> hd(m);
+        WTB_HIP_CHECK(hipMemcpyAsync(hd.data(), D.p, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+        WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+        const double dmax = hd[m - 1];
+        int off = 0;
+        while (off < m && hd[off] <= 1e-12 * dmax) ++off;
+        const int r = m - off;
+        if (r < b) throw Error{WTB_ERR_NOCONV, "Rayleigh-Ritz basis lost rank"};
+        hipLaunchKernelGGL(svqb_t, dim3((m + 63) / 64, r), dim3(64), 0, c.stream, m, r, off, U.p, m3, D.p, T.p, m3);
+        WTB_LAUNCH_CHECK();
+        // A' = T^H A T  (r x r) into G
+        gemm_nn(c, m, r, m, A.p, m3, T.p, m3, Tmp.p, m3);
+        gram(c, r, r, m, T.p, m3, Tmp.p, m3, G.p, m3);
+        hipLaunchKernelGGL(hermitize, dim3((r + 63) / 64, r), dim3(64), 0, c.stream, r, G.p, m3);
+        WTB_LAUNCH_CHECK();
+        small_heev(c, r, G.p, m3, lamr.p, U.p, m3);
+        // C = T Y[:, :b]  (m x b)
+        gemm_nn(c, m, b, r, T.p, m3, U.p, m3, Cm.p, m3);
+        WTB_HIP_CHECK(hipMemcpyAsync(lam.p, lamr.p, sizeof(double) * b, hipMemcpyDeviceToDevice, c.stream));
+        // new X, AX (and P, AP)
+        gemm_nn(c, N, b, m, S.p, N, Cm.p, m3, T1.p, N);
+        if (withP && m > b) gemm_nn(c, N, b, m - b, S.p + (size_t)b * N, N, Cm.p + b, m3, T1.p + (size_t)b * N, N);
+        WTB_HIP_CHECK(hipMemcpyAsync(S.p, T1.p, sizeof(cd) * N * b, hipMemcpyDeviceToDevice, c.stream));
+        if (withP && m > b)
+            WTB_HIP_CHECK(hipMemcpyAsync(P.p, T1.p + (size_t)b * N, sizeof(cd) * N * b, hipMemcpyDeviceToDevice, c.stream));
+        gemm_nn(c, N, b, m, AS.p, N, Cm.p, m3, T1.p, N);
+        if (withP && m > b) gemm_nn(c, N, b, m - b, AS.p + (size_t)b * N, N, Cm.p + b, m3, T1.p + (size_t)b * N, N);
+        WTB_HIP_CHECK(hipMemcpyAsync(AS.p, T1.p, sizeof(cd) * N * b, hipMemcpyDeviceToDevice, c.stream));
+        if (withP && m > b)
+            WTB_HIP_CHECK(hipMemcpyAsync(AP.p, T1.p + (size_t)b * N, sizeof(cd) * N * b, hipMemcpyDeviceToDevice, c.stream));
+        trr.end();
+        h_lam.resize(b);
+        WTB_HIP_CHECK(hipMemcpyAsync(h_lam.data(), lam.p, sizeof(double) * b, hipMemcpyDeviceToHost, c.stream));
+        WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+    }
+};
+
+}  // namespace
+
+void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *rep) {
+    const long N = c.bse.N;
+    if (nev < 1 || nev > N) throw Error{WTB_ERR_ARG, "nev out of range"};
+    int b = (int)std::min<long>(N, nev + std::max(8, nev / 4));
+    if (N <= 1024 || 3L * b >= N) {
+        // small problem: full dense diagonalisation (the reference's mkl_solver
+        // path, module_bse.F90:2614-2642)
+        hipEvent_t a0, a1;
+        WTB_HIP_CHECK(hipEventCreate(&a0));
+        WTB_HIP_CHECK(hipEventCreate(&a1));
+        WTB_HIP_CHECK(hipEventRecord(a0, c.stream));
+        c.ws_dense.alloc((size_t)N * N);
+        DBuf<cd> Z;
+        DBuf<double> W;
+        Z.alloc((size_t)N * N);
+        W.alloc(N);
+        bse_block_to_c64(c, 0, N, 0, N, c.ws_dense.p, N);
+        small_heev(c, (int)N, c.ws_dense.p, (int)N, W.p, Z.p, (int)N);
+        Solution &s = c.sol;
+        s.evals.alloc(nev);
+        s.evecs.alloc((size_t)N * nev);
+        WTB_HIP_CHECK(hipMemcpyAsync(s.evals.p, W.p, sizeof(double) * nev, hipMemcpyDeviceToDevice, c.stream));
+        WTB_HIP_CHECK(hipMemcpyAsync(s.evecs.p, Z.p, sizeof(cd) * N * nev, hipMemcpyDeviceToDevice, c.stream));
+        WTB_HIP_CHECK(hipEventRecord(a1, c.stream));
+        WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+        s.nev = nev;
+        s.valid = true;
+        if (rep) {
+            *rep = wtb_report{};
+            rep->nconv = nev;
+            rep->t_solve_ms = elapsed_ms(a0, a1);
+            rep->t_rr_ms = rep->t_solve_ms;
+        }
+        (void)hipEventDestroy(a0);
+        (void)hipEventDestroy(a1);
+        return;
+    }
+    hipEvent_t t0, t1, tr;
+    WTB_HIP_CHECK(hipEventCreate(&t0));
+    WTB_HIP_CHECK(hipEventCreate(&t1));
+    WTB_HIP_CHECK(hipEventCreate(&tr));
+    WTB_HIP_CHECK(hipEventRecord(t0, c.stream));
+    Lobpcg L(c, nev, b);
+    // initial block: unit vectors on the b smallest diagonal entries
+    std::vector<double> d(N);
+    WTB_HIP_CHECK(hipMemcpyAsync(d.data(), c.bse.diag.p, sizeof(double) * N, hipMemcpyDeviceToHost, c.stream));
+    WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+    std::vector<int> order(N);
+    std::iota(order.begin(), order.end(), 0);
+    std::partial_sort(order.begin(), order.begin() + b, order.end(),
+                      [&](int x, int y) { return d[x] < d[y] || (d[x] == d[y] && x < y); });
+    WTB_HIP_CHECK(hipMemcpyAsync(L.idx.p, order.data(), sizeof(int) * b, hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(set_units, L.gN(b), dim3(256), 0, c.stream, N, b, L.idx.p, L.S.p);
+    WTB_LAUNCH_CHECK();
+    const int mv_prec = (precision == WTB_FP64) ? WTB_FP64 : WTB_FP32;
+    long mvcols = 0;
+    L.apply(mv_prec, b, L.S.p, L.AS.p);
+    mvcols += b;
+    L.rayleigh_ritz(b, false);
+    // FP32 products cannot resolve residuals much below ~1e-6 relative: the
+    // mixed iteration stops there and the FP64 refinement takes over.
+    const double tol_it = (precision == WTB_FP64) ? tol : std::max(tol, 2e-6);
+    bool haveP = false;
+    int it = 0, nconv = 0;
+    std::vector<double> rel(b);
+    std::vector<double> prev_best(b, 1e300);
+    int stall = 0;
+    for (it = 0; it < max_it; ++it) {
+        std::vector<double> n2 = L.norms(b, L.S.p, L.AS.p, L.lam.p);
+        nconv = 0;
+        std::vector<int> active;
+        for (int j = 0; j < b; ++j) {
+            rel[j] = std::sqrt(n2[j]) / std::max(std::fabs(L.h_lam[j]), 1e-300);
+            if (j < nev && rel[j] < tol_it) ++nconv;
+            if (rel[j] >= tol_it) active.push_back(j);
+        }
+        if (nconv == nev) break;
+        // stagnation guard for the reduced-precision phase
+        double worst = 0;
+        for (int j = 0; j < nev; ++j) worst = std::max(worst, rel[j]);
+        if (precision != WTB_FP64 && worst > 0.5 * prev_best[0]) {
+            if (++stall >= 3) break;
+        } else {
+            stall = 0;
+        }
+        prev_best[0] = std::min(prev_best[0], worst);
+        const int na = (int)active.size();
+        WTB_HIP_CHECK(hipMemcpyAsync(L.act.p, active.data(), sizeof(int) * na, hipMemcpyHostToDevice, c.stream));
+        // W = T R (unit scale first, normalised below)
+        std::vector<double> ones(na, 1.0);
+        WTB_HIP_CHECK(hipMemcpyAsync(L.sc.p, ones.data(), sizeof(double) * na, hipMemcpyHostToDevice, c.stream));
+        cd *W = L.S.p + (size_t)b * N;
+        cd *AW = L.AS.p + (size_t)b * N;
+        hipLaunchKernelGGL(precondition, L.gN(na), dim3(256), 0, c.stream, N, L.S.p, L.AS.p, L.lam.p,
+                           c.bse.diag.p, L.act.p, L.sc.p, W);
+        WTB_LAUNCH_CHECK();
+        std::vector<double> wn = L.norms(na, nullptr, W, nullptr);
+        std::vector<double> inv(na);
+        for (int a = 0; a < na; ++a) inv[a] = wn[a] > 0 ? 1.0 / std::sqrt(wn[a]) : 1.0;
+        WTB_HIP_CHECK(hipMemcpyAsync(L.sc.p, inv.data(), sizeof(double) * na, hipMemcpyHostToDevice, c.stream));
+        hipLaunchKernelGGL(scale_cols, L.gN(na), dim3(256), 0, c.stream, N, L.sc.p, W);
+        WTB_LAUNCH_CHECK();
+        L.apply(mv_prec, na, W, AW);
+        mvcols += na;
+        int m = b + na;
+        if (haveP) {
+            cd *Pd = L.S.p + (size_t)(b + na) * N;
+            cd *APd = L.AS.p + (size_t)(b + na) * N;
+            hipLaunchKernelGGL(gather_scale, L.gN(na), dim3(256), 0, c.stream, N, L.act.p, nullptr, L.P.p, L.AP.p,
+                               Pd, APd);
+            WTB_LAUNCH_CHECK();
+            std::vector<double> pn = L.norms(na, nullptr, Pd, nullptr);
+            for (int a = 0; a < na; ++a) inv[a] = pn[a] > 0 ? 1.0 / std::sqrt(pn[a]) : 1.0;
+            WTB_HIP_CHECK(hipMemcpyAsync(L.sc.p, inv.data(), sizeof(double) * na, hipMemcpyHostToDevice, c.stream));
+            hipLaunchKernelGGL(scale_cols, L.gN(na), dim3(256), 0, c.stream, N, L.sc.p, Pd);
+            hipLaunchKernelGGL(scale_cols, L.gN(na), dim3(256), 0, c.stream, N, L.sc.p, APd);
+            WTB_LAUNCH_CHECK();
+            m += na;
+        }
+        L.rayleigh_ritz(m, true);
+        haveP = true;
+    }
+    WTB_HIP_CHECK(hipEventRecord(tr, c.stream));
+    double max_rel = 0;
+    if (precision != WTB_FP64) {
+        // FP64 residual refinement: H X in FP64 arithmetic, Rayleigh-Ritz on X
+        L.apply(WTB_FP64, b, L.S.p, L.AS.p);
+        mvcols += b;
+        L.rayleigh_ritz(b, false);
+    }
+    std::vector<double> n2 = L.norms(b, L.S.p, L.AS.p, L.lam.p);
+    nconv = 0;
+    for (int j = 0; j < nev; ++j) {
+        double r = std::sqrt(n2[j]) / std::max(std::fabs(L.h_lam[j]), 1e-300);
+        max_rel = std::max(max_rel, r);
+        if (r < std::max(tol, precision == WTB_FP64 ? tol : 1e-6)) ++nconv;
+    }
+    WTB_HIP_CHECK(hipEventRecord(t1, c.stream));
+    // results
+    Solution &s = c.sol;
+    s.evals.alloc(nev);
+    s.evecs.alloc((size_t)N * nev);
+    WTB_HIP_CHECK(hipMemcpyAsync(s.evals.p, L.lam.p, sizeof(double) * nev, hipMemcpyDeviceToDevice, c.stream));
+    WTB_HIP_CHECK(hipMemcpyAsync(s.evecs.p, L.S.p, sizeof(cd) * N * nev, hipMemcpyDeviceToDevice, c.stream));
+    WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+    s.nev = nev;
+    s.valid = true;
+    if (rep) {
+        rep->iterations = it;
+        rep->nconv = nconv;
+        rep->matvec_columns = mvcols;
+        rep->max_rel_residual = max_rel;
+        rep->t_solve_ms = elapsed_ms(t0, t1);
+        rep->t_refine_ms = precision != WTB_FP64 ? elapsed_ms(tr, t1) : 0.0;
+        rep->t_matvec_ms = L.tmv.total();
+        rep->t_rr_ms = L.trr.total();
+    }
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    (void)hipEventDestroy(tr);
+}
+
+}  // namespace wtb

@@ -1,0 +1,153 @@
+// Internal declarations of the gfx950 exciton library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+
+#include <cstdio>
+#include <string>
+#include <vector>
+
+#include "../../include/wtb_hip.h"
+
+namespace wtb {
+
+using cd = double2;   // complex FP64 (re, im)
+using cf = float2;    // complex FP32
+
+constexpr int kMaxOrb = 32;   // H(k) eigensolver limit (one wavefront per k)
+constexpr int kMaxNcv = 16;   // nc*nv limit of the assembly kernel
+
+// Status with message, raised through the C-ABI.
+struct Error {
+    int code;
+    std::string msg;
+};
+
+#define WTB_HIP_CHECK(expr)                                                              \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess)                                                            \
+            throw ::wtb::Error{WTB_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)}; \
+    } while (0)
+
+#define WTB_BLAS_CHECK(expr)                                                             \
+    do {                                                                                 \
+        rocblas_status s_ = (expr);                                                      \
+        if (s_ != rocblas_status_success)                                                \
+            throw ::wtb::Error{WTB_ERR_HIP, std::string(#expr) + ": " +                  \
+                                                rocblas_status_to_string(s_)};           \
+    } while (0)
+
+#define WTB_LAUNCH_CHECK() WTB_HIP_CHECK(hipGetLastError())
+
+// Device buffer owned by the context.
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        if (count <= n && p) return;
+        free();
+        if (count == 0) return;
+        hipError_t e = hipMalloc(&p, count * sizeof(T));
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error{WTB_ERR_NOMEM, "hipMalloc of " + std::to_string(count * sizeof(T)) +
+                                           " bytes failed: " + hipGetErrorString(e)};
+        }
+        n = count;
+    }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { free(); }
+};
+
+struct Model {
+    int norb = 0, nvec = 0;
+    double rlat[9] = {0};        // rows a_i
+    double efermi = 0, scissor = 0;
+    int flags = 0;
+    DBuf<cd> hop;                // [r][m][n]
+    DBuf<double> rcart;          // [r][3] Cartesian R, reference summation order
+    DBuf<double> invdeg;         // [r] 1/deg
+    DBuf<int> onsite;            // [r] 1 if R == 0
+};
+
+struct Bands {
+    int nk = 0, nv = 0, nc = 0, nb = 0;
+    DBuf<double> eall;   // [k][n] all bands, scissor applied
+    DBuf<cd> uall;       // [k][o][n] component o of band n
+    DBuf<int> nocp;      // [k]
+    DBuf<double> esel;   // [k][j] selected bands
+    DBuf<cd> usel;       // [(o*nb + j)*nk + k]  (k fastest, as the reference's vector(k,j,o))
+    DBuf<cf> usel32;     // FP32 copy for the mixed-precision assembly
+    DBuf<int> status;    // device error flag
+    bool valid = false;
+};
+
+struct Bse {
+    long N = 0;
+    int precision = -1;
+    int convention = 0;
+    wtb_coulomb coul{};
+    DBuf<cd> h64;        // N x N column-major (FP64 store)
+    DBuf<cf> h32;        // N x N column-major (FP32 store)
+    DBuf<double> diag;   // E_c - E_v
+    bool valid = false;
+};
+
+struct Solution {
+    int nev = 0;
+    DBuf<double> evals;  // nev
+    DBuf<cd> evecs;      // N x nev column-major
+    bool valid = false;
+};
+
+struct Ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    rocblas_handle blas = nullptr;
+    std::string err;
+    Model model;
+    int nk = 0;
+    int ngrid[3] = {1, 1, 1};
+    DBuf<double> kpt;    // [k][3]
+    Bands bands;
+    Bse bse;
+    Solution sol;
+    DBuf<cd> dip;        // [a][N], a = x, y, z, sp, sm
+    bool dip_valid = false;
+    // solver workspace
+    DBuf<cd> ws_c;       // Jacobi rotation accumulator
+    DBuf<int> ws_i;      // Jacobi eigenvalue permutation
+    DBuf<cd> ws_dense;   // dense FP64 copy for small problems
+};
+
+// ---- kernels (launch wrappers) ----
+void launch_hk_eig(Ctx &c, int nv, int nc);
+void launch_bse_build(Ctx &c, const wtb_coulomb &coul, int precision, int convention);
+void launch_optics(Ctx &c, double cshift);
+void launch_oscillator(Ctx &c, int nev, int a, int b, double *act_dev);
+
+// Y = H X, X and Y column-major (ldx = ldy = N), complex FP64 host-visible
+// device buffers.  precision selects the kernel.
+void matvec(Ctx &c, int precision, int ncol, const cd *X, cd *Y);
+
+// Hermitian eigensolver for small dense matrices on the GPU (FP64 Jacobi).
+// A (m x m column-major, lda) is destroyed; W receives ascending eigenvalues,
+// Z (ldz) the eigenvectors.  Work arrays are allocated from the context.
+void small_heev(Ctx &c, int m, cd *A, int lda, double *W, cd *Z, int ldz);
+
+// Copy H(i0:i0+ni, j0:j0+nj) of the stored matrix into out (ni x nj, ldo) as FP64.
+void bse_block_to_c64(Ctx &c, long i0, long ni, long j0, long nj, cd *out, long ldo);
+
+// Block eigensolver driver (LOBPCG with soft locking, diagonal preconditioner).
+void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *rep);
+
+float elapsed_ms(hipEvent_t a, hipEvent_t b);
+
+}  // namespace wtb
