@@ -191,9 +191,16 @@ bse_build_kernel(int nk, int norb, int textbook, CoulombDev coul,
             const int p = kpl * kTK + kl;
             C val;
             if (I == J) {
-                const int k = k0 + kl;
-                double de = esel[(size_t)k * NB + NV + ci] - esel[(size_t)k * NB + vi];
-                val = mk<C>(de + (double)pv[p], 0.0);
+                // The FP32 store keeps a zero diagonal: E_c - E_v + V(0) lives
+                // in the FP64 diagonal array and is applied in FP64 by the
+                // matvec, so FP32 rounding only touches the Coulomb part.
+                if (sizeof(R) == sizeof(float)) {
+                    val = mk<C>(0.0, 0.0);
+                } else {
+                    const int k = k0 + kl;
+                    double de = esel[(size_t)k * NB + NV + ci] - esel[(size_t)k * NB + vi];
+                    val = mk<C>(de + (double)pv[p], 0.0);
+                }
             } else {
                 const C a = pmc[(size_t)p * NC2 + ci * NC + cj];
                 C b = pmv[(size_t)p * NV2 + vi * NV + vj];

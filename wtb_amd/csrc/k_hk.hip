@@ -80,7 +80,7 @@ hk_eig_kernel(int nk, int n, int nvec, int nv, int nc, int flags, double efermi,
     for (int r0 = 0; r0 < nvec; r0 += kLanes) {
         int r = r0 + tid;
         if (r < nvec) {
-            double x = (kx * rcart[3 * r] + ky * rcart[3 * r + 1]) + kz * rcart[3 * r + 2];
+            double x = kdotr(kx, ky, kz, rcart + 3 * r);
             if (compat) x = (double)(float)x;
             double sn, cs;
             sincos(x, &sn, &cs);

@@ -9,8 +9,8 @@
 // its 16 x 16 block of H straight from HBM into the MFMA A operand.  Complex
 // products use four real MFMAs:
 //   Yr += Hr Xr - Hi Xi,   Yi += Hr Xi + Hi Xr.
-// MFMA 16x16x4 operand layout (lane l of 64):
-//   A[l % 16][l / 16], B[l / 16][l % 16], D[4 (l / 16) + i][l % 16], i = 0..3.
+// MFMA 16x16x4 operand layout (lane l of 64; D differs between f32 and f64):
+//   A[l % 16][l / 16], B[l / 16][l % 16]; D f32: [4 (l / 16) + i][l % 16], D f64: [4 i + l / 16][l % 16].
 // H is read exactly once per launch, so the kernel is MFMA-bound for
 // ncol >~ 20 (FP64) and the bytes/flop ratio is 2/ncol (FP64).
 #include "wtb_internal.h"
@@ -29,6 +29,8 @@ struct Mfma<double> {
     static __device__ __forceinline__ V fma(double a, double b, V c) {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    // f64 16x16x4 output: register i of lane l holds D[4 i + l/16][l % 16]
+    static __device__ __forceinline__ int drow(int lane, int i) { return 4 * i + (lane >> 4); }
 };
 template <>
 struct Mfma<float> {
@@ -36,6 +38,8 @@ struct Mfma<float> {
     static __device__ __forceinline__ V fma(float a, float b, V c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    // f32 16x16x4 output: register i of lane l holds D[4 (l/16) + i][l % 16]
+    static __device__ __forceinline__ int drow(int lane, int i) { return 4 * (lane >> 4) + i; }
 };
 
 constexpr int kRows = 64;
@@ -109,7 +113,7 @@ zgemm_hx_kernel(long N, int ncol, const TH *__restrict__ H, const TX *__restrict
         if (col >= ncol) continue;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const long r = row0 + 4 * (lane >> 4) + i;
+            const long r = row0 + M::drow(lane, i);
             if (r < N) {
                 Y[2 * (r + (long)col * N)] = (TX)acc_r[cb][i];
                 Y[2 * (r + (long)col * N) + 1] = (TX)acc_i[cb][i];
@@ -137,6 +141,17 @@ void launch_hx(hipStream_t st, long N, int ncol, const TH *H, const TX *X, TX *Y
     }
 }
 
+// Y += diag(d) X in FP64: the diagonal of the FP32 store is kept exactly.
+__global__ void add_diag_kernel(long N, int ncol, const double *__restrict__ d, const double *__restrict__ X,
+                                double *__restrict__ Y) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (i >= N || c >= ncol) return;
+    size_t o = 2 * (i + (size_t)c * N);
+    Y[o] += d[i] * X[o];
+    Y[o + 1] += d[i] * X[o + 1];
+}
+
 }  // namespace
 
 // Y = H X with the arithmetic of `precision`:
@@ -153,10 +168,17 @@ void matvec(Ctx &c, int precision, int ncol, const cd *X, cd *Y) {
     double *y = (double *)Y;
     if (s.precision == WTB_FP64) {
         launch_hx<double, double, double>(c.stream, N, ncol, (const double *)s.h64.p, x, y);
-    } else if (precision == WTB_FP64) {
-        launch_hx<float, double, double>(c.stream, N, ncol, (const float *)s.h32.p, x, y);
     } else {
-        launch_hx<float, double, float>(c.stream, N, ncol, (const float *)s.h32.p, x, y);
+        if (precision == WTB_FP64)
+            launch_hx<float, double, double>(c.stream, N, ncol, (const float *)s.h32.p, x, y);
+        else
+            launch_hx<float, double, float>(c.stream, N, ncol, (const float *)s.h32.p, x, y);
+        for (int c0 = 0; c0 < ncol; c0 += 65535) {
+            int nc = std::min(65535, ncol - c0);
+            hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)((N + 255) / 256), nc), dim3(256), 0, c.stream, N, nc,
+                               s.diag.p, x + 2 * (size_t)c0 * N, y + 2 * (size_t)c0 * N);
+            WTB_LAUNCH_CHECK();
+        }
     }
 }
 

@@ -30,7 +30,7 @@ dipole_kernel(int nk, int n, int nvec, int nv, int nc, int compat, double eta, d
     cd *ph = U + nb * n;                        // nvec
     const double kx = kpt[3 * k], ky = kpt[3 * k + 1], kz = kpt[3 * k + 2];
     for (int r = tid; r < nvec; r += kL) {
-        double x = (kx * rcart[3 * r] + ky * rcart[3 * r + 1]) + kz * rcart[3 * r + 2];
+        double x = kdotr(kx, ky, kz, rcart + 3 * r);   // same value as the H(k) build
         if (compat) x = (double)(float)x;
         double sn, cs;
         sincos(x, &sn, &cs);

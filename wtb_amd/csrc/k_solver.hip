@@ -17,6 +17,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <numeric>
 
 namespace wtb {
@@ -284,15 +286,62 @@ inline const rocblas_double_complex *rb(const cd *p) { return reinterpret_cast<c
 inline rocblas_double_complex *rb(cd *p) { return reinterpret_cast<rocblas_double_complex *>(p); }
 
 // C (m x n) = A^H B, A: K x m, B: K x n (column-major, ld = K)
+// sum of nsplit partial m x n products (ld m, stride m n) into C (ldc)
+__global__ void reduce_splitk(int m, int n, int nsplit, const cd *__restrict__ part, cd *__restrict__ C, int ldc) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y;
+    if (i >= m || j >= n) return;
+    double re = 0.0, im = 0.0;
+    for (int s = 0; s < nsplit; ++s) {
+        cd v = part[(size_t)s * m * n + i + (size_t)j * m];
+        re += v.x;
+        im += v.y;
+    }
+    C[i + (size_t)j * ldc] = make_double2(re, im);
+}
+
+// C (m x n) = A^H B for tall A (K x m) and B (K x n): K is split into chunks
+// computed as one strided-batched GEMM (a plain GEMM here has only
+// (m/64)(n/64) output tiles and leaves most CUs idle), then reduced in FP64.
 void gram(Ctx &c, int m, int n, long K, const cd *A, int lda, const cd *B, int ldb, cd *C, int ldc) {
-    WTB_BLAS_CHECK(rocblas_zgemm(c.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, n,
-                                 (rocblas_int)K, &kOne, rb(A), lda, rb(B), ldb, &kZero, rb(C), ldc));
+    const long kc_min = 512;
+    int nsplit = (int)std::min<long>(64, std::max<long>(1, K / kc_min));
+    if (nsplit <= 1 || (long)m * n > 512L * 512L) {
+        WTB_BLAS_CHECK(rocblas_zgemm(c.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, n,
+                                     (rocblas_int)K, &kOne, rb(A), lda, rb(B), ldb, &kZero, rb(C), ldc));
+        return;
+    }
+    const long kc = (K + nsplit - 1) / nsplit;
+    nsplit = (int)((K + kc - 1) / kc);
+    const int full = (int)(K / kc);           // chunks of exactly kc rows
+    c.ws_gram.alloc((size_t)nsplit * m * n);
+    WTB_BLAS_CHECK(rocblas_zgemm_strided_batched(
+        c.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, n, (rocblas_int)kc, &kOne, rb(A),
+        lda, (rocblas_stride)kc, rb(B), ldb, (rocblas_stride)kc, &kZero, rb(c.ws_gram.p), m,
+        (rocblas_stride)((size_t)m * n), full));
+    if (full < nsplit) {   // the ragged last chunk
+        const long k0 = (long)full * kc;
+        WTB_BLAS_CHECK(rocblas_zgemm(c.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, n,
+                                     (rocblas_int)(K - k0), &kOne, rb(A + k0), lda, rb(B + k0), ldb, &kZero,
+                                     rb(c.ws_gram.p + (size_t)full * m * n), m));
+    }
+    hipLaunchKernelGGL(reduce_splitk, dim3((m + 63) / 64, n), dim3(64), 0, c.stream, m, n, nsplit, c.ws_gram.p, C,
+                       ldc);
+    WTB_LAUNCH_CHECK();
 }
 
 // C (M x n) = A (M x k) B (k x n)
 void gemm_nn(Ctx &c, long M, int n, int k, const cd *A, long lda, const cd *B, int ldb, cd *C, long ldc) {
     WTB_BLAS_CHECK(rocblas_zgemm(c.blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, n, k,
                                  &kOne, rb(A), (rocblas_int)lda, rb(B), ldb, &kZero, rb(C), (rocblas_int)ldc));
+}
+
+const rocblas_double_complex kMinusOne = {-1.0, 0.0};
+
+// C (M x n) -= A (M x k) B (k x n)
+void gemm_nn_sub(Ctx &c, long M, int n, int k, const cd *A, long lda, const cd *B, int ldb, cd *C, long ldc) {
+    WTB_BLAS_CHECK(rocblas_zgemm(c.blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, n, k,
+                                 &kMinusOne, rb(A), (rocblas_int)lda, rb(B), ldb, &kOne, rb(C), (rocblas_int)ldc));
 }
 
 }  // namespace
@@ -390,34 +439,96 @@ struct Lobpcg {
         return h;
     }
 
+    // B -= Q (Q^H B) for Q (N x q) with orthonormal columns, B (N x nb); the
+    // same combination is applied to B2 -= Q2 (Q^H B) when Q2 is given (keeps
+    // B2 = H B).  FP64 classical Gram-Schmidt.
+    void project_out(int q, const cd *Q, const cd *Q2, int nb, cd *B, cd *B2) {
+        trr.begin();
+        gram(c, q, nb, N, Q, (int)N, B, (int)N, Tmp.p, m3);
+        gemm_nn_sub(c, N, nb, q, Q, N, Tmp.p, m3, B, N);
+        if (Q2) gemm_nn_sub(c, N, nb, q, Q2, N, Tmp.p, m3, B2, N);
+        trr.end();
+    }
+
+    // scale columns of B (and B2 with the same factors) to unit norm
+    void normalize(int nb, cd *B, cd *B2) {
+        std::vector<double> n2 = norms(nb, nullptr, B, nullptr);
+        std::vector<double> inv(nb);
+        for (int a = 0; a < nb; ++a) inv[a] = n2[a] > 0 ? 1.0 / std::sqrt(n2[a]) : 1.0;
+        WTB_HIP_CHECK(hipMemcpyAsync(sc.p, inv.data(), sizeof(double) * nb, hipMemcpyHostToDevice, c.stream));
+        hipLaunchKernelGGL(scale_cols, gN(nb), dim3(256), 0, c.stream, N, sc.p, B);
+        if (B2) hipLaunchKernelGGL(scale_cols, gN(nb), dim3(256), 0, c.stream, N, sc.p, B2);
+        WTB_LAUNCH_CHECK();
+    }
+
+    double drop_rel = 1e-12;   // SVQB cut relative to the largest Gram eigenvalue
+
     // Rayleigh-Ritz on S[:, :m] (first b columns become the new X, the
     // remainder combination the new P when withP).
-    void rayleigh_ritz(int m, bool withP) {
+    // B (N x nb) <- B R^-1 with G = B^H B = R^H R (and B2 <- B2 R^-1), two
+    // passes (CholQR2).  False when a pivot falls below piv_rel: B is then
+    // numerically rank deficient and left half-processed.
+    double piv_rel = 1e-12;
+    bool cholqr(int nb, cd *B, cd *B2, double piv) {
+        for (int pass = 0; pass < 2; ++pass) {
+            trr.begin();
+            gram(c, nb, nb, N, B, (int)N, B, (int)N, Tmp.p, m3);
+            hipLaunchKernelGGL(hermitize, dim3((nb + 63) / 64, nb), dim3(64), 0, c.stream, nb, Tmp.p, m3);
+            WTB_LAUNCH_CHECK();
+            int fail = cholesky_upper(c, nb, Tmp.p, m3, piv);
+            if (fail) {
+                trr.end();
+                return false;
+            }
+            WTB_BLAS_CHECK(rocblas_ztrsm(c.blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                         rocblas_diagonal_non_unit, (rocblas_int)N, nb, &kOne, rb(Tmp.p), m3, rb(B),
+                                         (rocblas_int)N));
+            if (B2)
+                WTB_BLAS_CHECK(rocblas_ztrsm(c.blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                             rocblas_diagonal_non_unit, (rocblas_int)N, nb, &kOne, rb(Tmp.p), m3,
+                                             rb(B2), (rocblas_int)N));
+            trr.end();
+        }
+        return true;
+    }
+
+    // Rayleigh-Ritz on S[:, :m].  With an orthonormal basis the projected
+    // problem is a standard Hermitian eigenproblem; otherwise (orth == false)
+    // the Gram matrix is diagonalised first and nearly dependent directions
+    // are dropped (SVQB).  The first b columns become the new X, the
+    // remainder combination the new P when withP.
+    void rayleigh_ritz(int m, bool withP, bool orth = true) {
         trr.begin();
-        gram(c, m, m, N, S.p, (int)N, S.p, (int)N, G.p, m3);
         gram(c, m, m, N, S.p, (int)N, AS.p, (int)N, A.p, m3);
-        hipLaunchKernelGGL(hermitize, dim3((m + 63) / 64, m), dim3(64), 0, c.stream, m, G.p, m3);
         hipLaunchKernelGGL(hermitize, dim3((m + 63) / 64, m), dim3(64), 0, c.stream, m, A.p, m3);
         WTB_LAUNCH_CHECK();
-        small_heev(c, m, G.p, m3, D.p, U.p, m3);
-        std::vector<double> hd(m);
-        WTB_HIP_CHECK(hipMemcpyAsync(hd.data(), D.p, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
-        WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
-        const double dmax = hd[m - 1];
-        int off = 0;
-        while (off < m && hd[off] <= 1e-12 * dmax) ++off;
-        const int r = m - off;
-        if (r < b) throw Error{WTB_ERR_NOCONV, "Rayleigh-Ritz basis lost rank"};
-        hipLaunchKernelGGL(svqb_t, dim3((m + 63) / 64, r), dim3(64), 0, c.stream, m, r, off, U.p, m3, D.p, T.p, m3);
-        WTB_LAUNCH_CHECK();
-        // A' = T^H A T  (r x r) into G
-        gemm_nn(c, m, r, m, A.p, m3, T.p, m3, Tmp.p, m3);
-        gram(c, r, r, m, T.p, m3, Tmp.p, m3, G.p, m3);
-        hipLaunchKernelGGL(hermitize, dim3((r + 63) / 64, r), dim3(64), 0, c.stream, r, G.p, m3);
-        WTB_LAUNCH_CHECK();
-        small_heev(c, r, G.p, m3, lamr.p, U.p, m3);
-        // C = T Y[:, :b]  (m x b)
-        gemm_nn(c, m, b, r, T.p, m3, U.p, m3, Cm.p, m3);
+        if (orth) {
+            heev_lowest(c, m, b, A.p, m3, lamr.p, Cm.p, m3);
+        } else {
+            gram(c, m, m, N, S.p, (int)N, S.p, (int)N, G.p, m3);
+            hipLaunchKernelGGL(hermitize, dim3((m + 63) / 64, m), dim3(64), 0, c.stream, m, G.p, m3);
+            WTB_LAUNCH_CHECK();
+            small_heev(c, m, G.p, m3, D.p, U.p, m3);
+            std::vector<double> hd(m);
+            WTB_HIP_CHECK(hipMemcpyAsync(hd.data(), D.p, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
+            WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+            const double dmax = hd[m - 1];
+            int off = 0;
+            while (off < m && hd[off] <= drop_rel * dmax) ++off;
+            const int r = m - off;
+            if (r < b) throw Error{WTB_ERR_NOCONV, "Rayleigh-Ritz basis lost rank"};
+            hipLaunchKernelGGL(svqb_t, dim3((m + 63) / 64, r), dim3(64), 0, c.stream, m, r, off, U.p, m3, D.p, T.p,
+                               m3);
+            WTB_LAUNCH_CHECK();
+            // A' = T^H A T  (r x r) into G
+            gemm_nn(c, m, r, m, A.p, m3, T.p, m3, Tmp.p, m3);
+            gram(c, r, r, m, T.p, m3, Tmp.p, m3, G.p, m3);
+            hipLaunchKernelGGL(hermitize, dim3((r + 63) / 64, r), dim3(64), 0, c.stream, r, G.p, m3);
+            WTB_LAUNCH_CHECK();
+            heev_lowest(c, r, b, G.p, m3, lamr.p, U.p, m3);
+            // C = T Y[:, :b]  (m x b)
+            gemm_nn(c, m, b, r, T.p, m3, U.p, m3, Cm.p, m3);
+        }
         WTB_HIP_CHECK(hipMemcpyAsync(lam.p, lamr.p, sizeof(double) * b, hipMemcpyDeviceToDevice, c.stream));
         // new X, AX (and P, AP)
         gemm_nn(c, N, b, m, S.p, N, Cm.p, m3, T1.p, N);
@@ -443,6 +554,8 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     const long N = c.bse.N;
     if (nev < 1 || nev > N) throw Error{WTB_ERR_ARG, "nev out of range"};
     int b = (int)std::min<long>(N, nev + std::max(8, nev / 4));
+    const char *venv = std::getenv("WTB_VERBOSE");
+    const bool verbose = venv && venv[0] && venv[0] != '0';
     if (N <= 1024 || 3L * b >= N) {
         // small problem: full dense diagonalisation (the reference's mkl_solver
         // path, module_bse.F90:2614-2642)
@@ -456,7 +569,7 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
         Z.alloc((size_t)N * N);
         W.alloc(N);
         bse_block_to_c64(c, 0, N, 0, N, c.ws_dense.p, N);
-        small_heev(c, (int)N, c.ws_dense.p, (int)N, W.p, Z.p, (int)N);
+        heev_lowest(c, (int)N, nev, c.ws_dense.p, (int)N, W.p, Z.p, (int)N);
         Solution &s = c.sol;
         s.evals.alloc(nev);
         s.evecs.alloc((size_t)N * nev);
@@ -482,6 +595,13 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     WTB_HIP_CHECK(hipEventCreate(&tr));
     WTB_HIP_CHECK(hipEventRecord(t0, c.stream));
     Lobpcg L(c, nev, b);
+    // With FP32 products the projected matrix carries absolute errors of
+    // ~1e-7 |H|; directions whose Gram eigenvalue is below 1e-6 of the largest
+    // would amplify that noise beyond the spectral resolution, so they are
+    // dropped from the basis.
+    L.drop_rel = (precision == WTB_FP64) ? 1e-12 : 1e-4;
+    const double piv_p = (precision == WTB_FP64) ? 1e-12 : 1e-2;
+    const double p_keep = (precision == WTB_FP64) ? 1e-6 : 3e-2;
     // initial block: unit vectors on the b smallest diagonal entries
     std::vector<double> d(N);
     WTB_HIP_CHECK(hipMemcpyAsync(d.data(), c.bse.diag.p, sizeof(double) * N, hipMemcpyDeviceToHost, c.stream));
@@ -500,7 +620,7 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     L.rayleigh_ritz(b, false);
     // FP32 products cannot resolve residuals much below ~1e-6 relative: the
     // mixed iteration stops there and the FP64 refinement takes over.
-    const double tol_it = (precision == WTB_FP64) ? tol : std::max(tol, 2e-6);
+    const double tol_it = (precision == WTB_FP64) ? tol : std::max(tol, 1e-6);
     bool haveP = false;
     int it = 0, nconv = 0;
     std::vector<double> rel(b);
@@ -515,15 +635,21 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             if (j < nev && rel[j] < tol_it) ++nconv;
             if (rel[j] >= tol_it) active.push_back(j);
         }
-        if (nconv == nev) break;
-        // stagnation guard for the reduced-precision phase
         double worst = 0;
         for (int j = 0; j < nev; ++j) worst = std::max(worst, rel[j]);
-        if (precision != WTB_FP64 && worst > 0.5 * prev_best[0]) {
+        if (verbose)
+            std::fprintf(stderr, "[wtb lobpcg] it=%d m_active=%zu nconv=%d/%d max_rel_res=%.3e lam0=%.9f\n", it,
+                         active.size(), nconv, nev, worst, L.h_lam[0]);
+        if (nconv == nev) break;
+        // stagnation guard for the reduced-precision phase: near the FP32
+        // residual floor, stop when three iterations gain less than 10 %
+        if (precision != WTB_FP64 && worst < 1e-4 && worst > 0.9 * prev_best[0]) {
             if (++stall >= 3) break;
         } else {
             stall = 0;
         }
+        // restart without the P block when the residual jumps up tenfold
+        if (worst > 10.0 * prev_best[0]) haveP = false;
         prev_best[0] = std::min(prev_best[0], worst);
         const int na = (int)active.size();
         WTB_HIP_CHECK(hipMemcpyAsync(L.act.p, active.data(), sizeof(int) * na, hipMemcpyHostToDevice, c.stream));
@@ -535,30 +661,54 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
         hipLaunchKernelGGL(precondition, L.gN(na), dim3(256), 0, c.stream, N, L.S.p, L.AS.p, L.lam.p,
                            c.bse.diag.p, L.act.p, L.sc.p, W);
         WTB_LAUNCH_CHECK();
-        std::vector<double> wn = L.norms(na, nullptr, W, nullptr);
-        std::vector<double> inv(na);
-        for (int a = 0; a < na; ++a) inv[a] = wn[a] > 0 ? 1.0 / std::sqrt(wn[a]) : 1.0;
-        WTB_HIP_CHECK(hipMemcpyAsync(L.sc.p, inv.data(), sizeof(double) * na, hipMemcpyHostToDevice, c.stream));
-        hipLaunchKernelGGL(scale_cols, L.gN(na), dim3(256), 0, c.stream, N, L.sc.p, W);
-        WTB_LAUNCH_CHECK();
+        // W orthogonal to X (two classical Gram-Schmidt passes in FP64), unit
+        // columns, then its image H W
+        L.normalize(na, W, nullptr);
+        L.project_out(b, L.S.p, nullptr, na, W, nullptr);
+        L.project_out(b, L.S.p, nullptr, na, W, nullptr);
+        L.normalize(na, W, nullptr);
+        // orthonormal W (CholQR2); a rank-deficient W falls back to SVQB
+        // (H W is formed afterwards from the orthonormal W: no error amplification)
+        bool orth = L.cholqr(na, W, nullptr, 1e-12);
         L.apply(mv_prec, na, W, AW);
         mvcols += na;
         int m = b + na;
         if (haveP) {
+            // P orthogonal to X and W; H P follows the same combinations
             cd *Pd = L.S.p + (size_t)(b + na) * N;
             cd *APd = L.AS.p + (size_t)(b + na) * N;
             hipLaunchKernelGGL(gather_scale, L.gN(na), dim3(256), 0, c.stream, N, L.act.p, nullptr, L.P.p, L.AP.p,
                                Pd, APd);
             WTB_LAUNCH_CHECK();
+            L.normalize(na, Pd, APd);
+            for (int pass = 0; pass < 2; ++pass) {
+                L.project_out(b, L.S.p, L.AS.p, na, Pd, APd);
+                L.project_out(na, W, AW, na, Pd, APd);
+            }
+            // H P is carried by linear combinations, not recomputed: a column
+            // whose projection kept only a fraction f of its norm has its
+            // product error scaled by 1/f on renormalisation.  Columns below
+            // p_keep are dropped (the threshold follows the product precision).
             std::vector<double> pn = L.norms(na, nullptr, Pd, nullptr);
-            for (int a = 0; a < na; ++a) inv[a] = pn[a] > 0 ? 1.0 / std::sqrt(pn[a]) : 1.0;
-            WTB_HIP_CHECK(hipMemcpyAsync(L.sc.p, inv.data(), sizeof(double) * na, hipMemcpyHostToDevice, c.stream));
-            hipLaunchKernelGGL(scale_cols, L.gN(na), dim3(256), 0, c.stream, N, L.sc.p, Pd);
-            hipLaunchKernelGGL(scale_cols, L.gN(na), dim3(256), 0, c.stream, N, L.sc.p, APd);
-            WTB_LAUNCH_CHECK();
-            m += na;
+            int np = 0;
+            for (int a = 0; a < na; ++a) {
+                if (std::sqrt(pn[a]) < p_keep) continue;
+                if (np != a) {
+                    WTB_HIP_CHECK(hipMemcpyAsync(Pd + (size_t)np * N, Pd + (size_t)a * N, sizeof(cd) * N,
+                                                 hipMemcpyDeviceToDevice, c.stream));
+                    WTB_HIP_CHECK(hipMemcpyAsync(APd + (size_t)np * N, APd + (size_t)a * N, sizeof(cd) * N,
+                                                 hipMemcpyDeviceToDevice, c.stream));
+                }
+                ++np;
+            }
+            if (np > 0) {
+                L.normalize(np, Pd, APd);
+                // a P block that has become dependent on [X W] is skipped this step
+                // H P carries product errors: R^-1 may amplify them only mildly
+                if (!orth || L.cholqr(np, Pd, APd, piv_p)) m += np;
+            }
         }
-        L.rayleigh_ritz(m, true);
+        L.rayleigh_ritz(m, true, orth);
         haveP = true;
     }
     WTB_HIP_CHECK(hipEventRecord(tr, c.stream));
@@ -574,7 +724,9 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     for (int j = 0; j < nev; ++j) {
         double r = std::sqrt(n2[j]) / std::max(std::fabs(L.h_lam[j]), 1e-300);
         max_rel = std::max(max_rel, r);
-        if (r < std::max(tol, precision == WTB_FP64 ? tol : 1e-6)) ++nconv;
+        // mixed: the FP32 products bound the attainable residual; 1e-5 relative
+        // keeps |l - l_exact| <= |r| well inside 0.1 meV for these spectra
+        if (r < std::max(tol, precision == WTB_FP64 ? tol : 1e-5)) ++nconv;
     }
     WTB_HIP_CHECK(hipEventRecord(t1, c.stream));
     // results

@@ -15,6 +15,14 @@ struct wtb_ctx {
 
 namespace {
 
+// Every host<->device copy of the API runs on the context stream and waits
+// for it: the stream is non-blocking, so a null-stream hipMemcpy would not
+// order after the kernels queued on it.
+void copy(Ctx &c, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    WTB_HIP_CHECK(hipMemcpyAsync(dst, src, bytes, kind, c.stream));
+    WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+}
+
 template <class F>
 int guarded(wtb_ctx *ctx, F &&f) {
     if (!ctx) return WTB_ERR_ARG;
@@ -43,13 +51,16 @@ __global__ void block_to_c64_f64(long N, long i0, long ni, long j0, long nj, con
     if (i < ni && j < nj) out[i + j * ldo] = H[(i0 + i) + (j0 + j) * N];
 }
 
+// the FP32 store has a zero diagonal; the FP64 diagonal array completes it
 __global__ void block_to_c64_f32(long N, long i0, long ni, long j0, long nj, const cf *__restrict__ H,
-                                 cd *__restrict__ out, long ldo) {
+                                 const double *__restrict__ diag, cd *__restrict__ out, long ldo) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     long j = blockIdx.y;
     if (i < ni && j < nj) {
         cf h = H[(i0 + i) + (j0 + j) * N];
-        out[i + j * ldo] = make_double2(h.x, h.y);
+        double re = h.x;
+        if (i0 + i == j0 + j) re = diag[i0 + i];
+        out[i + j * ldo] = make_double2(re, h.y);
     }
 }
 
@@ -67,7 +78,8 @@ void bse_block_to_c64(Ctx &c, long i0, long ni, long j0, long nj, cd *out, long 
     if (c.bse.precision == WTB_FP64)
         hipLaunchKernelGGL(block_to_c64_f64, grid, dim3(256), 0, c.stream, N, i0, ni, j0, nj, c.bse.h64.p, out, ldo);
     else
-        hipLaunchKernelGGL(block_to_c64_f32, grid, dim3(256), 0, c.stream, N, i0, ni, j0, nj, c.bse.h32.p, out, ldo);
+        hipLaunchKernelGGL(block_to_c64_f32, grid, dim3(256), 0, c.stream, N, i0, ni, j0, nj, c.bse.h32.p,
+                           c.bse.diag.p, out, ldo);
     WTB_LAUNCH_CHECK();
 }
 }  // namespace wtb
@@ -141,10 +153,10 @@ int wtb_set_tb_model(wtb_ctx *ctx, int norb, int nvec, const int *rvec, const in
         m.rcart.alloc(rc.size());
         m.invdeg.alloc(nvec);
         m.onsite.alloc(nvec);
-        WTB_HIP_CHECK(hipMemcpy(m.hop.p, hop.data(), sizeof(cd) * hop.size(), hipMemcpyHostToDevice));
-        WTB_HIP_CHECK(hipMemcpy(m.rcart.p, rc.data(), sizeof(double) * rc.size(), hipMemcpyHostToDevice));
-        WTB_HIP_CHECK(hipMemcpy(m.invdeg.p, inv.data(), sizeof(double) * nvec, hipMemcpyHostToDevice));
-        WTB_HIP_CHECK(hipMemcpy(m.onsite.p, ons.data(), sizeof(int) * nvec, hipMemcpyHostToDevice));
+        copy(c, m.hop.p, hop.data(), sizeof(cd) * hop.size(), hipMemcpyHostToDevice);
+        copy(c, m.rcart.p, rc.data(), sizeof(double) * rc.size(), hipMemcpyHostToDevice);
+        copy(c, m.invdeg.p, inv.data(), sizeof(double) * nvec, hipMemcpyHostToDevice);
+        copy(c, m.onsite.p, ons.data(), sizeof(int) * nvec, hipMemcpyHostToDevice);
         m.norb = norb;
         m.nvec = nvec;
         m.efermi = efermi;
@@ -165,7 +177,7 @@ int wtb_set_kpoints(wtb_ctx *ctx, int nk, const double *kpt, const int *ngrid) {
         for (int i = 0; i < nk; ++i)
             for (int cc = 0; cc < 3; ++cc) k[3 * i + cc] = kpt[i + (size_t)nk * cc];
         c.kpt.alloc(k.size());
-        WTB_HIP_CHECK(hipMemcpy(c.kpt.p, k.data(), sizeof(double) * k.size(), hipMemcpyHostToDevice));
+        copy(c, c.kpt.p, k.data(), sizeof(double) * k.size(), hipMemcpyHostToDevice);
         c.nk = nk;
         for (int i = 0; i < 3; ++i) c.ngrid[i] = ngrid[i];
         c.bands.valid = false;
@@ -185,15 +197,15 @@ int wtb_hk_eig(wtb_ctx *ctx, int nv, int nc, double *eig, double *vec, int *nocp
         const int nk = b.nk, nb = b.nb, n = c.model.norb;
         if (eig) {
             std::vector<double> e((size_t)nk * nb);
-            WTB_HIP_CHECK(hipMemcpy(e.data(), b.esel.p, sizeof(double) * e.size(), hipMemcpyDeviceToHost));
+            copy(c, e.data(), b.esel.p, sizeof(double) * e.size(), hipMemcpyDeviceToHost);
             for (int k = 0; k < nk; ++k)
                 for (int j = 0; j < nb; ++j) eig[k + (size_t)nk * j] = e[(size_t)k * nb + j];
         }
         if (vec) {
             // device usel[(o*nb + j)*nk + k] is exactly Fortran vector(nk, nb, norb)
-            WTB_HIP_CHECK(hipMemcpy(vec, b.usel.p, sizeof(cd) * (size_t)nk * nb * n, hipMemcpyDeviceToHost));
+            copy(c, vec, b.usel.p, sizeof(cd) * (size_t)nk * nb * n, hipMemcpyDeviceToHost);
         }
-        if (nocp) WTB_HIP_CHECK(hipMemcpy(nocp, b.nocp.p, sizeof(int) * nk, hipMemcpyDeviceToHost));
+        if (nocp) copy(c, nocp, b.nocp.p, sizeof(int) * nk, hipMemcpyDeviceToHost);
     });
 }
 
@@ -204,13 +216,13 @@ int wtb_get_bands(wtb_ctx *ctx, double *energies, double *vectors) {
         const int nk = b.nk, n = c.model.norb;
         if (energies) {
             std::vector<double> e((size_t)nk * n);
-            WTB_HIP_CHECK(hipMemcpy(e.data(), b.eall.p, sizeof(double) * e.size(), hipMemcpyDeviceToHost));
+            copy(c, e.data(), b.eall.p, sizeof(double) * e.size(), hipMemcpyDeviceToHost);
             for (int k = 0; k < nk; ++k)
                 for (int j = 0; j < n; ++j) energies[k + (size_t)nk * j] = e[(size_t)k * n + j];
         }
         if (vectors) {
             std::vector<cd> u((size_t)nk * n * n);
-            WTB_HIP_CHECK(hipMemcpy(u.data(), b.uall.p, sizeof(cd) * u.size(), hipMemcpyDeviceToHost));
+            copy(c, u.data(), b.uall.p, sizeof(cd) * u.size(), hipMemcpyDeviceToHost);
             cd *out = reinterpret_cast<cd *>(vectors);
             for (int k = 0; k < nk; ++k)
                 for (int o = 0; o < n; ++o)
@@ -238,8 +250,8 @@ int wtb_bse_solve(wtb_ctx *ctx, int nev, double tol, int max_it, double *E, doub
         wtb_report r{};
         lobpcg(c, nev, tol, max_it, c.bse.precision, &r);
         if (rep) *rep = r;
-        if (E) WTB_HIP_CHECK(hipMemcpy(E, c.sol.evals.p, sizeof(double) * nev, hipMemcpyDeviceToHost));
-        if (X) WTB_HIP_CHECK(hipMemcpy(X, c.sol.evecs.p, sizeof(cd) * (size_t)c.bse.N * nev, hipMemcpyDeviceToHost));
+        if (E) copy(c, E, c.sol.evals.p, sizeof(double) * nev, hipMemcpyDeviceToHost);
+        if (X) copy(c, X, c.sol.evecs.p, sizeof(cd) * (size_t)c.bse.N * nev, hipMemcpyDeviceToHost);
         if (r.nconv < nev)
             throw Error{WTB_ERR_NOCONV, "eigensolver: " + std::to_string(r.nconv) + " of " + std::to_string(nev) +
                                             " pairs converged"};
@@ -251,7 +263,7 @@ int wtb_optics(wtb_ctx *ctx, double cshift, double *d) {
         launch_optics(c, cshift);
         if (d) {
             const long N = (long)c.bands.nk * c.bands.nc * c.bands.nv;
-            WTB_HIP_CHECK(hipMemcpy(d, c.dip.p, sizeof(cd) * 5 * (size_t)N, hipMemcpyDeviceToHost));
+            copy(c, d, c.dip.p, sizeof(cd) * 5 * (size_t)N, hipMemcpyDeviceToHost);
         }
     });
 }
@@ -308,7 +320,7 @@ int wtb_bse_diag(wtb_ctx *ctx, double *diag) {
     return guarded(ctx, [&](Ctx &c) {
         if (!c.bse.valid) throw Error{WTB_ERR_STATE, "no BSE matrix"};
         if (!diag) throw Error{WTB_ERR_ARG, "null output"};
-        WTB_HIP_CHECK(hipMemcpy(diag, c.bse.diag.p, sizeof(double) * c.bse.N, hipMemcpyDeviceToHost));
+        copy(c, diag, c.bse.diag.p, sizeof(double) * c.bse.N, hipMemcpyDeviceToHost);
     });
 }
 
@@ -328,7 +340,7 @@ int wtb_slepc_solver_compat(const double *HX, double *EX, double *HE, int NX, in
         const long N = NX;
         c.bse.h64.alloc((size_t)N * N);
         c.bse.diag.alloc(N);
-        WTB_HIP_CHECK(hipMemcpy(c.bse.h64.p, HX, sizeof(cd) * N * N, hipMemcpyHostToDevice));
+        copy(c, c.bse.h64.p, HX, sizeof(cd) * N * N, hipMemcpyHostToDevice);
         hipLaunchKernelGGL(upload_diag, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c.stream, N, c.bse.h64.p,
                            c.bse.diag.p);
         WTB_LAUNCH_CHECK();
@@ -339,8 +351,8 @@ int wtb_slepc_solver_compat(const double *HX, double *EX, double *HE, int NX, in
         lobpcg(c, NE, 1e-8, 200, WTB_FP64, &r);
         std::vector<double> e(NE);
         std::vector<cd> x((size_t)N * NE);
-        WTB_HIP_CHECK(hipMemcpy(e.data(), c.sol.evals.p, sizeof(double) * NE, hipMemcpyDeviceToHost));
-        WTB_HIP_CHECK(hipMemcpy(x.data(), c.sol.evecs.p, sizeof(cd) * x.size(), hipMemcpyDeviceToHost));
+        copy(c, e.data(), c.sol.evals.p, sizeof(double) * NE, hipMemcpyDeviceToHost);
+        copy(c, x.data(), c.sol.evecs.p, sizeof(cd) * x.size(), hipMemcpyDeviceToHost);
         // order by |E| as EPS_SMALLEST_MAGNITUDE
         std::vector<int> ord(NE);
         for (int i = 0; i < NE; ++i) ord[i] = i;

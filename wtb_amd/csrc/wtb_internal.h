@@ -125,7 +125,24 @@ struct Ctx {
     DBuf<cd> ws_c;       // Jacobi rotation accumulator
     DBuf<int> ws_i;      // Jacobi eigenvalue permutation
     DBuf<cd> ws_dense;   // dense FP64 copy for small problems
+    DBuf<cd> ws_heev_c;  // Householder scalars
+    DBuf<double> ws_heev_d;  // tridiagonal matrix, its eigenvectors, inverse-iteration work
+    DBuf<int> ws_flag;   // device status word of the small factorisations
+    DBuf<cd> ws_gram;    // split-K partial Gram products
 };
+
+// k.R = (kx Rx + ky Ry) + kz Rz, evaluated without FMA contraction in the
+// reference's order.  On a Monkhorst-Pack mesh k.R can land exactly on a
+// float32 rounding midpoint, so the sign of the double rounding error decides
+// the float32 phase of the compat mode: every evaluation must use the same
+// plain IEEE operations.
+__device__ __forceinline__ double kdotr(double kx, double ky, double kz, const double *R) {
+#pragma clang fp contract(off)
+    const double a = kx * R[0];
+    const double b = ky * R[1];
+    const double c = kz * R[2];
+    return (a + b) + c;
+}
 
 // ---- kernels (launch wrappers) ----
 void launch_hk_eig(Ctx &c, int nv, int nc);
@@ -141,6 +158,15 @@ void matvec(Ctx &c, int precision, int ncol, const cd *X, cd *Y);
 // A (m x m column-major, lda) is destroyed; W receives ascending eigenvalues,
 // Z (ldz) the eigenvectors.  Work arrays are allocated from the context.
 void small_heev(Ctx &c, int m, cd *A, int lda, double *W, cd *Z, int ldz);
+
+// Lowest nvec eigenpairs of a dense Hermitian matrix (n <= 1024) by
+// Householder tridiagonalisation, bisection and inverse iteration.  A is
+// destroyed; W ascending; Z (n x nvec, ldz).
+void heev_lowest(Ctx &c, int n, int nvec, cd *A, int lda, double *W, cd *Z, int ldz);
+
+// In-place upper Cholesky factor G = R^H R (n x n, ldg).  Returns 0, or
+// 1 + the first column whose pivot falls below rel * its original diagonal.
+int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel);
 
 // Copy H(i0:i0+ni, j0:j0+nj) of the stored matrix into out (ni x nj, ldo) as FP64.
 void bse_block_to_c64(Ctx &c, long i0, long ni, long j0, long nj, cd *out, long ldo);

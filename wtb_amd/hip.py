@@ -175,12 +175,15 @@ class Context:
             coulomb = coulomb_params(**coulomb)
         self._check(self.lib.wtb_bse_build(self.h, C.byref(coulomb), int(precision), int(convention)))
 
-    def bse_solve(self, nev, tol=1e-8, max_it=200, vectors=True):
+    def bse_solve(self, nev, tol=1e-8, max_it=200, vectors=True, check=True):
+        """Lowest nev pairs.  With check=False a WTB_ERR_NOCONV status is
+        reported through rep['nconv'] instead of raising."""
         E = np.zeros(nev)
         X = np.zeros((self.N, nev), np.complex128, order="F") if vectors else None
         rep = Report()
-        self._check(self.lib.wtb_bse_solve(self.h, nev, float(tol), int(max_it), _ptr(E), _ptr(X),
-                                           C.byref(rep)))
+        st = self.lib.wtb_bse_solve(self.h, nev, float(tol), int(max_it), _ptr(E), _ptr(X), C.byref(rep))
+        if not (st == -3 and not check):
+            self._check(st)
         return E, X, rep.as_dict()
 
     def optics(self, cshift):
