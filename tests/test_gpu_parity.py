@@ -119,7 +119,7 @@ def test_bse_matrix_cubic_v3dl():
     ctx.close()
 
 
-@pytest.mark.parametrize("precision", [hip.FP64, hip.MIXED])
+@pytest.mark.parametrize("precision", [hip.FP64, hip.MIXED, hip.FP32])
 @pytest.mark.parametrize("ncol", [1, 5, 16, 40, 130])
 def test_matvec(precision, ncol, rng):
     say("case matvec", precision, ncol)
@@ -129,11 +129,15 @@ def test_matvec(precision, ncol, rng):
     ctx.bse_build(COUL, precision)
     H = ctx.get_block(0, ctx.N, 0, ctx.N)
     X = rng.standard_normal((ctx.N, ncol)) + 1j * rng.standard_normal((ctx.N, ncol))
+    X[:, 0] *= 1e-3                       # columns of very different scale
     Y = ctx.matvec(X, precision)
     Yr = H @ X
-    rel = np.abs(Y - Yr).max() / np.abs(Yr).max()
-    assert rel < (1e-13 if precision == hip.FP64 else 1e-5)
-    if precision == hip.MIXED:
+    # per-column error relative to the column's norm; the split-FP16 product
+    # keeps ~22 bits per operand with FP32 accumulation
+    rel = (np.linalg.norm(Y - Yr, axis=0) / np.linalg.norm(Yr, axis=0)).max()
+    say("rel error", rel)
+    assert rel < (1e-13 if precision == hip.FP64 else 2e-6)
+    if precision != hip.FP64:
         Y64 = ctx.matvec(X, hip.FP64)     # FP64 arithmetic on the FP32 store
         assert np.abs(Y64 - Yr).max() / np.abs(Yr).max() < 1e-13
     ctx.close()

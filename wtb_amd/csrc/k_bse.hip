@@ -17,6 +17,7 @@
 #include "wtb_internal.h"
 
 #include <cmath>
+#include <cstring>
 
 namespace wtb {
 
@@ -90,8 +91,10 @@ template <class R, int NC, int NV, int S>
 __global__ void __launch_bounds__(kThreads)
 bse_build_kernel(int nk, int norb, int textbook, CoulombDev coul,
                  const double *__restrict__ kpt, const double *__restrict__ esel,
-                 const typename CT<R>::C *__restrict__ usel, typename CT<R>::C *__restrict__ H) {
+                 const typename CT<R>::C *__restrict__ usel, typename CT<R>::C *__restrict__ H,
+                 unsigned *__restrict__ hmax) {
     using C = typename CT<R>::C;
+    float vmax = 0.f;   // largest |Re|, |Im| written (scales the FP16 split of the matvec)
     constexpr int NB = NC + NV, NCV = NC * NV, NC2 = NC * NC, NV2 = NV * NV;
     constexpr int TKP = 4 * S;
     extern __shared__ __align__(16) unsigned char smem[];
@@ -210,7 +213,12 @@ bse_build_kernel(int nk, int norb, int textbook, CoulombDev coul,
                 val.y = vq * (a.x * b.y + a.y * b.x);
             }
             H[I + J * N] = val;
+            vmax = fmaxf(vmax, fmaxf(fabsf((float)val.x), fabsf((float)val.y)));
         }
+    }
+    if (hmax) {
+        for (int off = 32; off > 0; off >>= 1) vmax = fmaxf(vmax, __shfl_xor(vmax, off, 64));
+        if ((threadIdx.x & 63) == 0) atomicMax(hmax, __float_as_uint(vmax));
     }
 }
 
@@ -227,7 +235,8 @@ void launch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const typename CT<
     if (shmem > 160 * 1024) throw Error{WTB_ERR_UNSUPPORTED, "assembly tile does not fit in LDS"};
     dim3 grid((nk + kTK - 1) / kTK, (nk + TKP - 1) / TKP);
     hipLaunchKernelGGL((bse_build_kernel<R, NC, NV, S>), grid, dim3(kThreads), shmem, c.stream, nk, norb,
-                       textbook, cd_, c.kpt.p, c.bands.esel.p, usel, H);
+                       textbook, cd_, c.kpt.p, c.bands.esel.p, usel, H,
+                       sizeof(R) == sizeof(float) ? c.bse.hmax.p : nullptr);
     WTB_LAUNCH_CHECK();
 }
 
@@ -277,11 +286,24 @@ void launch_bse_build(Ctx &c, const wtb_coulomb &coul, int precision, int conven
     if (f64) s.h32.free(); else s.h64.free();
     if (f64) s.h64.alloc((size_t)N * N); else s.h32.alloc((size_t)N * N);
     s.diag.alloc(N);
+    s.hmax.alloc(1);
+    WTB_HIP_CHECK(hipMemsetAsync(s.hmax.p, 0, sizeof(unsigned), c.stream));
     if (f64) dispatch_tile<double>(c, cd_, convention == WTB_CONV_TEXTBOOK, b.usel.p, s.h64.p);
     else dispatch_tile<float>(c, cd_, convention == WTB_CONV_TEXTBOOK, b.usel32.p, s.h32.p);
     hipLaunchKernelGGL(bse_diag_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c.stream, nk, nc, nv,
                        cd_, b.esel.p, s.diag.p);
     WTB_LAUNCH_CHECK();
+    if (!f64) {
+        // power-of-two scale for the FP16 split: max |H| / hscale in [2^12, 2^13)
+        unsigned bits = 0;
+        WTB_HIP_CHECK(hipMemcpyAsync(&bits, s.hmax.p, sizeof(unsigned), hipMemcpyDeviceToHost, c.stream));
+        WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+        float hm;
+        std::memcpy(&hm, &bits, sizeof(float));
+        int e = 0;
+        if (hm > 0.f) std::frexp(hm, &e);
+        s.hscale = std::ldexp(1.0f, e - 13);
+    }
     s.N = N;
     s.precision = precision;
     s.convention = convention;

@@ -171,6 +171,8 @@ void matvec(Ctx &c, int precision, int ncol, const cd *X, cd *Y) {
     } else {
         if (precision == WTB_FP64)
             launch_hx<float, double, double>(c.stream, N, ncol, (const float *)s.h32.p, x, y);
+        else if (precision == WTB_MIXED)
+            matvec_f16split(c, ncol, X, Y);
         else
             launch_hx<float, double, float>(c.stream, N, ncol, (const float *)s.h32.p, x, y);
         for (int c0 = 0; c0 < ncol; c0 += 65535) {

@@ -613,7 +613,7 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     WTB_HIP_CHECK(hipMemcpyAsync(L.idx.p, order.data(), sizeof(int) * b, hipMemcpyHostToDevice, c.stream));
     hipLaunchKernelGGL(set_units, L.gN(b), dim3(256), 0, c.stream, N, b, L.idx.p, L.S.p);
     WTB_LAUNCH_CHECK();
-    const int mv_prec = (precision == WTB_FP64) ? WTB_FP64 : WTB_FP32;
+    const int mv_prec = precision;   // FP64 / split-FP16 (mixed) / FP32 products
     long mvcols = 0;
     L.apply(mv_prec, b, L.S.p, L.AS.p);
     mvcols += b;

@@ -97,6 +97,8 @@ struct Bse {
     DBuf<cd> h64;        // N x N column-major (FP64 store)
     DBuf<cf> h32;        // N x N column-major (FP32 store)
     DBuf<double> diag;   // E_c - E_v
+    DBuf<unsigned> hmax; // max |Re|, |Im| of the FP32 store (float bits)
+    float hscale = 1.f;  // power of two: max |H| / hscale ~ 2^13 for the FP16 split
     bool valid = false;
 };
 
@@ -129,6 +131,9 @@ struct Ctx {
     DBuf<double> ws_heev_d;  // tridiagonal matrix, its eigenvectors, inverse-iteration work
     DBuf<int> ws_flag;   // device status word of the small factorisations
     DBuf<cd> ws_gram;    // split-K partial Gram products
+    DBuf<_Float16> ws_split;   // FP16 split planes of X for the mixed matvec
+    DBuf<unsigned> ws_colmax;  // per-column max |x|
+    DBuf<float> ws_colscale;   // per-column split / output scales
 };
 
 // k.R = (kx Rx + ky Ry) + kz Rz, evaluated without FMA contraction in the
@@ -153,6 +158,9 @@ void launch_oscillator(Ctx &c, int nev, int a, int b, double *act_dev);
 // Y = H X, X and Y column-major (ldx = ldy = N), complex FP64 host-visible
 // device buffers.  precision selects the kernel.
 void matvec(Ctx &c, int precision, int ncol, const cd *X, cd *Y);
+
+// Y = H_offdiag X on the FP32 store with split-FP16 MFMA (mixed precision).
+void matvec_f16split(Ctx &c, int ncol, const cd *X, cd *Y);
 
 // Hermitian eigensolver for small dense matrices on the GPU (FP64 Jacobi).
 // A (m x m column-major, lda) is destroyed; W receives ascending eigenvalues,
