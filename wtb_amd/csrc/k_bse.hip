@@ -89,7 +89,7 @@ __device__ __forceinline__ C mk(double a, double b) { return C{(decltype(C{}.x))
 // S: k' pairs per thread (tile width TKP = 4*S).
 template <class R, int NC, int NV, int S>
 __global__ void __launch_bounds__(kThreads)
-bse_build_kernel(int nk, int norb, int textbook, CoulombDev coul,
+bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
                  const double *__restrict__ kpt, const double *__restrict__ esel,
                  const typename CT<R>::C *__restrict__ usel, typename CT<R>::C *__restrict__ H,
                  unsigned *__restrict__ hmax) {
@@ -185,7 +185,7 @@ bse_build_kernel(int nk, int norb, int textbook, CoulombDev coul,
     const long I0 = (long)k0 * NCV, J0 = (long)kp0 * NCV;
     for (int il = tid; il < ROWS; il += kThreads) {
         const long I = I0 + il;
-        if (I >= N) break;
+        if (I >= ld) break;     // rows N..ld-1 come out zero (k >= nk has zero V and overlaps)
         const int kl = il / NCV, ci = (il % NCV) / NV, vi = il % NV;
         for (int jl = 0; jl < COLS; ++jl) {
             const long J = J0 + jl;
@@ -212,7 +212,7 @@ bse_build_kernel(int nk, int norb, int textbook, CoulombDev coul,
                 val.x = vq * (a.x * b.x - a.y * b.y);
                 val.y = vq * (a.x * b.y + a.y * b.x);
             }
-            H[I + J * N] = val;
+            H[I + J * ld] = val;
             vmax = fmaxf(vmax, fmaxf(fabsf((float)val.x), fabsf((float)val.y)));
         }
     }
@@ -235,7 +235,7 @@ void launch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const typename CT<
     if (shmem > 160 * 1024) throw Error{WTB_ERR_UNSUPPORTED, "assembly tile does not fit in LDS"};
     dim3 grid((nk + kTK - 1) / kTK, (nk + TKP - 1) / TKP);
     hipLaunchKernelGGL((bse_build_kernel<R, NC, NV, S>), grid, dim3(kThreads), shmem, c.stream, nk, norb,
-                       textbook, cd_, c.kpt.p, c.bands.esel.p, usel, H,
+                       c.bse.ld, textbook, cd_, c.kpt.p, c.bands.esel.p, usel, H,
                        sizeof(R) == sizeof(float) ? c.bse.hmax.p : nullptr);
     WTB_LAUNCH_CHECK();
 }
@@ -284,7 +284,8 @@ void launch_bse_build(Ctx &c, const wtb_coulomb &coul, int precision, int conven
     // free the store of the other precision first: 288 GB is plenty, but
     // there is no reason to keep both.
     if (f64) s.h32.free(); else s.h64.free();
-    if (f64) s.h64.alloc((size_t)N * N); else s.h32.alloc((size_t)N * N);
+    s.ld = store_ld(N);
+    if (f64) s.h64.alloc((size_t)s.ld * N); else s.h32.alloc((size_t)s.ld * N);
     s.diag.alloc(N);
     s.hmax.alloc(1);
     WTB_HIP_CHECK(hipMemsetAsync(s.hmax.p, 0, sizeof(unsigned), c.stream));

@@ -2,7 +2,7 @@
 // the block eigensolver (replaces PETSc's MatMult inside EPSSolve,
 // src/module_slepc_solver.F90:74).
 //
-// H is N x N column-major complex (FP64 or FP32 store), X and Y are N x ncol
+// H is N x N column-major complex (FP64 or FP32 store, leading dimension ld), X and Y are N x ncol
 // column-major complex.  A 256-thread workgroup owns a 64-row slab of Y and up
 // to 128 columns; each wavefront owns 16 rows.  Per K-step of 16 the X slice
 // is staged in LDS as split real/imaginary planes and every wavefront streams
@@ -49,7 +49,7 @@ constexpr int kMaxCB = 8;   // 128 columns per launch
 // TH: storage type of H, TX: storage type of X and Y, TC: MFMA arithmetic type.
 template <class TH, class TX, class TC, int NCB>
 __global__ void __launch_bounds__(256)
-zgemm_hx_kernel(long N, int ncol, const TH *__restrict__ H, const TX *__restrict__ X, TX *__restrict__ Y) {
+zgemm_hx_kernel(long N, long ld, int ncol, const TH *__restrict__ H, const TX *__restrict__ X, TX *__restrict__ Y) {
     using M = Mfma<TC>;
     using V = typename M::V;
     constexpr int NC = NCB * 16;
@@ -88,7 +88,7 @@ zgemm_hx_kernel(long N, int ncol, const TH *__restrict__ H, const TX *__restrict
             const long kc = k0 + 4 * s + akk;
             TC ar = 0, ai = 0;
             if (rok && kc < N) {
-                const TH *hp = H + 2 * (arow + kc * N);
+                const TH *hp = H + 2 * (arow + kc * ld);
                 ar = (TC)hp[0];
                 ai = (TC)hp[1];
             }
@@ -123,7 +123,7 @@ zgemm_hx_kernel(long N, int ncol, const TH *__restrict__ H, const TX *__restrict
 }
 
 template <class TH, class TX, class TC>
-void launch_hx(hipStream_t st, long N, int ncol, const TH *H, const TX *X, TX *Y) {
+void launch_hx(hipStream_t st, long N, long ld, int ncol, const TH *H, const TX *X, TX *Y) {
     for (int c0 = 0; c0 < ncol; c0 += kMaxCB * 16) {
         int nc = std::min(ncol - c0, kMaxCB * 16);
         int ncb = (nc + 15) / 16;
@@ -132,7 +132,7 @@ void launch_hx(hipStream_t st, long N, int ncol, const TH *H, const TX *X, TX *Y
         TX *Yc = Y + 2 * (size_t)c0 * N;
         switch (ncb) {
 #define WTB_HX_CASE(K) \
-    case K: hipLaunchKernelGGL((zgemm_hx_kernel<TH, TX, TC, K>), grid, dim3(256), 0, st, N, nc, H, Xc, Yc); break;
+    case K: hipLaunchKernelGGL((zgemm_hx_kernel<TH, TX, TC, K>), grid, dim3(256), 0, st, N, ld, nc, H, Xc, Yc); break;
             WTB_HX_CASE(1) WTB_HX_CASE(2) WTB_HX_CASE(3) WTB_HX_CASE(4)
             WTB_HX_CASE(5) WTB_HX_CASE(6) WTB_HX_CASE(7) WTB_HX_CASE(8)
 #undef WTB_HX_CASE
@@ -167,14 +167,14 @@ void matvec(Ctx &c, int precision, int ncol, const cd *X, cd *Y) {
     const double *x = (const double *)X;
     double *y = (double *)Y;
     if (s.precision == WTB_FP64) {
-        launch_hx<double, double, double>(c.stream, N, ncol, (const double *)s.h64.p, x, y);
+        launch_hx<double, double, double>(c.stream, N, s.ld, ncol, (const double *)s.h64.p, x, y);
     } else {
         if (precision == WTB_FP64)
-            launch_hx<float, double, double>(c.stream, N, ncol, (const float *)s.h32.p, x, y);
+            launch_hx<float, double, double>(c.stream, N, s.ld, ncol, (const float *)s.h32.p, x, y);
         else if (precision == WTB_MIXED)
             matvec_f16split(c, ncol, X, Y);
         else
-            launch_hx<float, double, float>(c.stream, N, ncol, (const float *)s.h32.p, x, y);
+            launch_hx<float, double, float>(c.stream, N, s.ld, ncol, (const float *)s.h32.p, x, y);
         for (int c0 = 0; c0 < ncol; c0 += 65535) {
             int nc = std::min(65535, ncol - c0);
             hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)((N + 255) / 256), nc), dim3(256), 0, c.stream, N, nc,

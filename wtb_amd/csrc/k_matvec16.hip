@@ -10,10 +10,14 @@
 // 8-byte complex FP32 store once per launch (8 N^2 bytes).
 //
 // H is Hermitian, so row i of H is the conjugate of stored column i: each
-// wavefront streams 16 stored columns along k (contiguous), i.e. the A
-// operand of output rows i is conj(H[k + i N]).  X arrives pre-split in FP16
-// planes (re_hi, re_lo, im_hi, im_lo), column-major with k contiguous, and is
-// staged through LDS once per workgroup and K-step.
+// wavefront streams 16 stored columns along k (contiguous 64-byte runs per
+// lane), i.e. the A operand of output rows i is conj(H[k + i ld]).  X arrives
+// pre-split in FP16 planes (re_hi, re_lo, im_hi, im_lo), column-major with k
+// contiguous.  Each iteration covers KT k: the next H slice is loaded into
+// registers and the next X slice into a second LDS buffer while the MFMAs of
+// the current slice run, with one barrier per iteration.  Loads carry no
+// bounds checks (the store's rows N..ld-1 are zero, indexes are clamped), so
+// the compiler keeps the whole prefetch in flight across the MFMA phase.
 #include "wtb_internal.h"
 
 #include <cmath>
@@ -25,15 +29,14 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 constexpr int kRows = 64;      // output rows per workgroup (16 per wavefront)
-constexpr int kK = 32;         // MFMA K depth
-constexpr int kColStride = 40; // halfs per staged column (32 + pad against bank conflicts)
+constexpr int kKT = 64;        // k per iteration of the split-FP16 kernel
 
 __device__ __forceinline__ f4 mfma16(h8 a, h8 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
 // X (N x ncol, complex FP64, column-major) -> four FP16 planes
-// P[p][col][k] (k contiguous, kpad rows), scaled per column by xs[col].
+// P[p][col][k] (k contiguous, kpad rows, zero padded), scaled per column.
 __global__ void split_x_kernel(long N, long kpad, int ncol, const double *__restrict__ X,
                                const float *__restrict__ xs, _Float16 *__restrict__ P) {
     long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -66,18 +69,92 @@ __global__ void colmax_kernel(long N, int ncol, const double *__restrict__ X, un
     if ((threadIdx.x & 63) == 0) atomicMax(mx + col, __float_as_uint(m));
 }
 
-template <int NCB>
+__global__ void col_scales_kernel(int ncol, const unsigned *__restrict__ mx, float hscale, float *__restrict__ xs,
+                                  float *__restrict__ ysc) {
+    int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncol) return;
+    float m = __uint_as_float(mx[c]);
+    // scale so that max |x| s < 2^13 (FP16 max 65504, room for the products)
+    int e = 0;
+    if (m > 0.f) frexpf(m, &e);          // m = f 2^e, 0.5 <= f < 1
+    float s = ldexpf(1.0f, 13 - e);
+    xs[c] = s;
+    ysc[c] = hscale / s;
+}
+
+struct Split {
+    h8 rh, rl, ih, il, nih, nil;
+};
+
+// 8 complex FP32 values (conjugated row of H) -> split FP16 operands
+__device__ __forceinline__ Split split8(const float4 *h, float hinv) {
+    Split s;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const float v[4] = {h[q].x, h[q].y, h[q].z, h[q].w};
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            const int e = 2 * q + t;
+            const float re = v[2 * t] * hinv, im = -v[2 * t + 1] * hinv;   // conj
+            const _Float16 a = (_Float16)re, b = (_Float16)im;
+            s.rh[e] = a;
+            s.rl[e] = (_Float16)(re - (float)a);
+            s.ih[e] = b;
+            s.il[e] = (_Float16)(im - (float)b);
+        }
+    }
+    s.nih = -s.ih;
+    s.nil = -s.il;
+    return s;
+}
+
+template <int NCB, int KT>
 __global__ void __launch_bounds__(256)
-hx_f16split_kernel(long N, long kpad, int ncol, float hscale_inv, const float *__restrict__ H,
+hx_f16split_kernel(long N, long ld, int ncol, float hinv, const float2 *__restrict__ H,
                    const _Float16 *__restrict__ P, const float *__restrict__ ysc, double *__restrict__ Y) {
     constexpr int NC = NCB * 16;
-    __shared__ __align__(16) _Float16 xs[4][NC][kColStride];
+    constexpr int NCH = KT / 32;                 // MFMA depths per iteration
+    constexpr int XS = KT + 8;                   // halfs per staged column (pad against bank conflicts)
+    constexpr int QN = KT / 8;                   // 16-byte chunks per column and plane
+    constexpr int XCH = 4 * NC * QN / 256;       // 16-byte X chunks per thread per iteration
+    static_assert(4 * NC * QN % 256 == 0, "X slice must split evenly over the workgroup");
+    __shared__ __align__(16) _Float16 xs[2][4][NC][XS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int blk = lane >> 4, li = lane & 15;
-    const long row = (long)blockIdx.x * kRows + wave * 16 + li;   // output row = stored column
-    const bool rok = row < N;
-    const float2 *hcol = reinterpret_cast<const float2 *>(H) + (size_t)(rok ? row : 0) * N;
-    const size_t plane = (size_t)ncol * kpad;
+    // Rows past N (last workgroup) and X columns past ncol read clamped, valid
+    // addresses: their products land in outputs that are never written.
+    // Rows N..ld-1 of H and of the X planes are zero, so every 64-row tile of
+    // a column is read whole without bounds checks.
+    const long row = min((long)blockIdx.x * kRows + wave * 16 + li, N - 1);
+    const float2 *hcol = H + (size_t)row * ld + 8 * blk;
+    const size_t plane = (size_t)ncol * ld;
+    const long niter = (N + KT - 1) / KT;
+
+    auto load_x = [&](long it, h8 (&r)[XCH]) {
+        const long k0 = min(it, niter - 1) * KT;
+#pragma unroll
+        for (int t = 0; t < XCH; ++t) {
+            const int c = tid + 256 * t;
+            const int q = c % QN, col = (c / QN) % NC, p = c / (QN * NC);
+            r[t] = *reinterpret_cast<const h8 *>(P + p * plane + (size_t)min(col, ncol - 1) * ld + k0 + 8 * q);
+        }
+    };
+    auto store_x = [&](int buf, const h8 (&r)[XCH]) {
+#pragma unroll
+        for (int t = 0; t < XCH; ++t) {
+            const int c = tid + 256 * t;
+            const int q = c % QN, col = (c / QN) % NC, p = c / (QN * NC);
+            *reinterpret_cast<h8 *>(&xs[buf][p][col][8 * q]) = r[t];
+        }
+    };
+    // lane's A slice of depth ch: k0 + 32 ch + 8 blk + [0, 8)
+    auto load_h = [&](long it, float4 (&h)[4 * NCH]) {
+        const float4 *src = reinterpret_cast<const float4 *>(hcol + min(it, niter - 1) * KT);
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) h[4 * ch + q] = src[16 * ch + q];
+    };
 
     f4 yr[NCB], yi[NCB];
 #pragma unroll
@@ -85,66 +162,69 @@ hx_f16split_kernel(long N, long kpad, int ncol, float hscale_inv, const float *_
         yr[cb] = f4{0.f, 0.f, 0.f, 0.f};
         yi[cb] = f4{0.f, 0.f, 0.f, 0.f};
     }
+    auto compute = [&](int buf, const float4 (&hc)[4 * NCH]) {
+#pragma unroll
+        for (int ch = 0; ch < NCH; ++ch) {
+            const Split a = split8(hc + 4 * ch, hinv);
+            const int kk = 32 * ch + 8 * blk;
+            h8 b[2][4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) b[0][p] = *reinterpret_cast<const h8 *>(&xs[buf][p][li][kk]);
+#pragma unroll
+            for (int cb = 0; cb < NCB; ++cb) {
+                const int u = cb & 1;
+                if (cb + 1 < NCB) {
+#pragma unroll
+                    for (int p = 0; p < 4; ++p)
+                        b[u ^ 1][p] = *reinterpret_cast<const h8 *>(&xs[buf][p][(cb + 1) * 16 + li][kk]);
+                }
+                const h8 xrh = b[u][0], xrl = b[u][1], xih = b[u][2], xil = b[u][3];
+                // Yr += Ar Xr - Ai Xi,  Yi += Ar Xi + Ai Xr  (independent chains interleaved)
+                yr[cb] = mfma16(a.rh, xrh, yr[cb]);
+                yi[cb] = mfma16(a.rh, xih, yi[cb]);
+                yr[cb] = mfma16(a.rh, xrl, yr[cb]);
+                yi[cb] = mfma16(a.rh, xil, yi[cb]);
+                yr[cb] = mfma16(a.rl, xrh, yr[cb]);
+                yi[cb] = mfma16(a.rl, xih, yi[cb]);
+                yr[cb] = mfma16(a.nih, xih, yr[cb]);
+                yi[cb] = mfma16(a.ih, xrh, yi[cb]);
+                yr[cb] = mfma16(a.nih, xil, yr[cb]);
+                yi[cb] = mfma16(a.ih, xrl, yi[cb]);
+                yr[cb] = mfma16(a.nil, xih, yr[cb]);
+                yi[cb] = mfma16(a.il, xrh, yi[cb]);
+            }
+        }
+    };
 
-    for (long k0 = 0; k0 < N; k0 += kK) {
-        // stage the split X slice: 4 planes x NC columns x 32 k (16 B per thread-load)
-        for (int idx = tid; idx < 4 * NC * 4; idx += 256) {
-            const int q = idx & 3;                 // which 8-k chunk
-            const int col = (idx >> 2) % NC;
-            const int p = (idx >> 2) / NC;
-            h8 v = h8{0, 0, 0, 0, 0, 0, 0, 0};
-            if (col < ncol) v = *reinterpret_cast<const h8 *>(P + p * plane + (size_t)col * kpad + k0 + 8 * q);
-            *reinterpret_cast<h8 *>(&xs[p][col][8 * q]) = v;
-        }
-        // A operand: 8 consecutive k of stored column `row`, conjugated, split
-        h8 arh, arl, aih, ail, naih, nail;
-        {
-            float re[8], im[8];
-            const long kb = k0 + 8 * blk;
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                float2 h = make_float2(0.f, 0.f);
-                if (rok && kb + e < N) h = hcol[kb + e];
-                re[e] = h.x * hscale_inv;
-                im[e] = -h.y * hscale_inv;         // row i of H = conj(column i)
-            }
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-                _Float16 a = (_Float16)re[e], b = (_Float16)im[e];
-                arh[e] = a;
-                arl[e] = (_Float16)(re[e] - (float)a);
-                aih[e] = b;
-                ail[e] = (_Float16)(im[e] - (float)b);
-                naih[e] = -aih[e];
-                nail[e] = -ail[e];
-            }
-        }
+    h8 xr[XCH];
+    float4 ha[4 * NCH], hb[4 * NCH];
+    load_x(0, xr);
+    load_h(0, ha);
+    store_x(0, xr);
+    __syncthreads();
+    // iteration it computes from LDS buffer it & 1 and H registers ha / hb
+    // alternately while the next slice is in flight (the prefetch past the
+    // last slice re-reads it and is never used)
+    // (the scheduling fences keep the compiler from sinking the prefetch
+    // loads down to their first use, which would expose their latency)
+    auto step = [&](long it, int buf, const float4 (&hc)[4 * NCH], float4 (&hn)[4 * NCH]) {
+        load_x(it + 1, xr);
+        load_h(it + 1, hn);
+        __builtin_amdgcn_sched_barrier(0);
+        compute(buf, hc);
+        __builtin_amdgcn_sched_barrier(0);
+        store_x(buf ^ 1, xr);
         __syncthreads();
-#pragma unroll
-        for (int cb = 0; cb < NCB; ++cb) {
-            const int col = cb * 16 + li;
-            const h8 xrh = *reinterpret_cast<const h8 *>(&xs[0][col][8 * blk]);
-            const h8 xrl = *reinterpret_cast<const h8 *>(&xs[1][col][8 * blk]);
-            const h8 xih = *reinterpret_cast<const h8 *>(&xs[2][col][8 * blk]);
-            const h8 xil = *reinterpret_cast<const h8 *>(&xs[3][col][8 * blk]);
-            // Yr += Ar Xr - Ai Xi
-            yr[cb] = mfma16(arh, xrh, yr[cb]);
-            yr[cb] = mfma16(arh, xrl, yr[cb]);
-            yr[cb] = mfma16(arl, xrh, yr[cb]);
-            yr[cb] = mfma16(naih, xih, yr[cb]);
-            yr[cb] = mfma16(naih, xil, yr[cb]);
-            yr[cb] = mfma16(nail, xih, yr[cb]);
-            // Yi += Ar Xi + Ai Xr
-            yi[cb] = mfma16(arh, xih, yi[cb]);
-            yi[cb] = mfma16(arh, xil, yi[cb]);
-            yi[cb] = mfma16(arl, xih, yi[cb]);
-            yi[cb] = mfma16(aih, xrh, yi[cb]);
-            yi[cb] = mfma16(aih, xrl, yi[cb]);
-            yi[cb] = mfma16(ail, xrh, yi[cb]);
-        }
-        __syncthreads();
+    };
+    long it = 0;
+    for (; it + 2 <= niter; it += 2) {
+        step(it, 0, ha, hb);
+        step(it + 1, 1, hb, ha);
     }
+    if (it < niter) step(it, 0, ha, hb);
+
     // D layout (16x16 f32): register i of lane l -> row 4*(l/16)+i, column l%16
+    const long r0 = (long)blockIdx.x * kRows + wave * 16 + 4 * blk;
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) {
         const int col = cb * 16 + li;
@@ -152,7 +232,7 @@ hx_f16split_kernel(long N, long kpad, int ncol, float hscale_inv, const float *_
         const float s = ysc[col];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const long r = (long)blockIdx.x * kRows + wave * 16 + 4 * blk + i;
+            const long r = r0 + i;
             if (r < N) {
                 Y[2 * (r + (size_t)col * N)] = (double)(yr[cb][i] * s);
                 Y[2 * (r + (size_t)col * N) + 1] = (double)(yi[cb][i] * s);
@@ -161,33 +241,33 @@ hx_f16split_kernel(long N, long kpad, int ncol, float hscale_inv, const float *_
     }
 }
 
-__global__ void col_scales_kernel(int ncol, const unsigned *__restrict__ mx, float hscale, float *__restrict__ xs,
-                                  float *__restrict__ ysc) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncol) return;
-    float m = __uint_as_float(mx[c]);
-    // scale so that max |x| s ~ 2^13 (FP16 max 65504, leave room for the lo part)
-    int e = 0;
-    if (m > 0.f) frexpf(m, &e);          // m = f 2^e, 0.5 <= f < 1
-    float s = ldexpf(1.0f, 13 - e);
-    xs[c] = s;
-    ysc[c] = hscale / s;
+template <int KT>
+void launch_f16split(hipStream_t st, int ncb, dim3 grid, long N, long ld, int nc, float hinv, const cf *H,
+                     const _Float16 *P, const float *ysc, double *y) {
+    switch (ncb) {
+#define WTB_F16_CASE(K)                                                                                       \
+    case K:                                                                                                   \
+        hipLaunchKernelGGL((hx_f16split_kernel<K, KT>), grid, dim3(256), 0, st, N, ld, nc, hinv, H, P, ysc, y); \
+        break;
+        WTB_F16_CASE(1) WTB_F16_CASE(2) WTB_F16_CASE(3) WTB_F16_CASE(4)
+        WTB_F16_CASE(5) WTB_F16_CASE(6) WTB_F16_CASE(7) WTB_F16_CASE(8)
+#undef WTB_F16_CASE
+    }
 }
 
 }  // namespace
 
-// Y = H X on the FP32 store with split-FP16 MFMA; H's zero diagonal is
-// completed in FP64 by the caller.  hscale: power of two with max|H| / hscale
-// around 2^13.
+// Y = H_offdiag X on the FP32 store with split-FP16 MFMA; the caller adds the
+// FP64 diagonal.  hscale (set at assembly): power of two with
+// max|H| / hscale in [2^12, 2^13).
 void matvec_f16split(Ctx &c, int ncol, const cd *X, cd *Y) {
     Bse &s = c.bse;
-    const long N = s.N;
-    const long kpad = (N + kK - 1) / kK * kK;
+    const long N = s.N, ld = s.ld;
     for (int c0 = 0; c0 < ncol; c0 += 128) {
         const int nc = std::min(ncol - c0, 128);
         const double *x = (const double *)X + 2 * (size_t)c0 * N;
         double *y = (double *)Y + 2 * (size_t)c0 * N;
-        c.ws_split.alloc((size_t)4 * nc * kpad);
+        c.ws_split.alloc((size_t)4 * nc * ld);
         c.ws_colmax.alloc(nc);
         c.ws_colscale.alloc(2 * (size_t)nc);
         WTB_HIP_CHECK(hipMemsetAsync(c.ws_colmax.p, 0, sizeof(unsigned) * nc, c.stream));
@@ -195,22 +275,13 @@ void matvec_f16split(Ctx &c, int ncol, const cd *X, cd *Y) {
         hipLaunchKernelGGL(colmax_kernel, dim3(gx, nc), dim3(256), 0, c.stream, N, nc, x, c.ws_colmax.p);
         hipLaunchKernelGGL(col_scales_kernel, dim3((nc + 63) / 64), dim3(64), 0, c.stream, nc, c.ws_colmax.p,
                            s.hscale, c.ws_colscale.p, c.ws_colscale.p + nc);
-        hipLaunchKernelGGL(split_x_kernel, dim3((unsigned)((kpad + 255) / 256), nc), dim3(256), 0, c.stream, N, kpad,
+        hipLaunchKernelGGL(split_x_kernel, dim3((unsigned)((ld + 255) / 256), nc), dim3(256), 0, c.stream, N, ld,
                            nc, x, c.ws_colscale.p, c.ws_split.p);
         WTB_LAUNCH_CHECK();
         const int ncb = (nc + 15) / 16;
         dim3 grid((unsigned)((N + kRows - 1) / kRows));
-        const float hinv = 1.0f / s.hscale;
-        switch (ncb) {
-#define WTB_F16_CASE(K)                                                                                        \
-    case K:                                                                                                    \
-        hipLaunchKernelGGL((hx_f16split_kernel<K>), grid, dim3(256), 0, c.stream, N, kpad, nc, hinv,           \
-                           (const float *)s.h32.p, c.ws_split.p, c.ws_colscale.p + nc, y);                     \
-        break;
-            WTB_F16_CASE(1) WTB_F16_CASE(2) WTB_F16_CASE(3) WTB_F16_CASE(4)
-            WTB_F16_CASE(5) WTB_F16_CASE(6) WTB_F16_CASE(7) WTB_F16_CASE(8)
-#undef WTB_F16_CASE
-        }
+        launch_f16split<kKT>(c.stream, ncb, grid, N, ld, nc, 1.0f / s.hscale, s.h32.p, c.ws_split.p,
+                             c.ws_colscale.p + nc, y);
         WTB_LAUNCH_CHECK();
     }
 }

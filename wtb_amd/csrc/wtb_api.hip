@@ -44,20 +44,20 @@ int guarded(wtb_ctx *ctx, F &&f) {
     }
 }
 
-__global__ void block_to_c64_f64(long N, long i0, long ni, long j0, long nj, const cd *__restrict__ H,
+__global__ void block_to_c64_f64(long ld, long i0, long ni, long j0, long nj, const cd *__restrict__ H,
                                  cd *__restrict__ out, long ldo) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     long j = blockIdx.y;
-    if (i < ni && j < nj) out[i + j * ldo] = H[(i0 + i) + (j0 + j) * N];
+    if (i < ni && j < nj) out[i + j * ldo] = H[(i0 + i) + (j0 + j) * ld];
 }
 
 // the FP32 store has a zero diagonal; the FP64 diagonal array completes it
-__global__ void block_to_c64_f32(long N, long i0, long ni, long j0, long nj, const cf *__restrict__ H,
+__global__ void block_to_c64_f32(long ld, long i0, long ni, long j0, long nj, const cf *__restrict__ H,
                                  const double *__restrict__ diag, cd *__restrict__ out, long ldo) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     long j = blockIdx.y;
     if (i < ni && j < nj) {
-        cf h = H[(i0 + i) + (j0 + j) * N];
+        cf h = H[(i0 + i) + (j0 + j) * ld];
         double re = h.x;
         if (i0 + i == j0 + j) re = diag[i0 + i];
         out[i + j * ldo] = make_double2(re, h.y);
@@ -68,7 +68,7 @@ __global__ void block_to_c64_f32(long N, long i0, long ni, long j0, long nj, con
 
 namespace wtb {
 void bse_block_to_c64(Ctx &c, long i0, long ni, long j0, long nj, cd *out, long ldo) {
-    const long N = c.bse.N;
+    const long ld = c.bse.ld;
     if (nj > 65535) {
         for (long j = 0; j < nj; j += 65535)
             bse_block_to_c64(c, i0, ni, j0 + j, std::min(65535L, nj - j), out + j * ldo, ldo);
@@ -76,9 +76,9 @@ void bse_block_to_c64(Ctx &c, long i0, long ni, long j0, long nj, cd *out, long 
     }
     dim3 grid((unsigned)((ni + 255) / 256), (unsigned)nj);
     if (c.bse.precision == WTB_FP64)
-        hipLaunchKernelGGL(block_to_c64_f64, grid, dim3(256), 0, c.stream, N, i0, ni, j0, nj, c.bse.h64.p, out, ldo);
+        hipLaunchKernelGGL(block_to_c64_f64, grid, dim3(256), 0, c.stream, ld, i0, ni, j0, nj, c.bse.h64.p, out, ldo);
     else
-        hipLaunchKernelGGL(block_to_c64_f32, grid, dim3(256), 0, c.stream, N, i0, ni, j0, nj, c.bse.h32.p,
+        hipLaunchKernelGGL(block_to_c64_f32, grid, dim3(256), 0, c.stream, ld, i0, ni, j0, nj, c.bse.h32.p,
                            c.bse.diag.p, out, ldo);
     WTB_LAUNCH_CHECK();
 }
@@ -325,9 +325,9 @@ int wtb_bse_diag(wtb_ctx *ctx, double *diag) {
 }
 
 namespace {
-__global__ void upload_diag(long N, const cd *__restrict__ H, double *__restrict__ d) {
+__global__ void upload_diag(long N, long ld, const cd *__restrict__ H, double *__restrict__ d) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
-    if (i < N) d[i] = H[i + i * N].x;
+    if (i < N) d[i] = H[i + i * ld].x;
 }
 }  // namespace
 
@@ -337,14 +337,19 @@ int wtb_slepc_solver_compat(const double *HX, double *EX, double *HE, int NX, in
     int st = wtb_create(&ctx, 0);
     if (st) return st;
     st = guarded(ctx, [&](Ctx &c) {
-        const long N = NX;
-        c.bse.h64.alloc((size_t)N * N);
+        const long N = NX, ld = store_ld(N);
+        c.bse.h64.alloc((size_t)ld * N);
         c.bse.diag.alloc(N);
-        copy(c, c.bse.h64.p, HX, sizeof(cd) * N * N, hipMemcpyHostToDevice);
-        hipLaunchKernelGGL(upload_diag, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c.stream, N, c.bse.h64.p,
-                           c.bse.diag.p);
+        // the host matrix has leading dimension N; the store pads it to ld with zero rows
+        WTB_HIP_CHECK(hipMemsetAsync(c.bse.h64.p, 0, sizeof(cd) * ld * N, c.stream));
+        WTB_HIP_CHECK(hipMemcpy2DAsync(c.bse.h64.p, sizeof(cd) * ld, HX, sizeof(cd) * N, sizeof(cd) * N, N,
+                                       hipMemcpyHostToDevice, c.stream));
+        WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+        hipLaunchKernelGGL(upload_diag, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c.stream, N, ld,
+                           c.bse.h64.p, c.bse.diag.p);
         WTB_LAUNCH_CHECK();
         c.bse.N = N;
+        c.bse.ld = ld;
         c.bse.precision = WTB_FP64;
         c.bse.valid = true;
         wtb_report r{};

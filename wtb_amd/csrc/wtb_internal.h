@@ -89,13 +89,19 @@ struct Bands {
     bool valid = false;
 };
 
+// Leading dimension of the BSE store: N rounded up to a multiple of 64.  Rows
+// N..ld-1 of every column are zero, so the matvec streams whole 64-row tiles
+// of each column without bounds checks and every column starts 512-byte aligned.
+inline long store_ld(long N) { return (N + 63) / 64 * 64; }
+
 struct Bse {
     long N = 0;
+    long ld = 0;         // store_ld(N)
     int precision = -1;
     int convention = 0;
     wtb_coulomb coul{};
-    DBuf<cd> h64;        // N x N column-major (FP64 store)
-    DBuf<cf> h32;        // N x N column-major (FP32 store)
+    DBuf<cd> h64;        // ld x N column-major (FP64 store)
+    DBuf<cf> h32;        // ld x N column-major (FP32 store)
     DBuf<double> diag;   // E_c - E_v
     DBuf<unsigned> hmax; // max |Re|, |Im| of the FP32 store (float bits)
     float hscale = 1.f;  // power of two: max |H| / hscale ~ 2^13 for the FP16 split
