@@ -88,6 +88,8 @@ def test_bse_matrix(precision, conv):
                       textbook=conv == hip.CONV_TEXTBOOK)
     tol = 1e-12 if precision == hip.FP64 else 2e-6 * np.abs(Hr).max()
     assert np.abs(H - Hr).max() < tol
+    # the store is exactly Hermitian (the matvec reads rows as conjugated columns)
+    assert np.array_equal(H, H.conj().T)
     assert np.allclose(ctx.diag(), np.diag(Hr).real, atol=1e-12)
     ctx.close()
 

@@ -93,6 +93,11 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
                  const double *__restrict__ kpt, const double *__restrict__ esel,
                  const typename CT<R>::C *__restrict__ usel, typename CT<R>::C *__restrict__ H,
                  unsigned *__restrict__ hmax) {
+    // No FMA contraction: every product and sum is then rounded the same way
+    // for the pair (k,k') as for (k',k) with the roles of the factors swapped,
+    // so the stored matrix is exactly Hermitian, H[j][i] == conj(H[i][j]) bit
+    // for bit (the matvec reads row i as the conjugate of stored column i).
+#pragma clang fp contract(off)
     using C = typename CT<R>::C;
     float vmax = 0.f;   // largest |Re|, |Im| written (scales the FP16 split of the matvec)
     constexpr int NB = NC + NV, NCV = NC * NV, NC2 = NC * NC, NV2 = NV * NV;
