@@ -53,9 +53,12 @@ def sync():
         _torch.cuda.synchronize()
 
 
-def cpu_baseline(model, kpt, mesh, nev, fp64_mvcols, seconds=20.0):
-    """Time the CPU oracle (FP64 C restatement, OpenMP) on a bounded sample of
-    the same workload and scale it to one solve."""
+def cpu_baseline(model, kpt, mesh, nev, fp64_mvcols, ncols=64):
+    """Time the CPU oracle (FP64 C restatement, OpenMP) on the same workload:
+    all H(k) eigensolves, the full FP64 BSE assembly (16.8 GB on the host at
+    N = 32400) and one ncols-column dense FP64 matvec through it, then scale
+    the matvec to the column count the GPU FP64 solver needed.  About 10-20 s
+    of CPU work on 16 cores."""
     from oracle import oracle as O
     nthreads = O.num_threads()
     om = O.Model(model)
@@ -64,24 +67,51 @@ def cpu_baseline(model, kpt, mesh, nev, fp64_mvcols, seconds=20.0):
     t_hk = time.perf_counter() - t
     N = kpt.shape[0] * 4
     c = O.coulomb(**COUL)
-    rows = 64
     t = time.perf_counter()
-    h = O.bse_rows(kpt, r["eig"], r["vec"], c, mesh, model.rlat, 2, 2, 0, rows)
-    t_rows = time.perf_counter() - t
-    t_asm = t_rows * N / rows
-    x = np.ones((N, 16), np.complex128)
+    h = O.bse_rows(kpt, r["eig"], r["vec"], c, mesh, model.rlat, 2, 2, 0, N)
+    t_asm = time.perf_counter() - t
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((N, ncols)) + 1j * rng.standard_normal((N, ncols))
     t = time.perf_counter()
     O.matvec_rows(h, x)
     t_mv = time.perf_counter() - t
-    t_mv_full_col = t_mv * (N / rows) / 16          # one column through the full matrix
-    t_solve = t_mv_full_col * fp64_mvcols
+    del h
+    t_solve = t_mv * fp64_mvcols / ncols
     total = t_hk + t_asm + t_solve
     sample = (f"oracle (C, FP64, {nthreads} OpenMP threads) on the N={N} workload: all {kpt.shape[0]} "
-              f"H(k) eigensolves ({t_hk:.2f} s), BSE assembly of {rows} rows ({t_rows:.2f} s, x{N // rows} "
-              f"for the full matrix), a 16-column dense FP64 matvec over those rows ({t_mv:.3f} s) scaled to "
-              f"{fp64_mvcols} full-matrix columns (the GPU FP64 solver's count)")
+              f"H(k) eigensolves ({t_hk:.2f} s), the full FP64 BSE assembly ({t_asm:.2f} s) and one "
+              f"{ncols}-column dense FP64 matvec through it ({t_mv:.2f} s), the matvec scaled to the "
+              f"{fp64_mvcols} columns the GPU FP64 solver applied (eigensolver overhead beyond the "
+              f"matvecs not counted)")
     return {"value": total, "unit": "s", "cores": nthreads, "kind": "port", "sample": sample,
-            "stages_s": {"hk_eig": t_hk, "assembly": t_asm, "solve": t_solve}}
+            "stages_s": {"hk_eig": t_hk, "assembly": t_asm, "matvec_sample": t_mv, "solve": t_solve}}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_hx_f16split.json")
+
+
+def load_traffic():
+    """HBM bytes per launch of the split-FP16 kernel from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by a 1 GiB copy), if any."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+        return t["bytes_per_launch"], os.path.relpath(TRAFFIC_FILE, ROOT)
+    except (OSError, KeyError, ValueError):
+        return None, None
+
+
+def calibrate_copy():
+    """A plain 1 GiB device copy (read 1 GiB, write 1 GiB): the reference
+    kernel that scales the FETCH_SIZE / WRITE_SIZE counters of a --pmc run."""
+    if _torch is None:
+        return
+    a = _torch.ones(1 << 28, dtype=_torch.float32, device="cuda")
+    b = _torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    _torch.cuda.synchronize()
+    del a, b
 
 
 def main():
@@ -93,6 +123,8 @@ def main():
     ap.add_argument("--nev", type=int, default=64)
     ap.add_argument("--fp64-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run a 1 GiB device copy first (reference kernel for rocprofv3 --pmc scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -115,6 +147,8 @@ def main():
     N = kpt.shape[0] * 4
     log(f"model generated in {time.perf_counter() - t:.1f} s; N = {N}")
 
+    if args.calibrate_copy:
+        calibrate_copy()
     ctx = hip.Context(0)
     ctx.set_tb_model(model)
     ctx.set_kpoints(kpt, mesh)
@@ -141,15 +175,33 @@ def main():
     t_fp64 = float(np.mean(f64_times))
     dE_meV = float(np.max(np.abs(E_mixed - E64)) * 1e3)
 
-    # roofline of the dominant kernel: the mixed-path matvec (FP32 MFMA)
+    # roofline of the dominant kernel of the mixed solve: the split-FP16 MFMA
+    # matvec, HBM-bound (it streams the 8 ld N-byte FP32 store once a launch).
+    # Its launches were bracketed with HIP events on the library stream during
+    # the timed solves; achieved = algorithmic bytes / device time.
     rep = reps[-1]
-    mv_flops = 8.0 * N * N * rep["matvec_columns"]
-    fp32_cols = rep["matvec_columns"]
-    achieved = mv_flops / (rep["t_matvec_ms"] * 1e-3) / 1e12 if rep["t_matvec_ms"] > 0 else None
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                "frac": achieved / PEAK_FP32_TFLOPS if achieved else None, "traffic": None,
-                "kernel": "zgemm_hx_kernel (H*X, FP32 store, FP32 MFMA; includes the FP64 refinement pass)",
-                "per_unit": "8 N^2 flop per column of X"}
+    launches = sum(r["kernel_launches"] for r in reps)
+    kbytes = sum(r["kernel_bytes"] for r in reps)
+    kms = sum(r["t_kernel_ms"] for r in reps)
+    achieved = kbytes / (kms * 1e-3) / 1e9 if kms > 0 else None
+    traffic, traffic_src = load_traffic()
+    roofline = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS if achieved else None,
+                "traffic": traffic,
+                "kernel": "hx_f16split_kernel (Y = H X on the FP32 store, split-FP16 MFMA)",
+                "launches": launches, "avg_launch_ms": kms / launches if launches else None,
+                "bytes_per_launch": kbytes / launches if launches else None,
+                "per_unit": "8 B per stored complex FP32 element, ld*N elements per launch (ld = N rounded up to 64)",
+                "traffic_source": traffic_src,
+                "mfma_tflops": (sum(r["kernel_flops"] for r in reps) / (kms * 1e-3) / 1e12) if kms > 0 else None}
+    refine = {"kernel": "zgemm_hx_kernel<float,double> (FP64 MFMA on the FP32 store)",
+              "avg_launch_ms": rep["t_refine_kernel_ms"] / max(1, rep["refine_launches"]),
+              "tflops": rep["refine_flops"] / (rep["t_refine_kernel_ms"] * 1e-3) / 1e12
+              if rep["t_refine_kernel_ms"] > 0 else None, "peak_tflops": PEAK_FP64_TFLOPS}
+    fp64_kernel = {"kernel": "zgemm_hx_kernel<double,double> (FP64 MFMA on the FP64 store)",
+                   "avg_launch_ms": rep64["t_kernel_ms"] / max(1, rep64["kernel_launches"]),
+                   "tflops": rep64["kernel_flops"] / (rep64["t_kernel_ms"] * 1e-3) / 1e12
+                   if rep64["t_kernel_ms"] > 0 else None, "peak_tflops": PEAK_FP64_TFLOPS}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -161,7 +213,7 @@ def main():
     out = {
         "metric": METRIC, "value": t_mixed, "unit": "s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": t_mixed * 1e3, "higher_is_better": False,
-        "scaling": "weak", "vs_baseline": None, "dtype": "c64 MFMA matvec + c128 RR/refinement",
+        "scaling": "weak", "vs_baseline": None, "dtype": "c64 (split-fp16 MFMA, fp32 accumulate) + c128",
         "data": "synthetic (seeded random TB model, seed 2026)",
         "config": {"workload": f"hex22-{args.mesh}: 22-orbital hexagonal TB, {args.mesh}x{args.mesh} mesh, "
                                f"nv=nc=2, V2DRK r0=40, N={N}, nev={args.nev}",
@@ -169,7 +221,7 @@ def main():
         "fp64_s_per_solve": t_fp64, "speedup_vs_fp64": t_fp64 / t_mixed,
         "max_abs_dE_meV_vs_fp64": dE_meV,
         "stages_mixed_s": stages[-1], "report_mixed": rep, "report_fp64": rep64,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "refine_kernel": refine, "fp64_kernel": fp64_kernel, "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
     ctx.close()

@@ -83,6 +83,17 @@ typedef struct {
     double t_refine_ms;      /* FP64 refinement part of t_solve_ms            */
     double t_matvec_ms;      /* time inside H*X kernels                       */
     double t_rr_ms;          /* time in Rayleigh-Ritz / orthogonalisation     */
+    /* The solve's dominant matvec kernel (split-FP16 MFMA for WTB_MIXED, FP64
+     * MFMA for WTB_FP64, FP32 MFMA for WTB_FP32), timed with HIP events on the
+     * library stream around each launch: */
+    double t_kernel_ms;      /* summed device time of its launches            */
+    int kernel_launches;     /* number of launches                            */
+    double kernel_bytes;     /* algorithmic HBM bytes: the store, once a launch */
+    double kernel_flops;     /* 8 N^2 per column of X                         */
+    /* The FP64-arithmetic refinement kernel of the mixed path (zero for FP64): */
+    double t_refine_kernel_ms;
+    int refine_launches;
+    double refine_flops;
 } wtb_report;
 
 /* Context. device: HIP ordinal.  Replaces the implicit module state of

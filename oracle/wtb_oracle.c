@@ -583,15 +583,21 @@ int or_bse_rows(int nk, int nc, int nv, int norb, const double *kpt, const doubl
  * the dense FP64 matvec of a CPU eigensolver, OpenMP over rows. */
 void or_matvec_rows(long ni, long N, int ncol, const double complex *h, const double complex *x,
                     double complex *y) {
+    /* explicit real arithmetic: C99 complex '*' would go through the
+       NaN-checking library multiply */
 #pragma omp parallel for schedule(static)
     for (long i = 0; i < ni; ++i) {
-        double complex *yi = y + i * ncol;
-        for (int c = 0; c < ncol; ++c) yi[c] = 0;
-        const double complex *hi = h + i * N;
+        double *yi = (double *)(y + i * ncol);
+        for (int c = 0; c < 2 * ncol; ++c) yi[c] = 0.0;
+        const double *hi = (const double *)(h + i * N);
         for (long j = 0; j < N; ++j) {
-            double complex hij = hi[j];
-            const double complex *xj = x + j * ncol;
-            for (int c = 0; c < ncol; ++c) yi[c] += hij * xj[c];
+            const double hr = hi[2 * j], hm = hi[2 * j + 1];
+            const double *xj = (const double *)(x + j * ncol);
+            for (int c = 0; c < ncol; ++c) {
+                const double xr = xj[2 * c], xm = xj[2 * c + 1];
+                yi[2 * c] += hr * xr - hm * xm;
+                yi[2 * c + 1] += hr * xm + hm * xr;
+            }
         }
     }
 }

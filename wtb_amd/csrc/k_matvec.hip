@@ -194,13 +194,15 @@ zgemm_hx_kernel(long N, long ld, int ncol, const TH *__restrict__ H, const doubl
 }
 
 template <class TH, class TC>
-void launch_hx(hipStream_t st, long N, long ld, int ncol, const TH *H, const double *X, double *Y) {
+void launch_hx(hipStream_t st, long N, long ld, int ncol, const TH *H, const double *X, double *Y,
+               KernelClock &clk, KernelClock::Kind kind) {
     for (int c0 = 0; c0 < ncol; c0 += kMaxCB * 16) {
         int nc = std::min(ncol - c0, kMaxCB * 16);
         int ncb = (nc + 15) / 16;
         dim3 grid((unsigned)((N + kRows - 1) / kRows));
         const double *Xc = X + 2 * (size_t)c0 * N;
         double *Yc = Y + 2 * (size_t)c0 * N;
+        clk.begin(kind, st);
         switch (ncb) {
 #define WTB_HX_CASE(K) \
     case K: hipLaunchKernelGGL((zgemm_hx_kernel<TH, TC, K>), grid, dim3(256), 0, st, N, ld, nc, H, Xc, Yc); break;
@@ -208,6 +210,7 @@ void launch_hx(hipStream_t st, long N, long ld, int ncol, const TH *H, const dou
             WTB_HX_CASE(5) WTB_HX_CASE(6) WTB_HX_CASE(7) WTB_HX_CASE(8)
 #undef WTB_HX_CASE
         }
+        clk.end(kind, st, 2.0 * sizeof(TH) * (double)ld * N, 8.0 * (double)N * N * nc);
         WTB_LAUNCH_CHECK();
     }
 }
@@ -238,14 +241,17 @@ void matvec(Ctx &c, int precision, int ncol, const cd *X, cd *Y) {
     const double *x = (const double *)X;
     double *y = (double *)Y;
     if (s.precision == WTB_FP64) {
-        launch_hx<double, double>(c.stream, N, s.ld, ncol, (const double *)s.h64.p, x, y);
+        launch_hx<double, double>(c.stream, N, s.ld, ncol, (const double *)s.h64.p, x, y, c.kclock,
+                                  KernelClock::F64_ON_F64);
     } else {
         if (precision == WTB_FP64)
-            launch_hx<float, double>(c.stream, N, s.ld, ncol, (const float *)s.h32.p, x, y);
+            launch_hx<float, double>(c.stream, N, s.ld, ncol, (const float *)s.h32.p, x, y, c.kclock,
+                                     KernelClock::F64_ON_F32);
         else if (precision == WTB_MIXED)
             matvec_f16split(c, ncol, X, Y);
         else
-            launch_hx<float, float>(c.stream, N, s.ld, ncol, (const float *)s.h32.p, x, y);
+            launch_hx<float, float>(c.stream, N, s.ld, ncol, (const float *)s.h32.p, x, y, c.kclock,
+                                    KernelClock::F32_ON_F32);
         for (int c0 = 0; c0 < ncol; c0 += 65535) {
             int nc = std::min(65535, ncol - c0);
             hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)((N + 255) / 256), nc), dim3(256), 0, c.stream, N, nc,

@@ -280,8 +280,10 @@ void matvec_f16split(Ctx &c, int ncol, const cd *X, cd *Y) {
         WTB_LAUNCH_CHECK();
         const int ncb = (nc + 15) / 16;
         dim3 grid((unsigned)((N + kRows - 1) / kRows));
+        c.kclock.begin(KernelClock::F16SPLIT, c.stream);
         launch_f16split<kKT>(c.stream, ncb, grid, N, ld, nc, 1.0f / s.hscale, s.h32.p, c.ws_split.p,
                              c.ws_colscale.p + nc, y);
+        c.kclock.end(KernelClock::F16SPLIT, c.stream, 8.0 * (double)ld * N, 8.0 * (double)N * N * nc);
         WTB_LAUNCH_CHECK();
     }
 }

@@ -497,13 +497,17 @@ struct Lobpcg {
     // the Gram matrix is diagonalised first and nearly dependent directions
     // are dropped (SVQB).  The first b columns become the new X, the
     // remainder combination the new P when withP.
-    void rayleigh_ritz(int m, bool withP, bool orth = true) {
+    // Ritz pairs of the m columns of S: the lowest nout (default b) replace
+    // X = S[:, :nout] and AX (and P, AP when withP).
+    void rayleigh_ritz(int m, bool withP, bool orth = true, int nout = 0) {
+        const int nb = nout > 0 ? nout : b;
+        if (withP && nb != b) throw Error{WTB_ERR_STATE, "partial Rayleigh-Ritz with P"};
         trr.begin();
         gram(c, m, m, N, S.p, (int)N, AS.p, (int)N, A.p, m3);
         hipLaunchKernelGGL(hermitize, dim3((m + 63) / 64, m), dim3(64), 0, c.stream, m, A.p, m3);
         WTB_LAUNCH_CHECK();
         if (orth) {
-            heev_lowest(c, m, b, A.p, m3, lamr.p, Cm.p, m3);
+            heev_lowest(c, m, nb, A.p, m3, lamr.p, Cm.p, m3);
         } else {
             gram(c, m, m, N, S.p, (int)N, S.p, (int)N, G.p, m3);
             hipLaunchKernelGGL(hermitize, dim3((m + 63) / 64, m), dim3(64), 0, c.stream, m, G.p, m3);
@@ -516,7 +520,7 @@ struct Lobpcg {
             int off = 0;
             while (off < m && hd[off] <= drop_rel * dmax) ++off;
             const int r = m - off;
-            if (r < b) throw Error{WTB_ERR_NOCONV, "Rayleigh-Ritz basis lost rank"};
+            if (r < nb) throw Error{WTB_ERR_NOCONV, "Rayleigh-Ritz basis lost rank"};
             hipLaunchKernelGGL(svqb_t, dim3((m + 63) / 64, r), dim3(64), 0, c.stream, m, r, off, U.p, m3, D.p, T.p,
                                m3);
             WTB_LAUNCH_CHECK();
@@ -525,25 +529,25 @@ struct Lobpcg {
             gram(c, r, r, m, T.p, m3, Tmp.p, m3, G.p, m3);
             hipLaunchKernelGGL(hermitize, dim3((r + 63) / 64, r), dim3(64), 0, c.stream, r, G.p, m3);
             WTB_LAUNCH_CHECK();
-            heev_lowest(c, r, b, G.p, m3, lamr.p, U.p, m3);
+            heev_lowest(c, r, nb, G.p, m3, lamr.p, U.p, m3);
             // C = T Y[:, :b]  (m x b)
-            gemm_nn(c, m, b, r, T.p, m3, U.p, m3, Cm.p, m3);
+            gemm_nn(c, m, nb, r, T.p, m3, U.p, m3, Cm.p, m3);
         }
-        WTB_HIP_CHECK(hipMemcpyAsync(lam.p, lamr.p, sizeof(double) * b, hipMemcpyDeviceToDevice, c.stream));
+        WTB_HIP_CHECK(hipMemcpyAsync(lam.p, lamr.p, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));
         // new X, AX (and P, AP)
-        gemm_nn(c, N, b, m, S.p, N, Cm.p, m3, T1.p, N);
+        gemm_nn(c, N, nb, m, S.p, N, Cm.p, m3, T1.p, N);
         if (withP && m > b) gemm_nn(c, N, b, m - b, S.p + (size_t)b * N, N, Cm.p + b, m3, T1.p + (size_t)b * N, N);
-        WTB_HIP_CHECK(hipMemcpyAsync(S.p, T1.p, sizeof(cd) * N * b, hipMemcpyDeviceToDevice, c.stream));
+        WTB_HIP_CHECK(hipMemcpyAsync(S.p, T1.p, sizeof(cd) * N * nb, hipMemcpyDeviceToDevice, c.stream));
         if (withP && m > b)
             WTB_HIP_CHECK(hipMemcpyAsync(P.p, T1.p + (size_t)b * N, sizeof(cd) * N * b, hipMemcpyDeviceToDevice, c.stream));
-        gemm_nn(c, N, b, m, AS.p, N, Cm.p, m3, T1.p, N);
+        gemm_nn(c, N, nb, m, AS.p, N, Cm.p, m3, T1.p, N);
         if (withP && m > b) gemm_nn(c, N, b, m - b, AS.p + (size_t)b * N, N, Cm.p + b, m3, T1.p + (size_t)b * N, N);
-        WTB_HIP_CHECK(hipMemcpyAsync(AS.p, T1.p, sizeof(cd) * N * b, hipMemcpyDeviceToDevice, c.stream));
+        WTB_HIP_CHECK(hipMemcpyAsync(AS.p, T1.p, sizeof(cd) * N * nb, hipMemcpyDeviceToDevice, c.stream));
         if (withP && m > b)
             WTB_HIP_CHECK(hipMemcpyAsync(AP.p, T1.p + (size_t)b * N, sizeof(cd) * N * b, hipMemcpyDeviceToDevice, c.stream));
         trr.end();
-        h_lam.resize(b);
-        WTB_HIP_CHECK(hipMemcpyAsync(h_lam.data(), lam.p, sizeof(double) * b, hipMemcpyDeviceToHost, c.stream));
+        h_lam.resize(nb);
+        WTB_HIP_CHECK(hipMemcpyAsync(h_lam.data(), lam.p, sizeof(double) * nb, hipMemcpyDeviceToHost, c.stream));
         WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
     }
 };
@@ -589,6 +593,16 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
         (void)hipEventDestroy(a1);
         return;
     }
+    // per-launch device time of the matvec kernels, for the report
+    struct ClockGuard {
+        KernelClock &k;
+        ~ClockGuard() {
+            k.on = false;
+            k.reset();
+        }
+    } clock_guard{c.kclock};
+    c.kclock.reset();
+    c.kclock.on = rep != nullptr;
     hipEvent_t t0, t1, tr;
     WTB_HIP_CHECK(hipEventCreate(&t0));
     WTB_HIP_CHECK(hipEventCreate(&t1));
@@ -714,12 +728,13 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     WTB_HIP_CHECK(hipEventRecord(tr, c.stream));
     double max_rel = 0;
     if (precision != WTB_FP64) {
-        // FP64 residual refinement: H X in FP64 arithmetic, Rayleigh-Ritz on X
-        L.apply(WTB_FP64, b, L.S.p, L.AS.p);
-        mvcols += b;
-        L.rayleigh_ritz(b, false);
+        // FP64 residual refinement: H X in FP64 arithmetic on the nev wanted
+        // columns, then a Rayleigh-Ritz step on them
+        L.apply(WTB_FP64, nev, L.S.p, L.AS.p);
+        mvcols += nev;
+        L.rayleigh_ritz(nev, false, true, nev);
     }
-    std::vector<double> n2 = L.norms(b, L.S.p, L.AS.p, L.lam.p);
+    std::vector<double> n2 = L.norms(nev, L.S.p, L.AS.p, L.lam.p);
     nconv = 0;
     for (int j = 0; j < nev; ++j) {
         double r = std::sqrt(n2[j]) / std::max(std::fabs(L.h_lam[j]), 1e-300);
@@ -747,6 +762,20 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
         rep->t_refine_ms = precision != WTB_FP64 ? elapsed_ms(tr, t1) : 0.0;
         rep->t_matvec_ms = L.tmv.total();
         rep->t_rr_ms = L.trr.total();
+        const KernelClock &k = c.kclock;
+        const KernelClock::Kind main = c.bse.precision == WTB_FP64 ? KernelClock::F64_ON_F64
+                                       : precision == WTB_MIXED    ? KernelClock::F16SPLIT
+                                       : precision == WTB_FP32     ? KernelClock::F32_ON_F32
+                                                                   : KernelClock::F64_ON_F32;
+        rep->t_kernel_ms = k.ms(main);
+        rep->kernel_launches = (int)k.slot[main].ev.size();
+        rep->kernel_bytes = k.slot[main].bytes;
+        rep->kernel_flops = k.slot[main].flops;
+        if (main != KernelClock::F64_ON_F32 && main != KernelClock::F64_ON_F64) {
+            rep->t_refine_kernel_ms = k.ms(KernelClock::F64_ON_F32);
+            rep->refine_launches = (int)k.slot[KernelClock::F64_ON_F32].ev.size();
+            rep->refine_flops = k.slot[KernelClock::F64_ON_F32].flops;
+        }
     }
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);

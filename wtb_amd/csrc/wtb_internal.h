@@ -115,6 +115,55 @@ struct Solution {
     bool valid = false;
 };
 
+// Device time of the matvec kernels proper: while `on`, every launch of a
+// matvec kernel is bracketed by HIP events on the context stream and its
+// algorithmic bytes (the store, read once) and flops (8 N^2 per column) are
+// added to the slot of its kind.
+struct KernelClock {
+    enum Kind { F16SPLIT = 0, F64_ON_F64 = 1, F64_ON_F32 = 2, F32_ON_F32 = 3, NKIND = 4 };
+    struct Slot {
+        std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
+        double bytes = 0, flops = 0;
+    };
+    bool on = false;
+    Slot slot[NKIND];
+    void begin(Kind k, hipStream_t st) {
+        if (!on) return;
+        hipEvent_t a, b;
+        if (hipEventCreate(&a) != hipSuccess) return;
+        if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return; }
+        (void)hipEventRecord(a, st);
+        slot[k].ev.push_back({a, b});
+    }
+    void end(Kind k, hipStream_t st, double bytes, double flops) {
+        if (!on || slot[k].ev.empty()) return;
+        (void)hipEventRecord(slot[k].ev.back().second, st);
+        slot[k].bytes += bytes;
+        slot[k].flops += flops;
+    }
+    // summed milliseconds of a slot (the stream must have completed)
+    double ms(Kind k) const {
+        double t = 0;
+        for (auto &p : slot[k].ev) {
+            float m = 0.f;
+            (void)hipEventElapsedTime(&m, p.first, p.second);
+            t += m;
+        }
+        return t;
+    }
+    void reset() {
+        for (auto &s : slot) {
+            for (auto &p : s.ev) {
+                (void)hipEventDestroy(p.first);
+                (void)hipEventDestroy(p.second);
+            }
+            s.ev.clear();
+            s.bytes = s.flops = 0;
+        }
+    }
+    ~KernelClock() { reset(); }
+};
+
 struct Ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -140,6 +189,7 @@ struct Ctx {
     DBuf<_Float16> ws_split;   // FP16 split planes of X for the mixed matvec
     DBuf<unsigned> ws_colmax;  // per-column max |x|
     DBuf<float> ws_colscale;   // per-column split / output scales
+    KernelClock kclock;
 };
 
 // k.R = (kx Rx + ky Ry) + kz Rz, evaluated without FMA contraction in the

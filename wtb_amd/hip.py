@@ -47,7 +47,10 @@ class CoulombParams(C.Structure):
 class Report(C.Structure):
     _fields_ = [("iterations", C.c_int), ("nconv", C.c_int), ("matvec_columns", C.c_long),
                 ("max_rel_residual", C.c_double), ("t_solve_ms", C.c_double),
-                ("t_refine_ms", C.c_double), ("t_matvec_ms", C.c_double), ("t_rr_ms", C.c_double)]
+                ("t_refine_ms", C.c_double), ("t_matvec_ms", C.c_double), ("t_rr_ms", C.c_double),
+                ("t_kernel_ms", C.c_double), ("kernel_launches", C.c_int), ("kernel_bytes", C.c_double),
+                ("kernel_flops", C.c_double), ("t_refine_kernel_ms", C.c_double),
+                ("refine_launches", C.c_int), ("refine_flops", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}
