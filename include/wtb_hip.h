@@ -46,9 +46,9 @@ extern "C" {
 
 /* precision of the BSE kernel store and the Krylov matvecs */
 #define WTB_FP64   0             /* complex FP64 store, FP64 MFMA matvec                     */
-#define WTB_MIXED  1             /* complex FP32 store, split-bf16 MFMA matvec with FP32
-                                    accumulation, FP64 orthogonalisation / Rayleigh-Ritz /
-                                    final residual refinement                               */
+#define WTB_MIXED  1             /* complex FP32 store (FP64 diagonal), split-FP16 MFMA matvec
+                                    with FP32 accumulation, FP64 orthogonalisation /
+                                    Rayleigh-Ritz / final residual refinement               */
 #define WTB_FP32   2             /* complex FP32 store, FP32 MFMA matvec (no refinement)     */
 
 /* kernel overlap convention */
@@ -158,7 +158,7 @@ int wtb_oscillator(wtb_ctx *ctx, int nev, int comp_a, int comp_b, double *act);
  *   wtb_bse_matvec    : Y = H X for host X, Y complex(8) (N, ncol), with the
  *                       kernel of the given precision (WTB_FP64 / WTB_MIXED /
  *                       WTB_FP32).
- *   wtb_bse_diag      : the transition energies E_c - E_v (N). */
+ *   wtb_bse_diag      : the diagonal of H, E_c - E_v + V(q=0) (N). */
 int wtb_bse_get_block(wtb_ctx *ctx, long i0, long ni, long j0, long nj, double *out);
 int wtb_bse_matvec(wtb_ctx *ctx, int ncol, const double *X, double *Y, int precision);
 int wtb_bse_diag(wtb_ctx *ctx, double *diag);

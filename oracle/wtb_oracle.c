@@ -523,7 +523,12 @@ void or_imagrenorm(int n, int nelux, const double *ex, double *dielf) {
         if (dielf[3 * i] <= aux) ib = i;
         else break;
     }
-    if (ib + 1 >= nelux) return;
+    if (ib + 1 >= nelux) {
+        /* every grid point lies below E_1 - 0.15: the reference zeroes them all
+           (its read of point ib+1 is past the grid and never used) */
+        for (int i = 0; i < nelux; ++i) dielf[3 * i + 2] = 0.0;
+        return;
+    }
     double aux3 = dielf[3 * (ib + 1) + 2];
     for (int i = 0; i <= ib; ++i) dielf[3 * i + 2] = 0.0;
     for (int i = ib + 1; i < nelux; ++i) {

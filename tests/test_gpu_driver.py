@@ -1,0 +1,165 @@
+"""The Fortran driver (wtb_amd/wtb_bse, ISO_C_BINDING on the HIP library) on the
+honeycomb case: its output files against the oracle.
+
+bse_opt_diel.dat: exciton energies against the eigenvalues of the oracle
+matrix and the oscillator strengths summed over each degenerate level (the
+sum is invariant under rotations inside a degenerate eigenspace);
+sp_optics.dat: the sorted transition energies and |d|^2;
+bse_diel_*.dat: the oracle spectrum (optdiel + imagrenorm) evaluated on the
+driver's own 4-decimal bse_opt_diel.dat, as the reference re-reads it;
+bseoptprop files and exciton_lifetime.dat: the closed-form optical constants
+and lifetimes of those spectra.
+"""
+import os
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from wtb_amd import tbmodel as T
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DRIVER = os.path.join(ROOT, "wtb_amd", "wtb_bse")
+PI32 = float(np.float32(np.arccos(np.float32(-1.0))))
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def load(path):
+    return np.loadtxt(path, comments="#", ndmin=2)
+
+
+def run_driver(cards):
+    d = tempfile.mkdtemp(prefix="wtb")
+    m = T.honeycomb_two_band()
+    T.write_tb_file(m, os.path.join(d, "tb.dat"))
+    text = "\n".join([f'PARAMS_FILE= "{d}/tb.dat"', f'OUTPUT= "{d}/"'] + cards) + "\n"
+    say("driver cards:", text.replace("\n", " | "))
+    r = subprocess.run([DRIVER], input=text, capture_output=True, text=True, timeout=240, cwd=d)
+    say(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0, r.stdout + r.stderr
+    return d, m
+
+
+@pytest.fixture(scope="module")
+def honeycomb_run():
+    if not os.path.exists(DRIVER):
+        pytest.fail("wtb_amd/wtb_bse is not built (make -C wtb_amd/fortran)")
+    cards = ["BSE= T", "SPEC= T", "DTDIAG= T", "CPOL= T", "SYSDIM= \"2D\"", "NGX= 12", "NGY= 12", "NGZ= 1",
+             "NBANDSC= 1", "NBANDSV= 1", "COULOMB_POT= V2DRK", "R_0= 10.0", "EDIEL= 1.0", "EDIEL_T= 1.0",
+             "EDIEL_B= 1.0", "EDIEL_Z= 1.0", "LC= 1.0", "KTOL= 0.001", "CSHIFT= 0.05", "ENSPECI= 3.0",
+             "ENSPECF= 7.0", "NESPEC= 400", "NEV= 144", "PRECISION= FP64"]
+    return run_driver(cards)
+
+
+def oracle_case(m):
+    mesh = (12, 12, 1)
+    k = T.mp_mesh(*mesh, m.rlat)
+    om = O.Model(m)
+    ref = O.hk_eig(om, k, 1, 1)
+    c = O.coulomb(kind="V2DRK", r0=10.0, ediel=(1.0, 1.0, 1.0), ez=1.0, w=0.0, lc=1.0, ktol=0.001)
+    H = O.bse_matrix(k, ref["eig"], ref["vec"], c, mesh, m.rlat, 1, 1)
+    w, X = np.linalg.eigh(H)
+    d = O.dipoles(om, k, ref["eig"], ref["vec"], 1, 1, 0.05)      # (5, N): x, y, z, sp, sm
+    return mesh, k, ref, w, X, d
+
+
+def test_bse_opt_diel_energies_and_strengths(honeycomb_run):
+    d, m = honeycomb_run
+    say("case driver bse_opt_diel")
+    mesh, k, ref, w, X, dip = oracle_case(m)
+    out = load(os.path.join(d, "bse_opt_diel.dat"))
+    assert out.shape == (144, 7)
+    assert np.abs(out[:, 0] - w).max() <= 5.1e-5            # F15.4 rounding
+    # oscillator strengths summed over each degenerate level
+    levels = np.split(np.arange(144), np.where(np.diff(w) > 1e-6)[0] + 1)
+    for comp, col in ((0, 1), (1, 2), (2, 3)):
+        fr = np.abs(X.T @ dip[comp]) ** 2
+        for lv in levels:
+            assert abs(out[lv, col].sum() - fr[lv].sum()) <= 5.1e-5 * len(lv) + 1e-6 * fr[lv].sum()
+    pol = load(os.path.join(d, "bse_opt_diel-pol.dat"))
+    assert pol.shape == (144, 6)
+    for comp, col in ((3, 4), (4, 5)):
+        fr = np.abs(X.T @ dip[comp]) ** 2
+        for lv in levels:
+            assert abs(pol[lv, col].sum() - fr[lv].sum()) <= 5.1e-5 * len(lv) + 1e-6 * fr[lv].sum()
+
+
+def test_sp_optics_sorted_transitions(honeycomb_run):
+    d, m = honeycomb_run
+    say("case driver sp_optics")
+    mesh, k, ref, w, X, dip = oracle_case(m)
+    sp = load(os.path.join(d, "sp_optics.dat"))
+    de = (ref["eig"][:, 1] - ref["eig"][:, 0]).astype(np.float32)
+    order = np.argsort(de, kind="stable")
+    assert np.abs(sp[:, 0] - de[order]).max() <= 5.1e-5
+    # transitions of equal energy may come in either order: compare the
+    # (energy, |d_x|^2, |d_y|^2) rows as sorted sets, on the written 4 decimals
+    ax = (np.abs(dip[0]) ** 2).astype(np.float32)
+    ay = (np.abs(dip[1]) ** 2).astype(np.float32)
+    mine = np.round(np.stack([de, ax, ay], 1).astype(np.float64), 4)
+    mine = mine[np.lexsort(mine.T[::-1])]
+    theirs = sp[:, :3][np.lexsort(sp[:, :3].T[::-1])]
+    assert np.abs(mine - theirs).max() <= 1.01e-4
+
+
+def test_spectra_match_oracle_on_the_written_file(honeycomb_run):
+    d, m = honeycomb_run
+    say("case driver spectra")
+    out = load(os.path.join(d, "bse_opt_diel.dat"))
+    vc = abs(np.dot(np.cross(m.rlat[0], m.rlat[1]), m.rlat[2]))
+    spinf = 2.0                                   # NP model
+    for name, col, diag in (("xx", 1, True), ("yy", 2, True), ("zz", 3, True),
+                            ("xy", 4, False), ("xz", 5, False), ("yz", 6, False)):
+        got = load(os.path.join(d, f"bse_diel_{name}.dat"))
+        exp = O.spectrum(out[:, 0], out[:, col], diag, (12, 12, 1), vc, 3.0, 7.0, 400, 0.05, renorm=True)
+        assert got.shape == (400, 3)
+        assert np.abs(got[:, 0] - exp[:, 0]).max() < 1e-12
+        scale = max(1.0, np.abs(exp[:, 1:]).max() * spinf)
+        # the driver prints float32 values list-directed
+        assert np.abs(got[:, 1] - np.float32(spinf * exp[:, 1])).max() < 1e-6 * scale, name
+        assert np.abs(got[:, 2] - np.float32(spinf * exp[:, 2])).max() < 1e-6 * scale, name
+
+
+def test_optical_constants_and_lifetimes(honeycomb_run):
+    d, m = honeycomb_run
+    say("case driver optical constants")
+    e = {c: load(os.path.join(d, f"bse_diel_{c}.dat")) for c in ("xx", "yy", "zz", "xy", "xz", "yz")}
+    comps = ("xx", "yy", "zz", "xy", "xz", "yz")
+    e1 = np.stack([e[c][:, 1] for c in comps], 1)
+    e2 = np.stack([e[c][:, 2] for c in comps], 1)
+    en = e["xx"][:, 0]
+    mod = np.sqrt(e1 * e1 + e2 * e2)
+    n = np.sqrt((mod + e1) / 2)
+    kap = np.sqrt((mod - e1) / 2)
+    hc = float(np.float32(19.746e-6))
+    expect = {
+        "bse_refractive_index.dat": n,
+        "bse_extinction_coef.dat": kap,
+        "bse_reflectibility.dat": ((n - 1) ** 2 + kap ** 2) / ((n + 1) ** 2 + kap ** 2),
+        "bse_absorption_coef.dat": (float(np.sqrt(np.float32(2.0))) * en)[:, None] * (1.0 / hc) * np.sqrt(mod - e1),
+        "bse_en_loss_func.dat": e2 / (e1 * e1 + e2 * e2),
+    }
+    for fname, val in expect.items():
+        got = load(os.path.join(d, fname))
+        assert got.shape == (400, 7)
+        ok = np.isfinite(val)
+        assert np.abs(got[:, 0] - en).max() <= 5.1e-5
+        assert np.abs(got[:, 1:][ok] - val[ok]).max() <= 5.1e-5 + 1e-7 * np.abs(val[ok]).max(), fname
+    # 2-D radiative lifetimes from the written exciton energies / strengths
+    out = load(os.path.join(d, "bse_opt_diel.dat"))
+    lt = load(os.path.join(d, "exciton_lifetime.dat"))
+    cfe, hbar = float(np.float32(7.297e-3)), float(np.float32(6.582e-16))
+    a2 = np.linalg.norm(np.cross(m.rlat[0], m.rlat[1]))
+    f = out[:, 1]
+    sel = f > 1e-3
+    ref = (144 * a2 * hbar) / ((8.0 * PI32 * cfe * out[:, 0]) * f)
+    assert np.allclose(lt[sel, 2], ref[sel], rtol=1e-3)

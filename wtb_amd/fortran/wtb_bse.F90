@@ -1,0 +1,696 @@
+! Exciton (BSE) driver on the gfx950 library: the reference's input cards and
+! output files around the hot path, with the hot calls going through the
+! ISO_C_BINDING interface of module wtb_hip.
+!
+!   wtb_bse < input            (cards as in src/module_input.F90:49-348)
+!
+! Files written in OUTPUT (src/module_bse.F90:2023-2594, 835-1036, 3271-3349,
+! 3766-3931): log_bse-diel.dat, sp_optics.dat, sp_optics-pol.dat,
+! bse_opt_diel.dat, bse_opt_diel-pol.dat and, with SPEC=T, bse_diel_{xx,xy,
+! xz,yy,yz,zz}.dat, bse_refractive_index.dat, bse_extinction_coef.dat,
+! bse_reflectibility.dat, bse_absorption_coef.dat, bse_en_loss_func.dat,
+! exciton_lifetime.dat.
+!
+! Cards of this driver beyond the reference's: NEV= (eigenpairs, default 100
+! as the reference's SLEPc call), PRECISION= (FP64 | MIXED | FP32), KERNEL=
+! (REF | TEXTBOOK overlap convention), F32_COMPAT= (T: float32 phases as the
+! reference), DEVICE= (HIP ordinal).  Rows of bse_opt_diel.dat beyond NEV hold
+! zeros (the reference leaves them unwritten).
+
+module wtb_cards
+    implicit none
+    integer :: ngrid(3) = [1, 1, 1], nc = 1, nv = 1, nev = 100, device = 0
+    double precision :: ebse0 = 0.0d0, ebsef = 3.0d0, numbse = 6001.0d0
+    double precision :: cshift = 0.01d0, ktol = 0.001d0, rk = 0.0d0
+    double precision :: mshift(3) = 0.0d0
+    double precision :: ediel(3) = 1.0d0, ez = 1.0d0, w = 0.0d0, lc = 1.0d0, r0 = 1.0d0
+    character(len=70) :: params = "", outputfolder = "./", meshtype = "MKH"
+    character(len=5) :: coultype = "V3D", sysdim = "3D"
+    character(len=8) :: precision = "FP64", kernel = "REF"
+    logical :: bse = .false., spec = .false., dtfull = .false., cpol = .false., renorm = .true.
+    logical :: compat = .true.
+contains
+    ! "CARD= value" lines from standard input, defaults of module_input.F90:53-120
+    subroutine read_cards()
+        character(len=70) :: a, b
+        integer :: erro
+        do
+            read(*, *, iostat=erro) a, b
+            if (erro /= 0) exit
+            select case (a)
+            case ("PARAMS_FILE=");  params = b
+            case ("OUTPUT=");       outputfolder = b
+            case ("MESH_TYPE=");    meshtype = trim(adjustl(b))
+            case ("RK=");           read(b, *) rk
+            case ("COULOMB_POT=");  coultype = b
+            case ("SYSDIM=");       read(b, *) sysdim
+            case ("NGX=");          read(b, *) ngrid(1)
+            case ("NGY=");          read(b, *) ngrid(2)
+            case ("NGZ=");          read(b, *) ngrid(3)
+            case ("NBANDSC=");      read(b, *) nc
+            case ("NBANDSV=");      read(b, *) nv
+            case ("SHIFT_1=");      read(b, *) mshift(1)
+            case ("SHIFT_2=");      read(b, *) mshift(2)
+            case ("SHIFT_3=");      read(b, *) mshift(3)
+            case ("ENSPECI=");      read(b, *) ebse0
+            case ("ENSPECF=");      read(b, *) ebsef
+            case ("NESPEC=");       read(b, *) numbse
+            case ("CSHIFT=");       read(b, *) cshift
+            case ("KTOL=");         read(b, *) ktol
+            case ("R_0=");          read(b, *) r0
+            case ("LC=");           read(b, *) lc
+            case ("EDIEL_Z=");      read(b, *) ez
+            case ("W_COUL=");       read(b, *) w
+            case ("EDIEL_T=");      read(b, *) ediel(1)
+            case ("EDIEL=");        read(b, *) ediel(2)
+            case ("EDIEL_B=");      read(b, *) ediel(3)
+            case ("DTDIAG=");       read(b, *) dtfull
+            case ("CPOL=");         read(b, *) cpol
+            case ("RNMD=");         read(b, *) renorm
+            case ("BSE=");          read(b, *) bse
+            case ("SPEC=");         read(b, *) spec
+            case ("NEV=");          read(b, *) nev
+            case ("PRECISION=");    precision = trim(adjustl(b))
+            case ("KERNEL=");       kernel = trim(adjustl(b))
+            case ("F32_COMPAT=");   read(b, *) compat
+            case ("DEVICE=");       read(b, *) device
+            case default
+                continue
+            end select
+        end do
+    end subroutine read_cards
+end module wtb_cards
+
+
+module wtb_tbfile
+    implicit none
+    character(len=2) :: systype
+    double precision :: scs, efermi, rlat(3, 3)
+    integer :: norb, nvec
+    integer, allocatable :: ffactor(:), rvec(:, :)
+    double precision, allocatable :: hop(:, :, :), ihop(:, :, :)
+contains
+    ! TB parameter file (hamiltonian_input_read, module_hamiltonian.F90:35-194):
+    ! systype, scissor, E_F, lattice rows, then one block (or an up and a down
+    ! block for SP, merged block-diagonally) of
+    !   label / norb / nvec / ffactor(nvec) / nvec*norb*norb lines "R(3) m n re im"
+    ! with the element of line (i, j, k) stored at (i, k, j).
+    subroutine read_tb(fname)
+        character(len=*), intent(in) :: fname
+        integer :: u, erro, nb(2), nr(2), nblk, s, i, j, k, ob, orr, flag
+        character(len=70) :: label
+        type blk
+            integer, allocatable :: ff(:), rv(:, :)
+            double precision, allocatable :: h(:, :, :), ih(:, :, :)
+        end type blk
+        type(blk) :: b(2)
+        open(newunit=u, file=trim(fname), status='old', iostat=erro)
+        if (erro /= 0) stop "Error opening wannier90 hr input file"
+        read(u, *) systype
+        read(u, *) scs
+        read(u, *) efermi
+        read(u, *) rlat(1, 1), rlat(1, 2), rlat(1, 3)
+        read(u, *) rlat(2, 1), rlat(2, 2), rlat(2, 3)
+        read(u, *) rlat(3, 1), rlat(3, 2), rlat(3, 3)
+        nblk = 1
+        if (systype == "SP") nblk = 2
+        do s = 1, nblk
+            read(u, *) label
+            read(u, *) nb(s)
+            read(u, *) nr(s)
+            allocate(b(s)%ff(nr(s)), b(s)%rv(nr(s), 3))
+            allocate(b(s)%h(nr(s), nb(s), nb(s)), b(s)%ih(nr(s), nb(s), nb(s)))
+            read(u, *) (b(s)%ff(i), i = 1, nr(s))
+            do i = 1, nr(s)
+                do j = 1, nb(s)
+                    do k = 1, nb(s)
+                        read(u, *) b(s)%rv(i, 1), b(s)%rv(i, 2), b(s)%rv(i, 3), flag, flag, &
+                                   b(s)%h(i, k, j), b(s)%ih(i, k, j)
+                    end do
+                end do
+            end do
+        end do
+        close(u)
+        norb = sum(nb(1:nblk))
+        nvec = sum(nr(1:nblk))
+        allocate(ffactor(nvec), rvec(nvec, 3), hop(nvec, norb, norb), ihop(nvec, norb, norb))
+        hop = 0.0d0
+        ihop = 0.0d0
+        ob = 0
+        orr = 0
+        do s = 1, nblk
+            ffactor(orr + 1:orr + nr(s)) = b(s)%ff
+            rvec(orr + 1:orr + nr(s), :) = b(s)%rv
+            hop(orr + 1:orr + nr(s), ob + 1:ob + nb(s), ob + 1:ob + nb(s)) = b(s)%h
+            ihop(orr + 1:orr + nr(s), ob + 1:ob + nb(s), ob + 1:ob + nb(s)) = b(s)%ih
+            ob = ob + nb(s)
+            orr = orr + nr(s)
+        end do
+    end subroutine read_tb
+
+    ! Gamma-centred Monkhorst-Pack mesh, i outer / l inner, plus the shift in
+    ! reciprocal coordinates (monhkhorst_pack + recvec, module_kmesh.F90:338-407;
+    ! pi is the float32 acos(-1.) of the reference)
+    subroutine mp_mesh(ng, shift, kpt)
+        integer, intent(in) :: ng(3)
+        double precision, intent(in) :: shift(3)
+        double precision, intent(out) :: kpt(:, :)
+        double precision, parameter :: pi = acos(-1._4)
+        double precision :: b1(3), b2(3), b3(3), v23(3), v31(3), v12(3), vol, ks(3)
+        integer :: i, j, l, n
+        v23 = cross(rlat(2, :), rlat(3, :))
+        v31 = cross(rlat(3, :), rlat(1, :))
+        v12 = cross(rlat(1, :), rlat(2, :))
+        vol = abs((rlat(1, 1) * v23(1)) + (rlat(1, 2) * v23(2)) + (rlat(1, 3) * v23(3)))
+        b1 = ((2.0 * pi) / vol) * v23
+        b2 = ((2.0 * pi) / vol) * v31
+        b3 = ((2.0 * pi) / vol) * v12
+        ks = b1 * shift(1) + b2 * shift(2) + b3 * shift(3)
+        n = 0
+        do i = 0, ng(1) - 1
+            do j = 0, ng(2) - 1
+                do l = 0, ng(3) - 1
+                    n = n + 1
+                    kpt(n, :) = (b1 / dble(ng(1))) * dble(i) + (b2 / dble(ng(2))) * dble(j) &
+                                + (b3 / dble(ng(3))) * dble(l) + ks
+                end do
+            end do
+        end do
+    end subroutine mp_mesh
+
+    ! mesh from a k-point density (rkmesh / rkmesh2D, module_kmesh.F90:12-53):
+    ! n_i = max(1, int(rk |b_i| / 2 pi + 1/2)), n_3 = 1 in 2-D
+    subroutine rk_grid(rk, twod, ng)
+        double precision, intent(in) :: rk
+        logical, intent(in) :: twod
+        integer, intent(out) :: ng(3)
+        double precision, parameter :: pi = acos(-1._4)
+        double precision :: b(3, 3), vol, vsize
+        integer :: i
+        b(1, :) = cross(rlat(2, :), rlat(3, :))
+        b(2, :) = cross(rlat(3, :), rlat(1, :))
+        b(3, :) = cross(rlat(1, :), rlat(2, :))
+        vol = abs((rlat(1, 1) * b(1, 1)) + (rlat(1, 2) * b(1, 2)) + (rlat(1, 3) * b(1, 3)))
+        do i = 1, 3
+            b(i, :) = ((2.0 * pi) / vol) * b(i, :)
+            vsize = sqrt(b(i, 1) * b(i, 1) + b(i, 2) * b(i, 2) + b(i, 3) * b(i, 3)) / (2 * pi)
+            ng(i) = int(max(1.0d0, (rk * vsize) + 0.5))
+        end do
+        if (twod) ng(3) = 1
+    end subroutine rk_grid
+
+    pure function cross(a, b) result(c)
+        double precision, intent(in) :: a(3), b(3)
+        double precision :: c(3)
+        c(1) = a(2) * b(3) - a(3) * b(2)
+        c(2) = a(3) * b(1) - a(1) * b(3)
+        c(3) = a(1) * b(2) - a(2) * b(1)
+    end function cross
+
+    ! |(a1 x a2) . a3| (vcell3D, module_math.F90:154-162)
+    double precision function vcell3d()
+        double precision :: v(3)
+        v = cross(rlat(1, :), rlat(2, :))
+        vcell3d = abs(v(1) * rlat(3, 1) + v(2) * rlat(3, 2) + v(3) * rlat(3, 3))
+    end function vcell3d
+
+    ! |a1 x a2| (vcell2D, module_math.F90:143-150)
+    double precision function vcell2d()
+        double precision :: v(3)
+        v = cross(rlat(1, :), rlat(2, :))
+        vcell2d = sqrt(v(1) * v(1) + v(2) * v(2) + v(3) * v(3))
+    end function vcell2d
+end module wtb_tbfile
+
+
+module wtb_spectra
+    implicit none
+contains
+    ! 10 (E_N - E_1) / N written as the reference's telescoping sum
+    ! (avgsme, module_optics.F90:313-335)
+    double precision function avgsme(e)
+        double precision, intent(in) :: e(:)
+        integer :: i
+        avgsme = 0.0d0
+        do i = 2, size(e)
+            avgsme = avgsme + e(i) - e(i - 1)
+        end do
+        avgsme = 10.0 * (avgsme / dble(size(e)))
+    end function avgsme
+
+    ! Lorentzian dielectric function at one photon energy (optdiel,
+    ! module_optics.F90:171-241): width = CSHIFT + avgsme,
+    ! prefactor 180.90 / (N_k V_cell) with the 3-D cell volume
+    subroutine optdiel(diagonal, vc, ngrid, elux, exciton, fosc, sme, rpart, ipart)
+        logical, intent(in) :: diagonal
+        double precision, intent(in) :: vc, elux, exciton(:), fosc(:), sme
+        integer, intent(in) :: ngrid(3)
+        double precision, intent(out) :: rpart, ipart
+        double precision, parameter :: gama = 180.90_4          ! float32 literal, as the reference
+        double precision :: smef, caux, eaux, delta
+        integer :: i
+        caux = gama / (real(ngrid(1) * ngrid(2) * ngrid(3)) * vc)
+        delta = 0.0d0
+        if (diagonal) delta = 1.0d0
+        smef = avgsme(exciton) + sme
+        rpart = 0.0d0
+        ipart = 0.0d0
+        do i = 1, size(exciton)
+            eaux = ((elux - exciton(i))**2) + (smef**2)
+            rpart = rpart + fosc(i) * ((exciton(i) - elux) / eaux)
+            ipart = ipart + fosc(i) * (smef / eaux)
+        end do
+        ipart = ipart * caux
+        rpart = delta + rpart * caux
+    end subroutine optdiel
+
+    ! Im(eps) renormalisation (imagrenorm, module_optics.F90:338-401): below
+    ! E_1 - 0.15 eV the imaginary part is zeroed, above it the value at the
+    ! first point past that energy is subtracted and the result clamped at 0
+    subroutine imagrenorm(exciton, dielf)
+        double precision, intent(in) :: exciton(:)
+        double precision, intent(inout) :: dielf(:, :)
+        double precision, parameter :: factor = 0.15_4        ! float32 literal, as the reference
+        double precision :: aux, base
+        integer :: i, ibegin, n
+        n = size(dielf, 1)
+        aux = exciton(1) - factor
+        if (aux <= 0.0d0) return
+        ibegin = 1
+        do i = 1, n
+            if (dielf(i, 1) <= aux) then
+                ibegin = i
+            else
+                exit
+            end if
+        end do
+        ! every point below E_1 - 0.15: the reference reads past the grid here
+        if (ibegin >= n) then
+            dielf(:, 3) = 0.0d0
+            return
+        end if
+        base = dielf(ibegin + 1, 3)
+        dielf(1:ibegin, 3) = 0.0d0
+        do i = ibegin + 1, n
+            dielf(i, 3) = max(dielf(i, 3) - base, 0.0d0)
+        end do
+    end subroutine imagrenorm
+
+    ! optical constants from (e1, e2) (module_bse.F90:5075-5124)
+    double precision function refractive(e1, e2)
+        double precision, intent(in) :: e1, e2
+        refractive = sqrt((sqrt(e1 * e1 + e2 * e2) + e1) / 2.0)
+    end function refractive
+
+    double precision function extinction(e1, e2)
+        double precision, intent(in) :: e1, e2
+        extinction = sqrt((sqrt(e1 * e1 + e2 * e2) - e1) / 2.0)
+    end function extinction
+
+    double precision function reflectivity(e1, e2)
+        double precision, intent(in) :: e1, e2
+        double precision :: n, k
+        n = refractive(e1, e2)
+        k = extinction(e1, e2)
+        reflectivity = ((n - 1.0) * (n - 1.0) + k * k) / ((n + 1.0) * (n + 1.0) + k * k)
+    end function reflectivity
+
+    ! alpha = sqrt(2) E / (hbar c) sqrt(|eps| - e1), hc in eV cm (float32 literal)
+    double precision function absorption(e1, e2, efoton)
+        double precision, intent(in) :: e1, e2, efoton
+        double precision, parameter :: hc = 19.746E-06_4       ! float32 literal, as the reference
+        absorption = (sqrt(2.0) * efoton) * (1.0 / hc) * sqrt(sqrt(e1 * e1 + e2 * e2) - e1)
+    end function absorption
+
+    double precision function energy_loss(e1, e2)
+        double precision, intent(in) :: e1, e2
+        energy_loss = e2 / (e1 * e1 + e2 * e2)
+    end function energy_loss
+
+    ! radiative lifetime of an exciton of energy e and oscillator strength f
+    ! (exclft, module_bse.F90:326-370)
+    double precision function lifetime(sysdim, ngrid, vc2d, a11, f, e)
+        character(len=*), intent(in) :: sysdim
+        integer, intent(in) :: ngrid(3)
+        double precision, intent(in) :: vc2d, a11, f, e
+        ! float32 literals, as the reference
+        double precision, parameter :: cfe = 7.297E-3_4, lspd = 299.792E16_4, hbar = 6.582E-16_4
+        double precision, parameter :: pi = acos(-1._4)
+        double precision :: nk
+        nk = dble(ngrid(1) * ngrid(2) * ngrid(3))
+        select case (trim(sysdim))
+        case ("3D")
+            lifetime = 1.0 / ((((4.0 * cfe) * (e * e * e)) * (f / nk)) / (3.0 * (lspd * lspd) * (hbar * hbar * hbar)))
+        case ("2D")
+            lifetime = (nk * vc2d * hbar) / ((8.0 * pi * cfe * e) * f)
+        case ("1D")
+            lifetime = (nk * a11 * hbar * hbar * lspd) / ((2.0 * pi * cfe * e * e) * f)
+        case default
+            write(*, *) "Wrong System dimension"
+            stop
+        end select
+    end function lifetime
+
+    ! stable ascending order of key (the reference's bubble sort by one column)
+    subroutine stable_order(key, idx)
+        real, intent(in) :: key(:)
+        integer, intent(out) :: idx(:)
+        integer, allocatable :: tmp(:)
+        integer :: n, width, lo, mid, hi, i, j, k
+        n = size(key)
+        allocate(tmp(n))
+        idx = [(i, i = 1, n)]
+        width = 1
+        do while (width < n)
+            lo = 1
+            do while (lo <= n)
+                mid = min(lo + width - 1, n)
+                hi = min(lo + 2 * width - 1, n)
+                i = lo
+                j = mid + 1
+                k = lo
+                do while (i <= mid .and. j <= hi)
+                    if (key(idx(j)) < key(idx(i))) then
+                        tmp(k) = idx(j)
+                        j = j + 1
+                    else
+                        tmp(k) = idx(i)
+                        i = i + 1
+                    end if
+                    k = k + 1
+                end do
+                do while (i <= mid)
+                    tmp(k) = idx(i); i = i + 1; k = k + 1
+                end do
+                do while (j <= hi)
+                    tmp(k) = idx(j); j = j + 1; k = k + 1
+                end do
+                lo = lo + 2 * width
+            end do
+            idx = tmp
+            width = 2 * width
+        end do
+    end subroutine stable_order
+end module wtb_spectra
+
+
+program wtb_bse
+    use iso_c_binding
+    use wtb_hip
+    use wtb_cards
+    use wtb_tbfile
+    use wtb_spectra
+    implicit none
+    type(c_ptr) :: ctx
+    type(wtb_coulomb) :: coul
+    type(wtb_report), target :: rep
+    integer :: nk, dimbse, i, k, c, v, ib, nevs, flags, prec, conv, erro, kind
+    double precision, allocatable :: kpt(:, :)
+    double precision, allocatable, target :: eigv(:, :), ws(:)
+    integer, allocatable, target :: nocpk(:)
+    complex(c_double_complex), allocatable, target :: dip(:, :)
+    double precision, allocatable :: act(:, :)
+    real, allocatable :: vecres(:, :)
+    integer, allocatable :: order(:)
+    double precision :: egap, spinf
+
+    call read_cards()
+    if (.not. bse) stop "wtb_bse: only the BSE=T path is provided"
+    call read_tb(params)
+    select case (trim(meshtype))
+    case ("MKH")
+        continue
+    case ("RK3D", "RK2D")
+        call rk_grid(rk, trim(meshtype) == "RK2D", ngrid)
+    case default
+        stop "wtb_bse: MESH_TYPE= MKH, RK3D or RK2D"
+    end select
+
+    nk = ngrid(1) * ngrid(2) * ngrid(3)
+    dimbse = nk * nc * nv
+    nevs = min(nev, dimbse)
+    allocate(kpt(nk, 3), eigv(nk, nc + nv), nocpk(nk), ws(dimbse), dip(dimbse, 5), act(dimbse, 8))
+    call mp_mesh(ngrid, mshift, kpt)
+
+    flags = WTB_FLAG_GAUGE_LAPACK
+    if (compat) flags = flags + WTB_FLAG_COMPAT_F32
+    select case (trim(precision))
+    case ("FP64");  prec = WTB_FP64
+    case ("MIXED"); prec = WTB_MIXED
+    case ("FP32");  prec = WTB_FP32
+    case default;   stop "wtb_bse: PRECISION= FP64, MIXED or FP32"
+    end select
+    conv = WTB_CONV_REFERENCE
+    if (trim(kernel) == "TEXTBOOK") conv = WTB_CONV_TEXTBOOK
+    select case (trim(coultype))
+    case ("V2DK");  kind = WTB_V2DK
+    case ("V3D");   kind = WTB_V3D
+    case ("V3DL");  kind = WTB_V3DL
+    case ("V2DT");  kind = WTB_V2DT
+    case ("V2DT2"); kind = WTB_V2DT2
+    case ("V2DOH"); kind = WTB_V2DOH
+    case ("V2DRK"); kind = WTB_V2DRK
+    case ("V1DT");  kind = WTB_V1DT
+    case ("V0DT");  kind = WTB_V0DT
+    case default
+        write(*, *) "Wrong Coulomb Potential"
+        stop
+    end select
+    coul = wtb_coulomb(kind, r0, ediel, ez, w, lc, ktol)
+
+    ! ---- GPU path: eigsys loop, optsp, matrizelbse, slepc_solver, dielbse*
+    call wtb_check(ctx, wtb_create(ctx, int(device, c_int)), "wtb_create")
+    call wtb_check(ctx, wtb_set_tb_model(ctx, norb, nvec, rvec, ffactor, hop, ihop, rlat, efermi, scs, flags), &
+                   "wtb_set_tb_model")
+    call wtb_check(ctx, wtb_set_kpoints(ctx, nk, kpt, ngrid), "wtb_set_kpoints")
+    call wtb_check(ctx, wtb_hk_eig(ctx, nv, nc, c_loc(eigv), c_null_ptr, c_loc(nocpk)), "wtb_hk_eig")
+    call wtb_check(ctx, wtb_optics(ctx, cshift, c_loc(dip)), "wtb_optics")
+    call wtb_check(ctx, wtb_bse_build(ctx, coul, prec, conv), "wtb_bse_build")
+    ws = 0.0d0
+    call wtb_check(ctx, wtb_bse_solve(ctx, nevs, 1.0d-8, 200, ws, c_null_ptr, c_loc(rep)), "wtb_bse_solve")
+    act = 0.0d0
+    call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_X, WTB_X, act(:, 1)), "wtb_oscillator xx")
+    call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_Y, WTB_Y, act(:, 2)), "wtb_oscillator yy")
+    call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_Z, WTB_Z, act(:, 3)), "wtb_oscillator zz")
+    if (dtfull) then
+        call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_X, WTB_Y, act(:, 4)), "wtb_oscillator xy")
+        call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_X, WTB_Z, act(:, 5)), "wtb_oscillator xz")
+        call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_Y, WTB_Z, act(:, 6)), "wtb_oscillator yz")
+    end if
+    if (cpol) then
+        call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_SP, WTB_SP, act(:, 7)), "wtb_oscillator sp")
+        call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_SM, WTB_SM, act(:, 8)), "wtb_oscillator sm")
+    end if
+    call wtb_check(ctx, wtb_destroy(ctx), "wtb_destroy")
+
+    ! ---- log_bse-diel.dat (module_bse.F90:2084-2111, 2271-2273, 2470-2474)
+    egap = 50.0
+    do k = 1, nk
+        egap = min(egap, eigv(k, nv + 1) - eigv(k, nv))
+    end do
+    open(unit=300, file=trim(outputfolder)//"/log_bse-diel.dat", status='unknown', iostat=erro)
+    if (erro /= 0) stop "Error opening log_bse-diel output file"
+    write(300, *) 'grid:', ngrid(1), ngrid(2), ngrid(3)
+    write(300, "(A14,1E15.4)") 'ktol-coulomb:', ktol
+    write(300, "(A13,3F15.4)") 'kmesh shift:', mshift(1), mshift(2), mshift(3)
+    write(300, *) 'conduction bands, nc:', nc, '  ', 'valence bands, nv:', nv
+    write(300, *) 'direct gap:', egap
+    write(300, *) 'exciton Hamiltonian diagonalized'
+    write(300, *) "exciton ground state", ws(1)
+    write(300, *) 'solver: nev', nevs, ' iterations', rep%iterations, ' converged', rep%nconv, &
+                  ' max rel residual', rep%max_rel_residual, ' solve ms', rep%t_solve_ms
+    close(300)
+
+    ! ---- sp_optics.dat / sp_optics-pol.dat: transitions sorted by energy
+    ! (module_bse.F90:2300-2375); float32 columns as the reference's vecres
+    allocate(vecres(dimbse, 15), order(dimbse))
+    ib = 0
+    do k = 1, nk
+        do c = 1, nc
+            do v = 1, nv
+                ib = ib + 1
+                vecres(ib, 1:3) = real(kpt(k, :))
+                vecres(ib, 4) = real(nocpk(k))
+                vecres(ib, 5) = real(nocpk(k) - nv + (nv + c))
+                vecres(ib, 6) = real(nocpk(k) - nv + v)
+                vecres(ib, 7) = real(eigv(k, nv + c) - eigv(k, v))
+                vecres(ib, 8) = real(dip(ib, 1) * conjg(dip(ib, 1)))
+                vecres(ib, 9) = real(dip(ib, 2) * conjg(dip(ib, 2)))
+                vecres(ib, 10) = real(dip(ib, 3) * conjg(dip(ib, 3)))
+                vecres(ib, 11) = real(dip(ib, 1) * conjg(dip(ib, 2)))
+                vecres(ib, 12) = real(dip(ib, 1) * conjg(dip(ib, 3)))
+                vecres(ib, 13) = real(dip(ib, 2) * conjg(dip(ib, 3)))
+                vecres(ib, 14) = real(dip(ib, 4) * conjg(dip(ib, 4)))
+                vecres(ib, 15) = real(dip(ib, 5) * conjg(dip(ib, 5)))
+            end do
+        end do
+    end do
+    call stable_order(vecres(:, 7), order)
+    open(unit=401, file=trim(outputfolder)//"/sp_optics.dat", status='unknown', iostat=erro)
+    if (erro /= 0) stop "Error opening sp_optics output file"
+    open(unit=403, file=trim(outputfolder)//"/sp_optics-pol.dat", status='unknown', iostat=erro)
+    if (erro /= 0) stop "Error opening sp_optics-pol output file"
+    write(401, *) "#", "  ", "energy", "  ", "xx", "  ", "yy", "  ", "zz", "  ", "xy", "  ", "xz", "  ", "yz"
+    write(403, *) "#", "  ", "energy", "  ", "xx", "  ", "yy", "  ", "zz", "  ", "sp", "  ", "sm"
+    do i = 1, dimbse
+        ib = order(i)
+        write(401, "(7F15.4)") vecres(ib, 7:13)
+        write(403, "(6F15.4)") vecres(ib, 7), vecres(ib, 8), vecres(ib, 9), vecres(ib, 10), vecres(ib, 14), &
+                               vecres(ib, 15)
+    end do
+    close(401)
+    close(403)
+
+    ! ---- bse_opt_diel.dat / bse_opt_diel-pol.dat (module_bse.F90:2140-2141, 2548-2569)
+    open(unit=301, file=trim(outputfolder)//"/bse_opt_diel.dat", status='unknown', iostat=erro)
+    if (erro /= 0) stop "Error opening bse_opt_diel output file"
+    open(unit=302, file=trim(outputfolder)//"/bse_opt_diel-pol.dat", status='unknown', iostat=erro)
+    if (erro /= 0) stop "Error opening output file"
+    write(301, *) "#", "  ", "exciton energy", "  ", "xx", "  ", "yy", "  ", "zz", " ", "xy", "  ", "xz", "  ", "yz"
+    write(302, *) "#", "  ", "exciton energy", "  ", "xx", "  ", "yy", "  ", "zz", " ", "sp", "  ", "sm"
+    do i = 1, dimbse
+        write(301, "(7F15.4)") ws(i), act(i, 1), act(i, 2), act(i, 3), act(i, 4), act(i, 5), act(i, 6)
+        if (cpol) write(302, "(6F15.4)") ws(i), act(i, 1), act(i, 2), act(i, 3), act(i, 7), act(i, 8)
+    end do
+    close(301)
+    close(302)
+
+    ! ---- spectra from the written (4-decimal) file, optical constants, lifetimes
+    if (spec) then
+        spinf = 1.0
+        if (systype == "NP") spinf = 2.0
+        call diel_files(spinf)
+        call optprop_files()
+        call lifetime_file()
+    end if
+    write(*, '(A,I0,A,I0,A,F12.6,A,ES10.3)') 'wtb_bse: N=', dimbse, ' nev=', nevs, ' E1=', ws(1), &
+        ' eV, max rel residual ', rep%max_rel_residual
+
+contains
+
+    ! bsedielraw (module_bse.F90:835-1036): re-reads bse_opt_diel.dat
+    subroutine diel_files(spinf)
+        double precision, intent(in) :: spinf
+        character(len=2) :: aread
+        double precision, allocatable :: e(:), f(:, :), d(:, :, :)
+        double precision :: vc, elux
+        integer :: u, j, m, npts
+        character(len=2), parameter :: tag(6) = ["xx", "yy", "zz", "xy", "xz", "yz"]
+        integer, parameter :: unit_of(6) = [300, 303, 305, 301, 302, 304]
+        npts = int(numbse)
+        allocate(e(dimbse), f(dimbse, 6), d(npts, 3, 6))
+        open(newunit=u, file=trim(outputfolder)//"/bse_opt_diel.dat", status='unknown', iostat=erro)
+        if (erro /= 0) stop "Error opening bse_opt_diel input file"
+        read(u, *) aread
+        do j = 1, dimbse
+            read(u, *) e(j), f(j, 1), f(j, 2), f(j, 3), f(j, 4), f(j, 5), f(j, 6)
+        end do
+        close(u)
+        vc = vcell3d()
+        do j = 1, npts
+            elux = ebse0 + dble(((ebsef - ebse0) * (j - 1)) / (numbse - 1.))
+            do m = 1, 6
+                d(j, 1, m) = elux
+                call optdiel(m <= 3, vc, ngrid, elux, e, f(:, m), cshift, d(j, 2, m), d(j, 3, m))
+            end do
+        end do
+        if (renorm) then
+            do m = 1, 6
+                call imagrenorm(e, d(:, :, m))
+            end do
+        end if
+        do m = 1, 6
+            open(unit=unit_of(m), file=trim(outputfolder)//"/bse_diel_"//tag(m)//".dat", status='unknown', &
+                 iostat=erro)
+            if (erro /= 0) stop "Error opening bse_diel output file"
+            write(unit_of(m), *) "#", "  ", "energy", "  ", "real", "  ", "imag"
+            do j = 1, npts
+                write(unit_of(m), *) d(j, 1, m), real(spinf * d(j, 2, m)), real(spinf * d(j, 3, m))
+            end do
+            close(unit_of(m))
+        end do
+    end subroutine diel_files
+
+    ! bseoptprop (module_bse.F90:3766-3931): re-reads the six bse_diel files
+    subroutine optprop_files()
+        character(len=2) :: aread
+        character(len=2), parameter :: tag(6) = ["xx", "yy", "zz", "xy", "xz", "yz"]
+        character(len=25), parameter :: outname(5) = [character(len=25) :: "bse_refractive_index.dat", &
+            "bse_extinction_coef.dat", "bse_reflectibility.dat", "bse_absorption_coef.dat", "bse_en_loss_func.dat"]
+        double precision, allocatable :: en(:), re(:, :), im(:, :)
+        double precision :: flag, val(6)
+        integer :: u(6), uo(5), j, m, q, npts
+        npts = int(numbse)
+        allocate(en(npts), re(npts, 6), im(npts, 6))
+        do m = 1, 6
+            open(newunit=u(m), file=trim(outputfolder)//"/bse_diel_"//tag(m)//".dat", status='old', iostat=erro)
+            if (erro /= 0) stop "Error opening bse_diel input file"
+            read(u(m), *) aread
+        end do
+        do j = 1, npts
+            do m = 1, 6
+                read(u(m), *) flag, re(j, m), im(j, m)
+                if (m == 1) en(j) = flag
+            end do
+        end do
+        do m = 1, 6
+            close(u(m))
+        end do
+        do q = 1, 5
+            open(newunit=uo(q), file=trim(outputfolder)//"/"//trim(outname(q)), status='unknown', iostat=erro)
+            if (erro /= 0) stop "Error opening optical property output file"
+            write(uo(q), *) "#", "  ", "energy", "  ", "xx", "  ", "yy", "  ", "zz", "  ", "xy", "  ", "xz", "  ", "yz"
+        end do
+        do j = 1, npts
+            do m = 1, 6
+                val(m) = refractive(re(j, m), im(j, m))
+            end do
+            write(uo(1), "(7F15.4)") en(j), val
+            do m = 1, 6
+                val(m) = extinction(re(j, m), im(j, m))
+            end do
+            write(uo(2), "(7F15.4)") en(j), val
+            do m = 1, 6
+                val(m) = reflectivity(re(j, m), im(j, m))
+            end do
+            write(uo(3), "(7F15.4)") en(j), val
+            do m = 1, 6
+                val(m) = absorption(re(j, m), im(j, m), en(j))
+            end do
+            write(uo(4), "(7F15.4)") en(j), val
+            do m = 1, 6
+                val(m) = energy_loss(re(j, m), im(j, m))
+            end do
+            write(uo(5), "(7F15.4)") en(j), val
+        end do
+        do q = 1, 5
+            close(uo(q))
+        end do
+    end subroutine optprop_files
+
+    ! lifetime (module_bse.F90:3271-3349): re-reads bse_opt_diel.dat
+    subroutine lifetime_file()
+        character(len=70) :: cflag
+        double precision :: fo(7), lft(4), vc2, ftot
+        integer :: u, uo, j
+        vc2 = vcell2d()
+        open(newunit=u, file=trim(outputfolder)//"/bse_opt_diel.dat", status='old', iostat=erro)
+        if (erro /= 0) stop "Error opening bse_opt_diel input file"
+        open(newunit=uo, file=trim(outputfolder)//"/exciton_lifetime.dat", status='unknown', iostat=erro)
+        if (erro /= 0) stop "Error opening exciton_lifetime output file"
+        read(u, *) cflag
+        write(uo, *) "#energy(eV) sum-lft(s)  x-lft(s) y-lft(s) z-lft(s)"
+        do j = 1, dimbse
+            read(u, *) fo
+            ftot = fo(2) + fo(3) + fo(4)
+            lft(1) = lifetime(sysdim, ngrid, vc2, rlat(1, 1), ftot, fo(1))
+            lft(2) = lifetime(sysdim, ngrid, vc2, rlat(1, 1), fo(2), fo(1))
+            lft(3) = lifetime(sysdim, ngrid, vc2, rlat(1, 1), fo(3), fo(1))
+            lft(4) = lifetime(sysdim, ngrid, vc2, rlat(1, 1), fo(4), fo(1))
+            write(uo, "(1F15.4,4E15.4)") fo(1), lft
+        end do
+        close(u)
+        close(uo)
+    end subroutine lifetime_file
+
+end program wtb_bse
