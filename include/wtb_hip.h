@@ -163,6 +163,25 @@ int wtb_bse_get_block(wtb_ctx *ctx, long i0, long ni, long j0, long nj, double *
 int wtb_bse_matvec(wtb_ctx *ctx, int ncol, const double *X, double *Y, int precision);
 int wtb_bse_diag(wtb_ctx *ctx, double *diag);
 
+/* Gradients of L = sum_n w[n] E_n over the nev states of the last
+ * wtb_bse_solve (Hellmann-Feynman contractions x^H dH x evaluated per (k,k')
+ * tile, chained through the TB eigenvector adjoint).  No counterpart in the
+ * reference; its callers are fitting loops over the screening parameters,
+ * the scissor shift and the TB hoppings.
+ *   w            : double w(nev); states of a degenerate level need equal weights
+ *   g            : scalar gradients
+ *   hop_re_grad, hop_im_grad : double (nvec,norb,norb), dL/dRe H(R)_mn and
+ *                  dL/dIm H(R)_mn with each entry an independent parameter of
+ *                  H(k) = sum_R e^{ik.R} H(R)/deg, or NULL */
+typedef struct {
+    double coulomb[7];       /* dL/d r0, EDIEL_T, EDIEL, EDIEL_B, EDIEL_Z, W_COUL, LC */
+    double scissor;          /* dL/d scissor                                          */
+    double efermi;           /* dL/d E_F (zero while the band selection is fixed)      */
+    double gauge_residual;   /* max |Im u_m^H ubar_m| over the selected bands: 0 for a
+                                loss invariant under TB eigenvector phases             */
+} wtb_grad;
+int wtb_grad_energies(wtb_ctx *ctx, const double *w, wtb_grad *g, double *hop_re_grad, double *hop_im_grad);
+
 /* Drop-in for slepc_solver(HX,EX,HE,NX,NE) (module_slepc_solver.F90:12-97):
  * uploads the caller's full host matrix HX(NX,NX), returns the NE
  * smallest-magnitude eigenpairs.  EX(NE) ascending in |E|; HE(NX,NE). */

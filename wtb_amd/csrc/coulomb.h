@@ -23,6 +23,8 @@ struct CoulombDev {
     double eta, kappa, pb, pt;
     // V2DK
     double v2dk_r0, v2dk_ed, v2dk_q0, v2dk_pref;
+    double v2dk_a0sg;  // a0 sqrt(n1 n2), for the parameter derivatives
+    double pi;         // float32 acos(-1.)
     // V2DT / V0DT
     double rc, cr;
 };

@@ -59,7 +59,9 @@ CoulombDev make_coulomb(const wtb_coulomb &in, const double *rlat, const int *ng
         c.v2dk_ed = ed;
         c.v2dk_pref = c.vbz2d * (c.cic / ed);
         c.v2dk_q0 = c.vbz2d * (c.cic / ed) * (a0 * std::sqrt(g) / (2.0 * pi)) * ((double)1.76f + auxi * 1.0);
+        c.v2dk_a0sg = a0 * std::sqrt(g);
     }
+    c.pi = pi;
     c.rc = 0.5 * a3[2];
     {
         const double s1 = std::sqrt(a1[0] * a1[0] + a1[1] * a1[1] + a1[2] * a1[2]);

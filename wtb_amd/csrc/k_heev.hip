@@ -164,7 +164,7 @@ __device__ int sturm_count(int n, const double *d, const double *e2, double x, d
 }
 
 // Y (n x nvec, column-major, real) = eigenvectors of T for the nvec smallest
-// eigenvalues W (ascending).  work: 5 n doubles per vector.
+// eigenvalues W (ascending).  work: 6 n doubles per vector.
 __global__ void __launch_bounds__(kT)
 tridiag_eig_kernel(int n, int nvec, const double *__restrict__ d, const double *__restrict__ e,
                    double *__restrict__ W, double *__restrict__ Y, double *__restrict__ work) {
@@ -213,13 +213,19 @@ tridiag_eig_kernel(int n, int nvec, const double *__restrict__ d, const double *
         W[k] = sw[k];
     }
     __syncthreads();
-    // inverse iteration, one thread per eigenvector
+    // inverse iteration, one thread per eigenvector.  The per-vector work
+    // arrays are interleaved across threads (element j of vector k at
+    // [j * nvec + k]), so each step of the sequential sweeps is one coalesced
+    // access for the whole wavefront.
     const double eps = 2.2e-16;
     for (int k = tid; k < nvec; k += kT) {
-        double *u = work + (size_t)k * 5 * n;      // U diagonal
-        double *u1 = u + n, *u2 = u1 + n;          // first / second superdiagonal
-        double *l = u2 + n;                        // multipliers
-        double *x = l + n;                         // iterate (piv flags folded into l sign below)
+        const size_t sn = (size_t)nvec;
+        double *__restrict__ u = work;                    // U diagonal
+        double *__restrict__ u1 = u + (size_t)n * sn;     // first superdiagonal
+        double *__restrict__ u2 = u1 + (size_t)n * sn;    // second superdiagonal
+        double *__restrict__ l = u2 + (size_t)n * sn;     // multipliers
+        double *__restrict__ pv = l + (size_t)n * sn;     // 1 where rows j, j+1 were swapped
+        double *__restrict__ z = pv + (size_t)n * sn;     // iterate
         double lam = sw[k];
         // perturb a repeated eigenvalue slightly so that clustered vectors start apart
         double sep = eps * tnorm * 10.0;
@@ -227,49 +233,63 @@ tridiag_eig_kernel(int n, int nvec, const double *__restrict__ d, const double *
         while (kk > 0 && fabs(sw[kk - 1] - sw[k]) < kClusterRel * tnorm) --kk;
         lam += (k - kk) * sep;
         // LU with partial pivoting of T - lam I (tridiagonal): rows may swap with the next
-        // store row j as (u[j], u1[j], u2[j]); piv[j] recorded in the sign bit of x (reused)
         double a_cur = sd[0] - lam, b_cur = se[0];
         for (int j = 0; j < n - 1; ++j) {
+            const size_t o = (size_t)j * sn + k;
             double c_j = se[j];                 // subdiagonal element below row j
             double a_nxt = sd[j + 1] - lam, b_nxt = se[j + 1];
             if (fabs(a_cur) >= fabs(c_j)) {
                 if (a_cur == 0.0) a_cur = eps * tnorm + 1e-300;
                 double m = c_j / a_cur;
-                u[j] = a_cur; u1[j] = b_cur; u2[j] = 0.0;
-                l[j] = m; x[j] = 0.0;           // no swap
+                u[o] = a_cur; u1[o] = b_cur; u2[o] = 0.0;
+                l[o] = m; pv[o] = 0.0;
                 a_cur = a_nxt - m * b_cur;
                 b_cur = b_nxt;
             } else {
                 double m = a_cur / c_j;
-                u[j] = c_j; u1[j] = a_nxt; u2[j] = b_nxt;
-                l[j] = m; x[j] = 1.0;           // swapped
+                u[o] = c_j; u1[o] = a_nxt; u2[o] = b_nxt;
+                l[o] = m; pv[o] = 1.0;
                 a_cur = b_cur - m * a_nxt;
                 b_cur = -m * b_nxt;
             }
         }
         if (a_cur == 0.0) a_cur = eps * tnorm + 1e-300;
-        u[n - 1] = a_cur; u1[n - 1] = 0.0; u2[n - 1] = 0.0;
-        // piv flags now in x; move them to a separate register-free encoding: keep
-        // them in x and use the y vector in Y's column as the iterate
-        double *z = Y + (size_t)k * n;
-        for (int j = 0; j < n; ++j) z[j] = 1.0 + 0.0001 * ((j * 7919 + k * 104729) % 1000) / 1000.0;
-        for (int iter = 0; iter < 3; ++iter) {
-            // forward: apply the row operations (L^-1 P) to z
-            for (int j = 0; j < n - 1; ++j) {
-                if (x[j] != 0.0) {               // swap z[j], z[j+1]
-                    double t = z[j]; z[j] = z[j + 1]; z[j + 1] = t;
-                }
-                z[j + 1] -= l[j] * z[j];
-            }
-            // back substitution with U (upper, two superdiagonals)
-            z[n - 1] /= u[n - 1];
-            if (n >= 2) z[n - 2] = (z[n - 2] - u1[n - 2] * z[n - 1]) / u[n - 2];
-            for (int j = n - 3; j >= 0; --j) z[j] = (z[j] - u1[j] * z[j + 1] - u2[j] * z[j + 2]) / u[j];
-            double nn = 0.0;
-            for (int j = 0; j < n; ++j) nn += z[j] * z[j];
-            nn = 1.0 / sqrt(nn);
-            for (int j = 0; j < n; ++j) z[j] *= nn;
+        {
+            const size_t o = (size_t)(n - 1) * sn + k;
+            u[o] = a_cur; u1[o] = 0.0; u2[o] = 0.0;
         }
+        for (int j = 0; j < n; ++j) z[(size_t)j * sn + k] = 1.0 + 0.0001 * ((j * 7919 + k * 104729) % 1000) / 1000.0;
+        for (int iter = 0; iter < 3; ++iter) {
+            // forward: apply the row operations (L^-1 P) to z, carrying z[j] in a register
+            double zj = z[k];
+            for (int j = 0; j < n - 1; ++j) {
+                const size_t o = (size_t)j * sn + k;
+                double zn = z[o + sn];
+                if (pv[o] != 0.0) {
+                    const double t = zj; zj = zn; zn = t;
+                }
+                zn -= l[o] * zj;
+                z[o] = zj;
+                zj = zn;
+            }
+            z[(size_t)(n - 1) * sn + k] = zj;
+            // back substitution with U (upper, two superdiagonals)
+            double z2 = 0.0, z1 = zj / u[(size_t)(n - 1) * sn + k];   // z[j+2], z[j+1]
+            z[(size_t)(n - 1) * sn + k] = z1;
+            double nn = z1 * z1;
+            for (int j = n - 2; j >= 0; --j) {
+                const size_t o = (size_t)j * sn + k;
+                const double zj0 = (z[o] - u1[o] * z1 - u2[o] * z2) / u[o];
+                z[o] = zj0;
+                nn += zj0 * zj0;
+                z2 = z1;
+                z1 = zj0;
+            }
+            nn = 1.0 / sqrt(nn);
+            for (int j = 0; j < n; ++j) z[(size_t)j * sn + k] *= nn;
+        }
+        double *yk = Y + (size_t)k * n;
+        for (int j = 0; j < n; ++j) yk[j] = z[(size_t)j * sn + k];
     }
     __syncthreads();
     // modified Gram-Schmidt inside clusters (sequential over vectors, parallel
@@ -403,7 +423,7 @@ void heev_lowest(Ctx &c, int n, int nvec, cd *A, int lda, double *W, cd *Z, int 
     if (n > kT) throw Error{WTB_ERR_UNSUPPORTED, "dense eigenproblem larger than 1024"};
     if (nvec > n) nvec = n;
     c.ws_heev_c.alloc((size_t)n);
-    c.ws_heev_d.alloc((size_t)2 * n + (size_t)n * nvec + (size_t)5 * n * nvec);
+    c.ws_heev_d.alloc((size_t)2 * n + (size_t)n * nvec + (size_t)6 * n * nvec);
     cd *tau = c.ws_heev_c.p;
     double *d = c.ws_heev_d.p, *e = d + n, *Y = e + n, *work = Y + (size_t)n * nvec;
     hipLaunchKernelGGL(tridiag_kernel, dim3(1), dim3(kT), (2 * (size_t)n + kT) * sizeof(cd), c.stream, n, A, lda,

@@ -324,6 +324,10 @@ int wtb_bse_diag(wtb_ctx *ctx, double *diag) {
     });
 }
 
+int wtb_grad_energies(wtb_ctx *ctx, const double *w, wtb_grad *g, double *hop_re_grad, double *hop_im_grad) {
+    return guarded(ctx, [&](Ctx &c) { grad_energies(c, w, g, hop_re_grad, hop_im_grad); });
+}
+
 namespace {
 __global__ void upload_diag(long N, long ld, const cd *__restrict__ H, double *__restrict__ d) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;

@@ -233,6 +233,8 @@ void heev_lowest(Ctx &c, int n, int nvec, cd *A, int lda, double *W, cd *Z, int 
 int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel);
 
 // Copy H(i0:i0+ni, j0:j0+nj) of the stored matrix into out (ni x nj, ldo) as FP64.
+// gradients of sum_n w_n E_n (k_grad.hip)
+void grad_energies(Ctx &c, const double *w, wtb_grad *out, double *g_re, double *g_im);
 void bse_block_to_c64(Ctx &c, long i0, long ni, long j0, long nj, cd *out, long ldo);
 
 // Block eigensolver driver (LOBPCG with soft locking, diagonal preconditioner).

@@ -44,6 +44,12 @@ module wtb_hip
         real(c_double) :: refine_flops
     end type wtb_report
 
+    ! gradients of sum_n w(n) E_n (wtb_grad_energies)
+    type, bind(C) :: wtb_grad
+        real(c_double) :: coulomb(7)      ! r0, EDIEL_T, EDIEL, EDIEL_B, EDIEL_Z, W_COUL, LC
+        real(c_double) :: scissor, efermi, gauge_residual
+    end type wtb_grad
+
     interface
         integer(c_int) function wtb_create(ctx, device) bind(C, name="wtb_create")
             import :: c_int, c_ptr
@@ -142,6 +148,15 @@ module wtb_hip
             type(c_ptr), value :: ctx
             real(c_double), intent(out) :: diag(*)
         end function wtb_bse_diag
+
+        ! gradients of sum_n w(n) E_n; hop_re_grad, hop_im_grad: c_loc of (nvec,norb,norb) arrays or c_null_ptr
+        integer(c_int) function wtb_grad_energies(ctx, w, g, hop_re_grad, hop_im_grad) bind(C, name="wtb_grad_energies")
+            import :: c_int, c_ptr, c_double, wtb_grad
+            type(c_ptr), value :: ctx
+            real(c_double), intent(in) :: w(*)
+            type(wtb_grad), intent(out) :: g
+            type(c_ptr), value :: hop_re_grad, hop_im_grad
+        end function wtb_grad_energies
 
         ! literal slepc_solver(HX,EX,HE,NX,NE) drop-in: uploads the host matrix
         integer(c_int) function wtb_slepc_solver_compat(hx, ex, he, nx, ne) bind(C, name="wtb_slepc_solver_compat")

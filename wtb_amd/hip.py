@@ -30,13 +30,21 @@ COMPONENTS = {"x": 0, "y": 1, "z": 2, "sp": 3, "sm": 4}
 EXPORTS = ["wtb_create", "wtb_destroy", "wtb_last_error", "wtb_set_tb_model", "wtb_set_kpoints",
            "wtb_hk_eig", "wtb_get_bands", "wtb_bse_build", "wtb_bse_solve", "wtb_optics",
            "wtb_oscillator", "wtb_bse_get_block", "wtb_bse_matvec", "wtb_bse_diag",
-           "wtb_slepc_solver_compat"]
+           "wtb_slepc_solver_compat", "wtb_grad_energies"]
 
 
 class WTBError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
         self.code = code
+
+
+class GradScalars(C.Structure):
+    _fields_ = [("coulomb", C.c_double * 7), ("scissor", C.c_double), ("efermi", C.c_double),
+                ("gauge_residual", C.c_double)]
+
+
+COULOMB_PARAMS = ("r0", "ediel_t", "ediel", "ediel_b", "ez", "w", "lc")
 
 
 class CoulombParams(C.Structure):
@@ -84,6 +92,7 @@ def load():
         "wtb_bse_matvec": [P, I, P, P, I],
         "wtb_bse_diag": [P, P],
         "wtb_slepc_solver_compat": [P, P, P, I, I],
+        "wtb_grad_energies": [P, P, C.POINTER(GradScalars), P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -151,6 +160,7 @@ class Context:
                                               _ptr(hre), _ptr(him), _ptr(rlat), float(model.efermi),
                                               float(model.scissor), int(flags)))
         self.norb = model.norb
+        self.nvec = model.nvec
 
     def set_kpoints(self, kpt, ngrid):
         k = np.asfortranarray(kpt, np.float64)
@@ -217,6 +227,24 @@ class Context:
         d = np.zeros(self.N)
         self._check(self.lib.wtb_bse_diag(self.h, _ptr(d)))
         return d
+
+    def grad_energies(self, w, hop=True):
+        """Gradient of L = sum_n w[n] E_n over the states of the last solve:
+        {'coulomb': {name: dL/dparam}, 'scissor', 'efermi', 'gauge_residual',
+        'hop': complex (nvec, norb, norb) dL/dRe H(R) + i dL/dIm H(R)}."""
+        w = np.ascontiguousarray(w, np.float64)
+        g = GradScalars()
+        gre = gim = None
+        if hop:
+            gre = np.zeros((self.nvec, self.norb, self.norb), order="F")
+            gim = np.zeros((self.nvec, self.norb, self.norb), order="F")
+        self._check(self.lib.wtb_grad_energies(self.h, _ptr(w), C.byref(g),
+                                               _ptr(gre) if hop else None, _ptr(gim) if hop else None))
+        out = {"coulomb": dict(zip(COULOMB_PARAMS, list(g.coulomb))), "scissor": g.scissor,
+               "efermi": g.efermi, "gauge_residual": g.gauge_residual}
+        if hop:
+            out["hop"] = gre + 1j * gim
+        return out
 
 
 def slepc_solver_compat(HX, NE):
