@@ -18,6 +18,11 @@ namespace wtb {
 namespace {
 
 constexpr int kT = 1024;
+// largest n whose Hermitian matrix (n^2 complex) is staged in LDS by the
+// tridiagonalisation and the Cholesky factorisation (with their work arrays
+// it stays under the 160 KB of a CU)
+constexpr int kTridLds = 90;
+constexpr int kCholLds = 96;
 // Eigenvalues closer than this (relative to |T|) are treated as one cluster
 // and their inverse-iteration vectors are re-orthogonalised.  Outside a
 // cluster inverse iteration leaves an orthogonality error of about
@@ -41,16 +46,26 @@ __device__ __forceinline__ double block_reduce(double v, double *red) {
 // A (n x n, column-major, lda): full Hermitian matrix, destroyed.  On exit
 // A[r + i lda] (r >= i+2) holds v_i(r) of reflector i (v_i(i+1) = 1), tau[i]
 // its scalar, d/e the real tridiagonal matrix T = Q^H A Q.
+// L: the whole matrix is staged in LDS (n <= kTridLds), otherwise the
+// trailing matrix is read from and written to global memory at every step.
+template <bool L>
 __global__ void __launch_bounds__(kT)
-tridiag_kernel(int n, cd *__restrict__ A, int lda, cd *__restrict__ tau, double *__restrict__ d,
+tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double *__restrict__ d,
                double *__restrict__ e) {
     extern __shared__ __align__(16) unsigned char smem[];
     cd *v = reinterpret_cast<cd *>(smem);     // n
     cd *y = v + n;                            // n
     cd *part = y + n;                         // kT partial sums
+    cd *M = part + kT;                        // n x n when L
     __shared__ double red[kT / 64];
     __shared__ cd s_tau;
     const int tid = threadIdx.x;
+    cd *A = L ? M : Ag;
+    const int lda = L ? n : ldg;
+    if (L) {
+        for (int idx = tid; idx < n * n; idx += kT) M[idx] = Ag[(idx % n) + (size_t)(idx / n) * ldg];
+        __syncthreads();
+    }
     for (int i = 0; i < n - 1; ++i) {
         const int s = n - i - 1;                               // trailing size
         // column norm below the subdiagonal
@@ -146,17 +161,28 @@ tridiag_kernel(int n, cd *__restrict__ A, int lda, cd *__restrict__ tau, double 
         d[n - 1] = A[(n - 1) + (size_t)(n - 1) * lda].x;
         if (n >= 1) tau[n > 1 ? n - 1 : 0] = make_double2(0.0, 0.0);
     }
+    if (L) {
+        // the reflectors (strict lower part) are read by the back-transformation
+        for (int idx = tid; idx < n * n; idx += kT) Ag[(idx % n) + (size_t)(idx / n) * ldg] = M[idx];
+    }
 }
 
 // ---------------------------------------------------------------------------
 // 2. tridiagonal eigenpairs
+// Number of eigenvalues of T below x (Sturm sequence of the LDL^T pivots).
+// 1/q comes from the hardware reciprocal refined by two Newton steps, which
+// keeps the pivots to within an ulp of the divided form at a fraction of the
+// latency of an IEEE division on the sequential chain.
 __device__ int sturm_count(int n, const double *d, const double *e2, double x, double pivmin) {
     int cnt = 0;
     double q = d[0] - x;
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q < 0.0;
     for (int j = 1; j < n; ++j) {
-        q = d[j] - x - e2[j - 1] / q;
+        double r = __builtin_amdgcn_rcp(q);
+        r = fma(r, fma(-q, r, 1.0), r);
+        r = fma(r, fma(-q, r, 1.0), r);
+        q = fma(-e2[j - 1], r, d[j] - x);
         if (fabs(q) < pivmin) q = -pivmin;
         cnt += q < 0.0;
     }
@@ -200,19 +226,51 @@ tridiag_eig_kernel(int n, int nvec, const double *__restrict__ d, const double *
     __syncthreads();
     const double tnorm = s_bounds[2];
     const double pivmin = fmax(1e-300, 1e-300 * tnorm);
-    // bisection: eigenvalue index k (0-based) = smallest x with count(x) > k
-    for (int k = tid; k < nvec; k += kT) {
-        double lo = s_bounds[0], hi = s_bounds[1];
-        for (int it = 0; it < 200; ++it) {
-            double mid = 0.5 * (lo + hi);
-            if (hi - lo <= 2.0 * 2.2e-16 * fmax(fabs(lo), fabs(hi)) + pivmin || mid == lo || mid == hi) break;
-            if (sturm_count(n, sd, se2, mid, pivmin) > k) hi = mid;
-            else lo = mid;
+    // multisection: eigenvalue k (0-based) is bracketed by [lo, hi] with
+    // count(lo) <= k < count(hi).  A group of G threads owns one eigenvalue and
+    // evaluates the Sturm count at G interior points per round, so the
+    // bracket shrinks (G + 1)-fold per round instead of 2-fold.
+    {
+        __shared__ double xs[kT];
+        __shared__ int cs[kT];
+        const int G = max(1, min(32, kT / max(nvec, 1)));
+        const int per = kT / G;
+        for (int k0 = 0; k0 < nvec; k0 += per) {
+            const int grp = tid / G, g = tid % G, kk = k0 + grp;
+            const bool own = grp < per && kk < nvec;
+            double lo = s_bounds[0], hi = s_bounds[1];
+            bool done = !own;
+            while (__syncthreads_or(!done)) {
+                double x = 0.0;
+                int cnt = 0;
+                if (!done) {
+                    x = lo + (double)(g + 1) * ((hi - lo) / (double)(G + 1));
+                    cnt = sturm_count(n, sd, se2, x, pivmin);
+                }
+                xs[tid] = x;
+                cs[tid] = cnt;
+                __syncthreads();
+                if (!done) {
+                    double nlo = lo, nhi = hi;
+                    for (int t = 0; t < G; ++t) {
+                        const double xt = xs[grp * G + t];
+                        if (cs[grp * G + t] > kk) nhi = fmin(nhi, xt);
+                        else nlo = fmax(nlo, xt);
+                    }
+                    if (nlo > nhi) nlo = nhi = 0.5 * (nlo + nhi);   // rounding at the last bits
+                    done = (nhi - nlo <= 2.0 * 2.2e-16 * fmax(fabs(nlo), fabs(nhi)) + pivmin) ||
+                           (nlo == lo && nhi == hi);
+                    lo = nlo;
+                    hi = nhi;
+                }
+            }
+            if (own && g == 0) {
+                sw[kk] = 0.5 * (lo + hi);
+                W[kk] = sw[kk];
+            }
         }
-        sw[k] = 0.5 * (lo + hi);
-        W[k] = sw[k];
+        __syncthreads();
     }
-    __syncthreads();
     // inverse iteration, one thread per eigenvector.  The per-vector work
     // arrays are interleaved across threads (element j of vector k at
     // [j * nvec + k]), so each step of the sequential sweeps is one coalesced
@@ -232,7 +290,8 @@ tridiag_eig_kernel(int n, int nvec, const double *__restrict__ d, const double *
         int kk = k;
         while (kk > 0 && fabs(sw[kk - 1] - sw[k]) < kClusterRel * tnorm) --kk;
         lam += (k - kk) * sep;
-        // LU with partial pivoting of T - lam I (tridiagonal): rows may swap with the next
+        // LU with partial pivoting of T - lam I (tridiagonal): rows may swap
+        // with the next; the diagonal of U is stored as its reciprocal
         double a_cur = sd[0] - lam, b_cur = se[0];
         for (int j = 0; j < n - 1; ++j) {
             const size_t o = (size_t)j * sn + k;
@@ -241,13 +300,13 @@ tridiag_eig_kernel(int n, int nvec, const double *__restrict__ d, const double *
             if (fabs(a_cur) >= fabs(c_j)) {
                 if (a_cur == 0.0) a_cur = eps * tnorm + 1e-300;
                 double m = c_j / a_cur;
-                u[o] = a_cur; u1[o] = b_cur; u2[o] = 0.0;
+                u[o] = 1.0 / a_cur; u1[o] = b_cur; u2[o] = 0.0;
                 l[o] = m; pv[o] = 0.0;
                 a_cur = a_nxt - m * b_cur;
                 b_cur = b_nxt;
             } else {
                 double m = a_cur / c_j;
-                u[o] = c_j; u1[o] = a_nxt; u2[o] = b_nxt;
+                u[o] = 1.0 / c_j; u1[o] = a_nxt; u2[o] = b_nxt;
                 l[o] = m; pv[o] = 1.0;
                 a_cur = b_cur - m * a_nxt;
                 b_cur = -m * b_nxt;
@@ -256,34 +315,63 @@ tridiag_eig_kernel(int n, int nvec, const double *__restrict__ d, const double *
         if (a_cur == 0.0) a_cur = eps * tnorm + 1e-300;
         {
             const size_t o = (size_t)(n - 1) * sn + k;
-            u[o] = a_cur; u1[o] = 0.0; u2[o] = 0.0;
+            u[o] = 1.0 / a_cur; u1[o] = 0.0; u2[o] = 0.0;
         }
         for (int j = 0; j < n; ++j) z[(size_t)j * sn + k] = 1.0 + 0.0001 * ((j * 7919 + k * 104729) % 1000) / 1000.0;
+        // The sweeps are sequential in j: each block of kB steps loads its
+        // operands first, so one memory latency is paid per block.
+        constexpr int kB = 8;
         for (int iter = 0; iter < 3; ++iter) {
             // forward: apply the row operations (L^-1 P) to z, carrying z[j] in a register
             double zj = z[k];
-            for (int j = 0; j < n - 1; ++j) {
-                const size_t o = (size_t)j * sn + k;
-                double zn = z[o + sn];
-                if (pv[o] != 0.0) {
-                    const double t = zj; zj = zn; zn = t;
+            for (int j0 = 0; j0 < n - 1; j0 += kB) {
+                double lb[kB], pb[kB], zb[kB];
+#pragma unroll
+                for (int t = 0; t < kB; ++t) {
+                    const int j = min(j0 + t, n - 2);
+                    const size_t o = (size_t)j * sn + k;
+                    lb[t] = l[o];
+                    pb[t] = pv[o];
+                    zb[t] = z[o + sn];
                 }
-                zn -= l[o] * zj;
-                z[o] = zj;
-                zj = zn;
+#pragma unroll
+                for (int t = 0; t < kB; ++t) {
+                    if (j0 + t < n - 1) {
+                        double zn = zb[t];
+                        if (pb[t] != 0.0) {
+                            const double tt = zj; zj = zn; zn = tt;
+                        }
+                        zn -= lb[t] * zj;
+                        z[(size_t)(j0 + t) * sn + k] = zj;
+                        zj = zn;
+                    }
+                }
             }
-            z[(size_t)(n - 1) * sn + k] = zj;
             // back substitution with U (upper, two superdiagonals)
-            double z2 = 0.0, z1 = zj / u[(size_t)(n - 1) * sn + k];   // z[j+2], z[j+1]
+            double z2 = 0.0, z1 = zj * u[(size_t)(n - 1) * sn + k];   // z[j+2], z[j+1]
             z[(size_t)(n - 1) * sn + k] = z1;
             double nn = z1 * z1;
-            for (int j = n - 2; j >= 0; --j) {
-                const size_t o = (size_t)j * sn + k;
-                const double zj0 = (z[o] - u1[o] * z1 - u2[o] * z2) / u[o];
-                z[o] = zj0;
-                nn += zj0 * zj0;
-                z2 = z1;
-                z1 = zj0;
+            for (int j1 = n - 2; j1 >= 0; j1 -= kB) {
+                double zb[kB], ub[kB], u1b[kB], u2b[kB];
+#pragma unroll
+                for (int t = 0; t < kB; ++t) {
+                    const int j = max(j1 - t, 0);
+                    const size_t o = (size_t)j * sn + k;
+                    zb[t] = z[o];
+                    ub[t] = u[o];
+                    u1b[t] = u1[o];
+                    u2b[t] = u2[o];
+                }
+#pragma unroll
+                for (int t = 0; t < kB; ++t) {
+                    if (j1 - t >= 0) {
+                        const double zj0 = (zb[t] - u1b[t] * z1 - u2b[t] * z2) * ub[t];
+                        z[(size_t)(j1 - t) * sn + k] = zj0;
+                        nn += zj0 * zj0;
+                        z2 = z1;
+                        z1 = zj0;
+                    }
+                }
             }
             nn = 1.0 / sqrt(nn);
             for (int j = 0; j < n; ++j) z[(size_t)j * sn + k] *= nn;
@@ -357,13 +445,21 @@ backtransform_kernel(int n, int nvec, const cd *__restrict__ A, int lda, const c
 
 // Upper Cholesky factor of a Hermitian positive definite G (n x n, ldg), in
 // place in the upper triangle: G = R^H R.  A pivot below rel * (original
-// diagonal) stops the factorisation and sets *flag = column + 1.
+// diagonal) stops the factorisation and sets *flag = column + 1.  L: G is
+// staged in LDS for the factorisation (n <= kCholLds).
+template <bool L>
 __global__ void __launch_bounds__(kT)
-cholesky_kernel(int n, cd *__restrict__ G, int ldg, double rel, int *__restrict__ flag) {
+cholesky_kernel(int n, cd *__restrict__ Gg, int ldgg, double rel, int *__restrict__ flag) {
     extern __shared__ __align__(16) unsigned char smem[];
     double *d0 = reinterpret_cast<double *>(smem);
+    cd *M = reinterpret_cast<cd *>(d0 + ((n + 1) & ~1));
     __shared__ int s_fail;
     const int tid = threadIdx.x;
+    cd *G = L ? M : Gg;
+    const int ldg = L ? n : ldgg;
+    if (L)
+        for (int idx = tid; idx < n * n; idx += kT) M[idx] = Gg[(idx % n) + (size_t)(idx / n) * ldgg];
+    __syncthreads();
     for (int j = tid; j < n; j += kT) d0[j] = G[j + (size_t)j * ldg].x;
     if (tid == 0) {
         s_fail = 0;
@@ -379,7 +475,7 @@ cholesky_kernel(int n, cd *__restrict__ G, int ldg, double rel, int *__restrict_
             }
         }
         __syncthreads();
-        if (s_fail) return;
+        if (s_fail) break;
         const double r = sqrt(piv), inv = 1.0 / r;
         for (int k = j + 1 + tid; k < n; k += kT) {
             cd g = G[j + (size_t)k * ldg];
@@ -400,6 +496,12 @@ cholesky_kernel(int n, cd *__restrict__ G, int ldg, double rel, int *__restrict_
         }
         __syncthreads();
     }
+    if (L && !s_fail) {
+        for (int idx = tid; idx < n * n; idx += kT) {
+            const int p = idx % n, q = idx / n;
+            if (p <= q) Gg[p + (size_t)q * ldgg] = M[idx];
+        }
+    }
 }
 
 }  // namespace
@@ -408,8 +510,12 @@ cholesky_kernel(int n, cd *__restrict__ G, int ldg, double rel, int *__restrict_
 int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel) {
     if (n > kT * 8) throw Error{WTB_ERR_UNSUPPORTED, "Cholesky size"};
     c.ws_flag.alloc(1);
-    hipLaunchKernelGGL(cholesky_kernel, dim3(1), dim3(kT), (size_t)n * sizeof(double), c.stream, n, G, ldg, rel,
-                       c.ws_flag.p);
+    const size_t d0 = (size_t)((n + 1) & ~1) * sizeof(double);
+    if (n <= kCholLds)
+        hipLaunchKernelGGL(cholesky_kernel<true>, dim3(1), dim3(kT), d0 + (size_t)n * n * sizeof(cd), c.stream, n, G,
+                           ldg, rel, c.ws_flag.p);
+    else
+        hipLaunchKernelGGL(cholesky_kernel<false>, dim3(1), dim3(kT), d0, c.stream, n, G, ldg, rel, c.ws_flag.p);
     WTB_LAUNCH_CHECK();
     int flag = 0;
     WTB_HIP_CHECK(hipMemcpyAsync(&flag, c.ws_flag.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -426,7 +532,11 @@ void heev_lowest(Ctx &c, int n, int nvec, cd *A, int lda, double *W, cd *Z, int 
     c.ws_heev_d.alloc((size_t)2 * n + (size_t)n * nvec + (size_t)6 * n * nvec);
     cd *tau = c.ws_heev_c.p;
     double *d = c.ws_heev_d.p, *e = d + n, *Y = e + n, *work = Y + (size_t)n * nvec;
-    hipLaunchKernelGGL(tridiag_kernel, dim3(1), dim3(kT), (2 * (size_t)n + kT) * sizeof(cd), c.stream, n, A, lda,
+    if (n <= kTridLds)
+        hipLaunchKernelGGL(tridiag_kernel<true>, dim3(1), dim3(kT), (2 * (size_t)n + kT + (size_t)n * n) * sizeof(cd),
+                           c.stream, n, A, lda, tau, d, e);
+    else
+        hipLaunchKernelGGL(tridiag_kernel<false>, dim3(1), dim3(kT), (2 * (size_t)n + kT) * sizeof(cd), c.stream, n, A, lda,
                        tau, d, e);
     WTB_LAUNCH_CHECK();
     hipLaunchKernelGGL(tridiag_eig_kernel, dim3(1), dim3(kT), 4 * (size_t)n * sizeof(double), c.stream, n, nvec,

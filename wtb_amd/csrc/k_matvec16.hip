@@ -10,8 +10,9 @@
 // 8-byte complex FP32 store once per launch (8 N^2 bytes).
 //
 // H is Hermitian, so row i of H is the conjugate of stored column i: each
-// wavefront streams 16 stored columns along k (contiguous 64-byte runs per
-// lane), i.e. the A operand of output rows i is conj(H[k + i ld]).  X arrives
+// wavefront streams 16 stored columns along k (the four lanes of a column read
+// 64 contiguous bytes per load instruction), i.e. the A operand of output rows
+// i is conj(H[k + i ld]).  X arrives
 // pre-split in FP16 planes (re_hi, re_lo, im_hi, im_lo), column-major with k
 // contiguous.  Each iteration covers KT k: the next H slice is loaded into
 // registers and the next X slice into a second LDS buffer while the MFMAs of
@@ -35,8 +36,20 @@ __device__ __forceinline__ f4 mfma16(h8 a, h8 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
 }
 
+// Order of k inside each 32-k block of the FP16 X planes.  MFMA lane group g
+// (lanes 16g .. 16g+15) supplies 8 k values per 16x16x32 step; they are taken
+// as k = 8q + 2g + t (q = 0..3, t = 0..1), so that the four lanes of one H
+// column read 64 contiguous bytes with each load instruction, and the X
+// planes store those 8 values contiguously (position 8g + 2q + t) so that the
+// B operand stays a single 16-byte LDS read.  A and B use the same k set.
+__device__ __forceinline__ long kperm(long k) {
+    const long b = k & ~31L, r = k & 31;
+    const long q = r >> 3, g = (r >> 1) & 3, t = r & 1;
+    return b + 8 * g + 2 * q + t;
+}
+
 // X (N x ncol, complex FP64, column-major) -> four FP16 planes
-// P[p][col][k] (k contiguous, kpad rows, zero padded), scaled per column.
+// P[p][col][kperm(k)] (kpad rows, zero padded), scaled per column.
 __global__ void split_x_kernel(long N, long kpad, int ncol, const double *__restrict__ X,
                                const float *__restrict__ xs, _Float16 *__restrict__ P) {
     long k = blockIdx.x * (long)blockDim.x + threadIdx.x;
@@ -50,7 +63,7 @@ __global__ void split_x_kernel(long N, long kpad, int ncol, const double *__rest
     _Float16 rh = (_Float16)re, ih = (_Float16)im;
     _Float16 rl = (_Float16)(re - (float)rh), il = (_Float16)(im - (float)ih);
     const size_t plane = (size_t)ncol * kpad;
-    const size_t o = k + (size_t)col * kpad;
+    const size_t o = kperm(k) + (size_t)col * kpad;
     P[o] = rh;
     P[plane + o] = rl;
     P[2 * plane + o] = ih;
@@ -126,7 +139,7 @@ hx_f16split_kernel(long N, long ld, int ncol, float hinv, const float2 *__restri
     // Rows N..ld-1 of H and of the X planes are zero, so every 64-row tile of
     // a column is read whole without bounds checks.
     const long row = min((long)blockIdx.x * kRows + wave * 16 + li, N - 1);
-    const float2 *hcol = H + (size_t)row * ld + 8 * blk;
+    const float2 *hcol = H + (size_t)row * ld + 2 * blk;
     const size_t plane = (size_t)ncol * ld;
     const long niter = (N + KT - 1) / KT;
 
@@ -147,13 +160,13 @@ hx_f16split_kernel(long N, long ld, int ncol, float hinv, const float2 *__restri
             *reinterpret_cast<h8 *>(&xs[buf][p][col][8 * q]) = r[t];
         }
     };
-    // lane's A slice of depth ch: k0 + 32 ch + 8 blk + [0, 8)
+    // lane's A slice of depth ch: k = k0 + 32 ch + 8 q + 2 blk + t (see kperm)
     auto load_h = [&](long it, float4 (&h)[4 * NCH]) {
         const float4 *src = reinterpret_cast<const float4 *>(hcol + min(it, niter - 1) * KT);
 #pragma unroll
         for (int ch = 0; ch < NCH; ++ch)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) h[4 * ch + q] = src[16 * ch + q];
+            for (int q = 0; q < 4; ++q) h[4 * ch + q] = src[16 * ch + 4 * q];
     };
 
     f4 yr[NCB], yi[NCB];
@@ -167,18 +180,16 @@ hx_f16split_kernel(long N, long ld, int ncol, float hinv, const float2 *__restri
         for (int ch = 0; ch < NCH; ++ch) {
             const Split a = split8(hc + 4 * ch, hinv);
             const int kk = 32 * ch + 8 * blk;
-            h8 b[2][4];
+            // every B fragment of this depth is requested before the first
+            // MFMA, so the LDS latency overlaps the earlier column blocks
+            h8 bf[NCB][4];
 #pragma unroll
-            for (int p = 0; p < 4; ++p) b[0][p] = *reinterpret_cast<const h8 *>(&xs[buf][p][li][kk]);
+            for (int cb = 0; cb < NCB; ++cb)
+#pragma unroll
+                for (int p = 0; p < 4; ++p) bf[cb][p] = *reinterpret_cast<const h8 *>(&xs[buf][p][cb * 16 + li][kk]);
 #pragma unroll
             for (int cb = 0; cb < NCB; ++cb) {
-                const int u = cb & 1;
-                if (cb + 1 < NCB) {
-#pragma unroll
-                    for (int p = 0; p < 4; ++p)
-                        b[u ^ 1][p] = *reinterpret_cast<const h8 *>(&xs[buf][p][(cb + 1) * 16 + li][kk]);
-                }
-                const h8 xrh = b[u][0], xrl = b[u][1], xih = b[u][2], xil = b[u][3];
+                const h8 xrh = bf[cb][0], xrl = bf[cb][1], xih = bf[cb][2], xil = bf[cb][3];
                 // Yr += Ar Xr - Ai Xi,  Yi += Ar Xi + Ai Xr  (independent chains interleaved)
                 yr[cb] = mfma16(a.rh, xrh, yr[cb]);
                 yi[cb] = mfma16(a.rh, xih, yi[cb]);
