@@ -145,18 +145,23 @@ def test_matvec(precision, ncol, rng):
     ctx.close()
 
 
-def test_solve_dense_small():
-    say("case dense honeycomb-12")
+@pytest.mark.parametrize("nk,nev", [(12, 144), (10, 100), (16, 96), (22, 200), (32, 64)])
+def test_solve_dense_small(nk, nev):
+    """Dense path (N <= 1024): the tridiagonalisation runs LDS-resident for
+    N <= 90 and blocked with panel widths 16 (N <= 256), 8 (N = 484) and 3
+    (N = 1024) above."""
+    say("case dense honeycomb", nk, nev)
     m = T.honeycomb_two_band()
-    mesh = (12, 12, 1)
+    mesh = (nk, nk, 1)
     ctx, k, eig, vec, nocp, ref = setup(m, mesh, 1, 1)
     c = dict(COUL, r0=10.0)
     ctx.bse_build(c, hip.FP64)
-    E, X, rep = ctx.bse_solve(144)
+    E, X, rep = ctx.bse_solve(nev)
     Hr = O.bse_matrix(k, ref["eig"], ref["vec"], O.coulomb(**c), mesh, m.rlat, 1, 1)
-    w = np.linalg.eigvalsh(Hr)
+    w = np.linalg.eigvalsh(Hr)[:nev]
     assert np.abs(E - w).max() < 1e-9
     assert np.abs(Hr @ X - X * E).max() < 1e-8
+    assert np.allclose(X.conj().T @ X, np.eye(nev), atol=1e-10)
     ctx.close()
 
 

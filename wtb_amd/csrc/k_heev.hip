@@ -1,12 +1,15 @@
 // Dense Hermitian eigensolver for the Rayleigh-Ritz step (FP64, n <= 1024).
 //
-//  1. Householder reduction to real symmetric tridiagonal form (unblocked,
-//     lower, ZHETD2 semantics with ZLARFG reflectors), one 1024-thread
-//     workgroup; the reflectors stay in the lower triangle of A.
-//  2. Eigenvalues of the tridiagonal matrix by Sturm-count bisection, one
-//     thread per eigenvalue; eigenvectors by inverse iteration with partial
-//     pivoting, one thread per vector, then modified Gram-Schmidt inside
-//     clusters of close eigenvalues (the scheme of LAPACK DSTEBZ/DSTEIN).
+//  1. Householder reduction to real symmetric tridiagonal form (lower, ZHETD2
+//     semantics with ZLARFG reflectors) in one 1024-thread workgroup; the
+//     reflectors stay in the lower triangle of A.  The whole matrix is staged
+//     in LDS for n <= kTridLds; above, the reduction is blocked (ZLATRD
+//     scheme): trailing updates are deferred over a panel of steps.
+//  2. Eigenvalues of the tridiagonal matrix by Sturm-count multisection over
+//     several workgroups; eigenvectors by inverse iteration with partial
+//     pivoting, one thread per vector with its factors in LDS, then modified
+//     Gram-Schmidt inside clusters of close eigenvalues (the scheme of LAPACK
+//     DSTEBZ/DSTEIN).
 //  3. Back-transformation Z = H(0) H(1) ... H(n-2) Y, independent per
 //     column block, spread over many workgroups.
 // Only the lowest `nvec` eigenpairs are formed.
@@ -46,9 +49,7 @@ __device__ __forceinline__ double block_reduce(double v, double *red) {
 // A (n x n, column-major, lda): full Hermitian matrix, destroyed.  On exit
 // A[r + i lda] (r >= i+2) holds v_i(r) of reflector i (v_i(i+1) = 1), tau[i]
 // its scalar, d/e the real tridiagonal matrix T = Q^H A Q.
-// L: the whole matrix is staged in LDS (n <= kTridLds), otherwise the
-// trailing matrix is read from and written to global memory at every step.
-template <bool L>
+// The whole matrix is staged in LDS (n <= kTridLds).
 __global__ void __launch_bounds__(kT)
 tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double *__restrict__ d,
                double *__restrict__ e) {
@@ -56,16 +57,14 @@ tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double
     cd *v = reinterpret_cast<cd *>(smem);     // n
     cd *y = v + n;                            // n
     cd *part = y + n;                         // kT partial sums
-    cd *M = part + kT;                        // n x n when L
+    cd *M = part + kT;                        // n x n
     __shared__ double red[kT / 64];
     __shared__ cd s_tau;
     const int tid = threadIdx.x;
-    cd *A = L ? M : Ag;
-    const int lda = L ? n : ldg;
-    if (L) {
-        for (int idx = tid; idx < n * n; idx += kT) M[idx] = Ag[(idx % n) + (size_t)(idx / n) * ldg];
-        __syncthreads();
-    }
+    cd *A = M;
+    const int lda = n;
+    for (int idx = tid; idx < n * n; idx += kT) M[idx] = Ag[(idx % n) + (size_t)(idx / n) * ldg];
+    __syncthreads();
     for (int i = 0; i < n - 1; ++i) {
         const int s = n - i - 1;                               // trailing size
         // column norm below the subdiagonal
@@ -161,9 +160,192 @@ tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double
         d[n - 1] = A[(n - 1) + (size_t)(n - 1) * lda].x;
         if (n >= 1) tau[n > 1 ? n - 1 : 0] = make_double2(0.0, 0.0);
     }
-    if (L) {
-        // the reflectors (strict lower part) are read by the back-transformation
-        for (int idx = tid; idx < n * n; idx += kT) Ag[(idx % n) + (size_t)(idx / n) * ldg] = M[idx];
+    // the reflectors (strict lower part) are read by the back-transformation
+    for (int idx = tid; idx < n * n; idx += kT) Ag[(idx % n) + (size_t)(idx / n) * ldg] = M[idx];
+}
+
+// Blocked form of the same reduction for n > kTridLds (the LATRD scheme).  The
+// trailing matrix in global memory is brought up to date only once per panel
+// of nb steps; inside a panel the current matrix is A - V W^H - W V^H, with
+// the panel's reflectors V and their companions W held in LDS.  Each step then
+// reads the stale trailing matrix once (for y = A v, corrected by the panel
+// terms) instead of reading it twice and writing it once.
+// LDS: V, W (nb x n each, column j of the panel at [j n]), y (n), 1024
+// partial sums.
+constexpr int kMaxNb = 16;
+__device__ __forceinline__ cd wave_sum(cd v) {
+    for (int off = 32; off > 0; off >>= 1) {
+        v.x += __shfl_xor(v.x, off, 64);
+        v.y += __shfl_xor(v.y, off, 64);
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(kT)
+tridiag_blocked_kernel(int n, int nb, cd *__restrict__ A, int lda, cd *__restrict__ tau, double *__restrict__ d,
+                       double *__restrict__ e) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cd *V = reinterpret_cast<cd *>(smem);
+    cd *W = V + (size_t)nb * n;
+    cd *y = W + (size_t)nb * n;
+    cd *buf = y + n;
+    __shared__ double red[kT / 64];
+    __shared__ cd s_c[2 * kMaxNb];      // W_j^H v, then V_j^H v
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    constexpr int kW = kT / 64;
+    int p0 = 0;
+    for (int i = 0; i < n - 1; ++i) {
+        const int ip = i - p0, t0 = i + 1;
+        // column i of the current matrix (rows i..n-1) into y; |.|^2 below the subdiagonal
+        double xn2 = 0.0;
+        for (int r = i + tid; r < n; r += kT) {
+            cd a = A[r + (size_t)i * lda];
+            for (int j = 0; j < ip; ++j) {
+                const cd vr = V[j * n + r], wr = W[j * n + r], vi = V[j * n + i], wi = W[j * n + i];
+                // a -= v_r conj(w_i) + w_r conj(v_i)
+                a.x -= (vr.x * wi.x + vr.y * wi.y) + (wr.x * vi.x + wr.y * vi.y);
+                a.y -= (vr.y * wi.x - vr.x * wi.y) + (wr.y * vi.x - wr.x * vi.y);
+            }
+            y[r] = a;
+            if (r >= i + 2) xn2 += a.x * a.x + a.y * a.y;
+        }
+        xn2 = block_reduce(xn2, red);
+        const cd alpha = y[t0];
+        if (tid == 0) d[i] = y[i].x;
+        cd *vp = V + (size_t)ip * n, *wp = W + (size_t)ip * n;
+        if (xn2 == 0.0 && alpha.y == 0.0) {
+            // H(i) = I: v = e_(i+1), w = 0; the stale column below is cleared
+            for (int r = t0 + tid; r < n; r += kT) {
+                vp[r] = make_double2(r == t0 ? 1.0 : 0.0, 0.0);
+                wp[r] = make_double2(0.0, 0.0);
+                if (r > t0) A[r + (size_t)i * lda] = make_double2(0.0, 0.0);
+            }
+            if (tid == 0) {
+                e[i] = alpha.x;
+                tau[i] = make_double2(0.0, 0.0);
+            }
+            __syncthreads();
+        } else {
+            const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+            const double beta = alpha.x >= 0.0 ? -nrm : nrm;
+            const cd t = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+            const double dd = (alpha.x - beta) * (alpha.x - beta) + alpha.y * alpha.y;
+            const cd scal = make_double2((alpha.x - beta) / dd, -alpha.y / dd);   // 1/(alpha - beta)
+            for (int r = t0 + tid; r < n; r += kT) {
+                cd vr = make_double2(1.0, 0.0);
+                if (r > t0) {
+                    const cd a = y[r];
+                    vr = make_double2(a.x * scal.x - a.y * scal.y, a.x * scal.y + a.y * scal.x);
+                    A[r + (size_t)i * lda] = vr;      // keep the reflector
+                }
+                vp[r] = vr;
+            }
+            if (tid == 0) {
+                e[i] = beta;
+                tau[i] = t;
+            }
+            __syncthreads();
+            // panel corrections W_j^H v and V_j^H v, one wavefront per product
+            for (int k = wv; k < 2 * ip; k += kW) {
+                const cd *B = k < ip ? W + (size_t)k * n : V + (size_t)(k - ip) * n;
+                cd acc = make_double2(0.0, 0.0);
+                for (int r = t0 + lane; r < n; r += 64) {
+                    const cd b = B[r], vr = vp[r];
+                    acc.x += b.x * vr.x + b.y * vr.y;
+                    acc.y += b.x * vr.y - b.y * vr.x;
+                }
+                acc = wave_sum(acc);
+                if (lane == 0) s_c[k < ip ? k : kMaxNb + (k - ip)] = acc;
+            }
+            // partial products of the stale trailing block with v: wavefront wv
+            // takes row chunk q (64 rows from rb) and every G-th column
+            const int rb = t0 & ~63, Qn = (n - rb + 63) / 64, G = kW / Qn;
+            const int q = wv % Qn, g = wv / Qn;
+            if (g < G) {
+                const int r = rb + 64 * q + lane;
+                const bool ok = r >= t0 && r < n;
+                const cd *ar = A + (ok ? r : t0);
+                cd acc = make_double2(0.0, 0.0);
+                int c = t0 + g;
+                for (; c + 7 * G < n; c += 8 * G) {
+                    cd a[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) a[u] = ar[(size_t)(c + u * G) * lda];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const cd vc = vp[c + u * G];
+                        acc.x += a[u].x * vc.x - a[u].y * vc.y;
+                        acc.y += a[u].x * vc.y + a[u].y * vc.x;
+                    }
+                }
+                for (; c < n; c += G) {
+                    const cd a = ar[(size_t)c * lda], vc = vp[c];
+                    acc.x += a.x * vc.x - a.y * vc.y;
+                    acc.y += a.x * vc.y + a.y * vc.x;
+                }
+                buf[g * Qn * 64 + 64 * q + lane] = acc;
+            }
+            __syncthreads();
+            // y = sum of partials - V (W^H v) - W (V^H v); mu = Re v^H y
+            double mu = 0.0;
+            for (int r = t0 + tid; r < n; r += kT) {
+                cd s = make_double2(0.0, 0.0);
+                for (int gg = 0; gg < G; ++gg) {
+                    const cd b = buf[gg * Qn * 64 + (r - rb)];
+                    s.x += b.x;
+                    s.y += b.y;
+                }
+                for (int j = 0; j < ip; ++j) {
+                    const cd vr = V[j * n + r], wr = W[j * n + r], cw = s_c[j], cv = s_c[kMaxNb + j];
+                    s.x -= (vr.x * cw.x - vr.y * cw.y) + (wr.x * cv.x - wr.y * cv.y);
+                    s.y -= (vr.x * cw.y + vr.y * cw.x) + (wr.x * cv.y + wr.y * cv.x);
+                }
+                y[r] = s;
+                const cd vr = vp[r];
+                mu += vr.x * s.x + vr.y * s.y;
+            }
+            mu = block_reduce(mu, red);
+            // w = tau y - |tau|^2 mu / 2 v (same thread per row as above)
+            const double t2 = t.x * t.x + t.y * t.y;
+            for (int r = t0 + tid; r < n; r += kT) {
+                const cd yr = y[r], vr = vp[r];
+                wp[r] = make_double2(t.x * yr.x - t.y * yr.y - 0.5 * t2 * mu * vr.x,
+                                     t.x * yr.y + t.y * yr.x - 0.5 * t2 * mu * vr.y);
+            }
+            __syncthreads();
+        }
+        // end of a panel: trailing block (rows, columns >= i+1) -= V W^H + W V^H;
+        // each thread takes one row and four columns
+        if (ip + 1 == nb || i == n - 2) {
+            const int np = ip + 1, s = n - t0, c4 = (s + 3) / 4;
+            for (int idx = tid; idx < s * c4; idx += kT) {
+                const int r = t0 + idx % s, cb = t0 + 4 * (idx / s);
+                cd a[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (cb + u < n) a[u] = A[r + (size_t)(cb + u) * lda];
+                for (int j = 0; j < np; ++j) {
+                    const cd vr = V[j * n + r], wr = W[j * n + r];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int c = min(cb + u, n - 1);
+                        const cd vc = V[j * n + c], wc = W[j * n + c];
+                        // a -= v_r conj(w_c) + w_r conj(v_c)
+                        a[u].x -= (vr.x * wc.x + vr.y * wc.y) + (wr.x * vc.x + wr.y * vc.y);
+                        a[u].y -= (vr.y * wc.x - vr.x * wc.y) + (wr.y * vc.x - wr.x * vc.y);
+                    }
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+                    if (cb + u < n) A[r + (size_t)(cb + u) * lda] = a[u];
+            }
+            __syncthreads();
+            p0 = i + 1;
+        }
+    }
+    if (tid == 0) {
+        d[n - 1] = A[(n - 1) + (size_t)(n - 1) * lda].x;
+        tau[n > 1 ? n - 1 : 0] = make_double2(0.0, 0.0);
     }
 }
 
@@ -173,218 +355,282 @@ tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double
 // 1/q comes from the hardware reciprocal refined by two Newton steps, which
 // keeps the pivots to within an ulp of the divided form at a fraction of the
 // latency of an IEEE division on the sequential chain.
+__device__ __forceinline__ double recip(double q) {
+    double r = __builtin_amdgcn_rcp(q);
+    r = fma(r, fma(-q, r, 1.0), r);
+    return fma(r, fma(-q, r, 1.0), r);
+}
+
 __device__ int sturm_count(int n, const double *d, const double *e2, double x, double pivmin) {
     int cnt = 0;
     double q = d[0] - x;
     if (fabs(q) < pivmin) q = -pivmin;
     cnt += q < 0.0;
-    for (int j = 1; j < n; ++j) {
-        double r = __builtin_amdgcn_rcp(q);
-        r = fma(r, fma(-q, r, 1.0), r);
-        r = fma(r, fma(-q, r, 1.0), r);
-        q = fma(-e2[j - 1], r, d[j] - x);
-        if (fabs(q) < pivmin) q = -pivmin;
-        cnt += q < 0.0;
+    // operands of 8 steps are read from LDS together, off the recurrence chain
+    constexpr int kS = 8;
+    for (int j0 = 1; j0 < n; j0 += kS) {
+        double db[kS], eb[kS];
+#pragma unroll
+        for (int t = 0; t < kS; ++t) {
+            const int j = min(j0 + t, n - 1);
+            db[t] = d[j] - x;
+            eb[t] = e2[j - 1];
+        }
+#pragma unroll
+        for (int t = 0; t < kS; ++t) {
+            if (j0 + t < n) {
+                q = fma(-eb[t], recip(q), db[t]);
+                if (fabs(q) < pivmin) q = -pivmin;
+                cnt += q < 0.0;
+            }
+        }
     }
     return cnt;
 }
 
-// Y (n x nvec, column-major, real) = eigenvectors of T for the nvec smallest
-// eigenvalues W (ascending).  work: 6 n doubles per vector.
-__global__ void __launch_bounds__(kT)
-tridiag_eig_kernel(int n, int nvec, const double *__restrict__ d, const double *__restrict__ e,
-                   double *__restrict__ W, double *__restrict__ Y, double *__restrict__ work) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    double *sd = reinterpret_cast<double *>(smem);   // n
-    double *se = sd + n;                              // n (e, e[n-1] = 0)
-    double *se2 = se + n;                             // n (e^2)
-    double *sw = se2 + n;                             // n eigenvalues
-    __shared__ double red[kT / 64];
-    __shared__ double s_bounds[3];
+// d, e (e[n-1] = 0) and e^2 into LDS, and the Gershgorin bounds of T:
+// bounds = {lo, hi, |T|}.  Every kernel below derives |T| the same way.
+__device__ void stage_tridiag(int n, const double *__restrict__ d, const double *__restrict__ e, double *sd,
+                              double *se, double *se2, double *bounds, double *red, int nt) {
     const int tid = threadIdx.x;
-    for (int j = tid; j < n; j += kT) {
-        sd[j] = d[j];
-        double ej = (j < n - 1) ? e[j] : 0.0;
+    double lo = 1e300, hi = -1e300, tn = 0.0;
+    for (int j = tid; j < n; j += nt) {
+        const double dj = d[j], ej = (j < n - 1) ? e[j] : 0.0, em = j > 0 ? fabs(e[j - 1]) : 0.0;
+        sd[j] = dj;
         se[j] = ej;
         se2[j] = ej * ej;
+        const double r = em + fabs(ej);
+        lo = fmin(lo, dj - r);
+        hi = fmax(hi, dj + r);
+        tn = fmax(tn, fabs(dj) + r);
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        lo = fmin(lo, __shfl_xor(lo, off, 64));
+        hi = fmax(hi, __shfl_xor(hi, off, 64));
+        tn = fmax(tn, __shfl_xor(tn, off, 64));
+    }
+    const int lane = tid & 63, w = tid >> 6, nw = nt / 64;
+    if (lane == 0) {
+        red[3 * w] = lo;
+        red[3 * w + 1] = hi;
+        red[3 * w + 2] = tn;
     }
     __syncthreads();
     if (tid == 0) {
-        // Gershgorin interval
-        double lo = 1e300, hi = -1e300, tn = 0.0;
-        for (int j = 0; j < n; ++j) {
-            double r = (j > 0 ? fabs(se[j - 1]) : 0.0) + (j < n - 1 ? fabs(se[j]) : 0.0);
-            lo = fmin(lo, sd[j] - r);
-            hi = fmax(hi, sd[j] + r);
-            tn = fmax(tn, fabs(sd[j]) + r);
+        for (int i = 1; i < nw; ++i) {
+            lo = fmin(lo, red[3 * i]);
+            hi = fmax(hi, red[3 * i + 1]);
+            tn = fmax(tn, red[3 * i + 2]);
         }
-        double pad = 2.2e-16 * tn * 4.0 * n + 1e-300;
-        s_bounds[0] = lo - pad;
-        s_bounds[1] = hi + pad;
-        s_bounds[2] = tn;
+        const double pad = 2.2e-16 * tn * 4.0 * n + 1e-300;
+        bounds[0] = lo - pad;
+        bounds[1] = hi + pad;
+        bounds[2] = tn;
     }
     __syncthreads();
-    const double tnorm = s_bounds[2];
+}
+
+// Eigenvalues W[0..nvec) (ascending) by multisection: one workgroup per
+// eigenvalue, its kBis threads evaluating the Sturm count at kBis interior
+// points of the bracket per round, so the bracket shrinks (kBis + 1)-fold per
+// round.  The new bracket is the largest point with count <= k and the
+// smallest with count > k (min/max reductions, robust to counts that are not
+// monotone in the last bits).
+constexpr int kBis = 256;
+__global__ void __launch_bounds__(kBis)
+bisect_kernel(int n, int nvec, const double *__restrict__ d, const double *__restrict__ e, double *__restrict__ W) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *sd = reinterpret_cast<double *>(smem);
+    double *se = sd + n, *se2 = se + n;
+    __shared__ double red[3 * kBis / 64];
+    __shared__ double bounds[3];
+    __shared__ double s_lo[kBis / 64], s_hi[kBis / 64];
+    stage_tridiag(n, d, e, sd, se, se2, bounds, red, kBis);
+    const double tnorm = bounds[2];
     const double pivmin = fmax(1e-300, 1e-300 * tnorm);
-    // multisection: eigenvalue k (0-based) is bracketed by [lo, hi] with
-    // count(lo) <= k < count(hi).  A group of G threads owns one eigenvalue and
-    // evaluates the Sturm count at G interior points per round, so the
-    // bracket shrinks (G + 1)-fold per round instead of 2-fold.
+    const int tid = threadIdx.x, kk = blockIdx.x;
+    double lo = bounds[0], hi = bounds[1];
+    for (int round = 0; round < 64; ++round) {
+        const double x = lo + (double)(tid + 1) * ((hi - lo) / (double)(kBis + 1));
+        const int cnt = sturm_count(n, sd, se2, x, pivmin);
+        double nlo = cnt > kk ? lo : x, nhi = cnt > kk ? x : hi;
+        nlo = fmax(nlo, lo);
+        nhi = fmin(nhi, hi);
+        for (int off = 32; off > 0; off >>= 1) {
+            nlo = fmax(nlo, __shfl_xor(nlo, off, 64));
+            nhi = fmin(nhi, __shfl_xor(nhi, off, 64));
+        }
+        if ((tid & 63) == 0) {
+            s_lo[tid >> 6] = nlo;
+            s_hi[tid >> 6] = nhi;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int w = 0; w < kBis / 64; ++w) {
+            nlo = fmax(nlo, s_lo[w]);
+            nhi = fmin(nhi, s_hi[w]);
+        }
+        __syncthreads();
+        if (nlo > nhi) nlo = nhi = 0.5 * (nlo + nhi);   // rounding at the last bits
+        const bool done = (nhi - nlo <= 2.0 * 2.2e-16 * fmax(fabs(nlo), fabs(nhi)) + pivmin) ||
+                          (nlo == lo && nhi == hi);
+        lo = nlo;
+        hi = nhi;
+        if (done) break;     // uniform: every thread holds the same bracket
+    }
+    if (tid == 0 && kk < nvec) W[kk] = 0.5 * (lo + hi);
+}
+
+// Inverse iteration (three steps from a fixed start vector) with partial
+// pivoting, one thread per eigenvector and `vpw` vectors per workgroup.  The
+// LU factors and the iterate of a vector live in LDS, interleaved across the
+// workgroup's vectors (element j of local vector k at [j vpw + k]), so each
+// step of the sequential sweeps costs an LDS access instead of a global one.
+// A repeated eigenvalue is shifted apart by multiples of 10 eps |T| so that
+// the vectors of a cluster start apart.  Y (n x nvec, column-major, real).
+__global__ void __launch_bounds__(64)
+inviter_kernel(int n, int nvec, int vpw, const double *__restrict__ d, const double *__restrict__ e,
+               const double *__restrict__ W, double *__restrict__ Y) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *sd = reinterpret_cast<double *>(smem);
+    double *se = sd + n, *se2 = se + n;
+    const size_t sn = (size_t)n * vpw;
+    double *u = se2 + n;          // reciprocal of the U diagonal
+    double *u1 = u + sn;          // first superdiagonal
+    double *u2 = u1 + sn;         // second superdiagonal
+    double *l = u2 + sn;          // multipliers
+    double *pv = l + sn;          // 1 where rows j, j+1 were swapped
+    double *z = pv + sn;          // iterate
+    __shared__ double red[3];
+    __shared__ double bounds[3];
+    stage_tridiag(n, d, e, sd, se, se2, bounds, red, 64);
+    const double tnorm = bounds[2];
+    const double eps = 2.2e-16;
+    const int kl = threadIdx.x, k = blockIdx.x * vpw + kl;
+    if (kl >= vpw || k >= nvec) return;
+    double lam = W[k];
+    int kk = k;
+    while (kk > 0 && fabs(W[kk - 1] - W[k]) < kClusterRel * tnorm) --kk;
+    lam += (k - kk) * (eps * tnorm * 10.0);
+    // LU of T - lam I: rows may swap with the next (branch-free: the pivot row
+    // is chosen by selects, one reciprocal per step)
+    double a_cur = sd[0] - lam, b_cur = se[0];
+    for (int j = 0; j < n - 1; ++j) {
+        const size_t o = (size_t)j * vpw + kl;
+        const double c_j = se[j];                 // subdiagonal element below row j
+        const double a_nxt = sd[j + 1] - lam, b_nxt = se[j + 1];
+        if (a_cur == 0.0 && c_j == 0.0) a_cur = eps * tnorm + 1e-300;
+        const bool sw = fabs(a_cur) < fabs(c_j);
+        const double piv = sw ? c_j : a_cur, oth = sw ? a_cur : c_j;
+        const double ip = recip(piv), m = oth * ip;
+        u[o] = ip;
+        u1[o] = sw ? a_nxt : b_cur;
+        u2[o] = sw ? b_nxt : 0.0;
+        l[o] = m;
+        pv[o] = sw ? 1.0 : 0.0;
+        const double na = sw ? b_cur - m * a_nxt : a_nxt - m * b_cur;
+        b_cur = sw ? -m * b_nxt : b_nxt;
+        a_cur = na;
+    }
+    if (a_cur == 0.0) a_cur = eps * tnorm + 1e-300;
     {
-        __shared__ double xs[kT];
-        __shared__ int cs[kT];
-        const int G = max(1, min(32, kT / max(nvec, 1)));
-        const int per = kT / G;
-        for (int k0 = 0; k0 < nvec; k0 += per) {
-            const int grp = tid / G, g = tid % G, kk = k0 + grp;
-            const bool own = grp < per && kk < nvec;
-            double lo = s_bounds[0], hi = s_bounds[1];
-            bool done = !own;
-            while (__syncthreads_or(!done)) {
-                double x = 0.0;
-                int cnt = 0;
-                if (!done) {
-                    x = lo + (double)(g + 1) * ((hi - lo) / (double)(G + 1));
-                    cnt = sturm_count(n, sd, se2, x, pivmin);
-                }
-                xs[tid] = x;
-                cs[tid] = cnt;
-                __syncthreads();
-                if (!done) {
-                    double nlo = lo, nhi = hi;
-                    for (int t = 0; t < G; ++t) {
-                        const double xt = xs[grp * G + t];
-                        if (cs[grp * G + t] > kk) nhi = fmin(nhi, xt);
-                        else nlo = fmax(nlo, xt);
-                    }
-                    if (nlo > nhi) nlo = nhi = 0.5 * (nlo + nhi);   // rounding at the last bits
-                    done = (nhi - nlo <= 2.0 * 2.2e-16 * fmax(fabs(nlo), fabs(nhi)) + pivmin) ||
-                           (nlo == lo && nhi == hi);
-                    lo = nlo;
-                    hi = nhi;
+        const size_t o = (size_t)(n - 1) * vpw + kl;
+        u[o] = recip(a_cur); u1[o] = 0.0; u2[o] = 0.0;
+    }
+    // generic start vector (distinct per vector)
+    for (int j = 0; j < n; ++j)
+        z[(size_t)j * vpw + kl] = 1.0 + 1e-4 * (double)(((unsigned)j * 7919u + (unsigned)k * 104729u) & 1023u);
+    constexpr int kB = 8;
+    double scale = 1.0;     // the iterate is z * scale
+    for (int iter = 0; iter < 3; ++iter) {
+        // forward: apply the row operations (L^-1 P) to z, carrying z[j] in a register
+        double zj = z[kl] * scale;
+        for (int j0 = 0; j0 < n - 1; j0 += kB) {
+            double lb[kB], pb[kB], zb[kB];
+#pragma unroll
+            for (int t = 0; t < kB; ++t) {
+                const size_t o = (size_t)min(j0 + t, n - 2) * vpw + kl;
+                lb[t] = l[o];
+                pb[t] = pv[o];
+                zb[t] = z[o + vpw];
+            }
+#pragma unroll
+            for (int t = 0; t < kB; ++t) {
+                if (j0 + t < n - 1) {
+                    double zn = zb[t] * scale;
+                    const bool s = pb[t] != 0.0;
+                    const double lo_ = s ? zn : zj, hi_ = s ? zj : zn;
+                    z[(size_t)(j0 + t) * vpw + kl] = lo_;
+                    zj = hi_ - lb[t] * lo_;
                 }
             }
-            if (own && g == 0) {
-                sw[kk] = 0.5 * (lo + hi);
-                W[kk] = sw[kk];
+        }
+        // back substitution with U (upper, two superdiagonals)
+        double z2 = 0.0, z1 = zj * u[(size_t)(n - 1) * vpw + kl];   // z[j+2], z[j+1]
+        z[(size_t)(n - 1) * vpw + kl] = z1;
+        double nn = z1 * z1;
+        for (int j1 = n - 2; j1 >= 0; j1 -= kB) {
+            double zb[kB], ub[kB], u1b[kB], u2b[kB];
+#pragma unroll
+            for (int t = 0; t < kB; ++t) {
+                const size_t o = (size_t)max(j1 - t, 0) * vpw + kl;
+                zb[t] = z[o];
+                ub[t] = u[o];
+                u1b[t] = u1[o];
+                u2b[t] = u2[o];
             }
+#pragma unroll
+            for (int t = 0; t < kB; ++t) {
+                if (j1 - t >= 0) {
+                    const double zj0 = (zb[t] - u1b[t] * z1 - u2b[t] * z2) * ub[t];
+                    z[(size_t)(j1 - t) * vpw + kl] = zj0;
+                    nn += zj0 * zj0;
+                    z2 = z1;
+                    z1 = zj0;
+                }
+            }
+        }
+        scale = 1.0 / sqrt(nn);
+    }
+    double *yk = Y + (size_t)k * n;
+    for (int j = 0; j < n; ++j) yk[j] = z[(size_t)j * vpw + kl] * scale;
+}
+
+// Modified Gram-Schmidt inside clusters of close eigenvalues (sequential
+// over vectors, parallel over the inner products).  Outside a cluster
+// inverse iteration leaves an orthogonality error of about eps |T| / gap.
+__global__ void __launch_bounds__(kT)
+cluster_gs_kernel(int n, int nvec, const double *__restrict__ d, const double *__restrict__ e,
+                  const double *__restrict__ W, double *__restrict__ Y) {
+    __shared__ double red[3 * kT / 64];
+    __shared__ double bounds[3];
+    __shared__ int s_any;
+    const int tid = threadIdx.x;
+    // |T| as the other kernels see it (no staging needed: bounds only)
+    {
+        double tn = 0.0;
+        for (int j = tid; j < n; j += kT) {
+            const double ej = (j < n - 1) ? e[j] : 0.0, em = j > 0 ? fabs(e[j - 1]) : 0.0;
+            tn = fmax(tn, fabs(d[j]) + em + fabs(ej));
+        }
+        for (int off = 32; off > 0; off >>= 1) tn = fmax(tn, __shfl_xor(tn, off, 64));
+        if ((tid & 63) == 0) red[tid >> 6] = tn;
+        if (tid == 0) s_any = 0;
+        __syncthreads();
+        if (tid == 0) {
+            for (int i = 1; i < kT / 64; ++i) tn = fmax(tn, red[i]);
+            bounds[2] = tn;
         }
         __syncthreads();
     }
-    // inverse iteration, one thread per eigenvector.  The per-vector work
-    // arrays are interleaved across threads (element j of vector k at
-    // [j * nvec + k]), so each step of the sequential sweeps is one coalesced
-    // access for the whole wavefront.
-    const double eps = 2.2e-16;
-    for (int k = tid; k < nvec; k += kT) {
-        const size_t sn = (size_t)nvec;
-        double *__restrict__ u = work;                    // U diagonal
-        double *__restrict__ u1 = u + (size_t)n * sn;     // first superdiagonal
-        double *__restrict__ u2 = u1 + (size_t)n * sn;    // second superdiagonal
-        double *__restrict__ l = u2 + (size_t)n * sn;     // multipliers
-        double *__restrict__ pv = l + (size_t)n * sn;     // 1 where rows j, j+1 were swapped
-        double *__restrict__ z = pv + (size_t)n * sn;     // iterate
-        double lam = sw[k];
-        // perturb a repeated eigenvalue slightly so that clustered vectors start apart
-        double sep = eps * tnorm * 10.0;
-        int kk = k;
-        while (kk > 0 && fabs(sw[kk - 1] - sw[k]) < kClusterRel * tnorm) --kk;
-        lam += (k - kk) * sep;
-        // LU with partial pivoting of T - lam I (tridiagonal): rows may swap
-        // with the next; the diagonal of U is stored as its reciprocal
-        double a_cur = sd[0] - lam, b_cur = se[0];
-        for (int j = 0; j < n - 1; ++j) {
-            const size_t o = (size_t)j * sn + k;
-            double c_j = se[j];                 // subdiagonal element below row j
-            double a_nxt = sd[j + 1] - lam, b_nxt = se[j + 1];
-            if (fabs(a_cur) >= fabs(c_j)) {
-                if (a_cur == 0.0) a_cur = eps * tnorm + 1e-300;
-                double m = c_j / a_cur;
-                u[o] = 1.0 / a_cur; u1[o] = b_cur; u2[o] = 0.0;
-                l[o] = m; pv[o] = 0.0;
-                a_cur = a_nxt - m * b_cur;
-                b_cur = b_nxt;
-            } else {
-                double m = a_cur / c_j;
-                u[o] = 1.0 / c_j; u1[o] = a_nxt; u2[o] = b_nxt;
-                l[o] = m; pv[o] = 1.0;
-                a_cur = b_cur - m * a_nxt;
-                b_cur = -m * b_nxt;
-            }
-        }
-        if (a_cur == 0.0) a_cur = eps * tnorm + 1e-300;
-        {
-            const size_t o = (size_t)(n - 1) * sn + k;
-            u[o] = 1.0 / a_cur; u1[o] = 0.0; u2[o] = 0.0;
-        }
-        for (int j = 0; j < n; ++j) z[(size_t)j * sn + k] = 1.0 + 0.0001 * ((j * 7919 + k * 104729) % 1000) / 1000.0;
-        // The sweeps are sequential in j: each block of kB steps loads its
-        // operands first, so one memory latency is paid per block.
-        constexpr int kB = 8;
-        for (int iter = 0; iter < 3; ++iter) {
-            // forward: apply the row operations (L^-1 P) to z, carrying z[j] in a register
-            double zj = z[k];
-            for (int j0 = 0; j0 < n - 1; j0 += kB) {
-                double lb[kB], pb[kB], zb[kB];
-#pragma unroll
-                for (int t = 0; t < kB; ++t) {
-                    const int j = min(j0 + t, n - 2);
-                    const size_t o = (size_t)j * sn + k;
-                    lb[t] = l[o];
-                    pb[t] = pv[o];
-                    zb[t] = z[o + sn];
-                }
-#pragma unroll
-                for (int t = 0; t < kB; ++t) {
-                    if (j0 + t < n - 1) {
-                        double zn = zb[t];
-                        if (pb[t] != 0.0) {
-                            const double tt = zj; zj = zn; zn = tt;
-                        }
-                        zn -= lb[t] * zj;
-                        z[(size_t)(j0 + t) * sn + k] = zj;
-                        zj = zn;
-                    }
-                }
-            }
-            // back substitution with U (upper, two superdiagonals)
-            double z2 = 0.0, z1 = zj * u[(size_t)(n - 1) * sn + k];   // z[j+2], z[j+1]
-            z[(size_t)(n - 1) * sn + k] = z1;
-            double nn = z1 * z1;
-            for (int j1 = n - 2; j1 >= 0; j1 -= kB) {
-                double zb[kB], ub[kB], u1b[kB], u2b[kB];
-#pragma unroll
-                for (int t = 0; t < kB; ++t) {
-                    const int j = max(j1 - t, 0);
-                    const size_t o = (size_t)j * sn + k;
-                    zb[t] = z[o];
-                    ub[t] = u[o];
-                    u1b[t] = u1[o];
-                    u2b[t] = u2[o];
-                }
-#pragma unroll
-                for (int t = 0; t < kB; ++t) {
-                    if (j1 - t >= 0) {
-                        const double zj0 = (zb[t] - u1b[t] * z1 - u2b[t] * z2) * ub[t];
-                        z[(size_t)(j1 - t) * sn + k] = zj0;
-                        nn += zj0 * zj0;
-                        z2 = z1;
-                        z1 = zj0;
-                    }
-                }
-            }
-            nn = 1.0 / sqrt(nn);
-            for (int j = 0; j < n; ++j) z[(size_t)j * sn + k] *= nn;
-        }
-        double *yk = Y + (size_t)k * n;
-        for (int j = 0; j < n; ++j) yk[j] = z[(size_t)j * sn + k];
-    }
+    const double tnorm = bounds[2];
+    for (int k = 1 + tid; k < nvec; k += kT)
+        if (fabs(W[k - 1] - W[k]) < kClusterRel * tnorm) s_any = 1;
     __syncthreads();
-    // modified Gram-Schmidt inside clusters (sequential over vectors, parallel
-    // over the inner products)
+    if (!s_any) return;
     for (int k = 1; k < nvec; ++k) {
         int k0 = k;
-        while (k0 > 0 && fabs(sw[k0 - 1] - sw[k0]) < kClusterRel * tnorm) --k0;
+        while (k0 > 0 && fabs(W[k0 - 1] - W[k0]) < kClusterRel * tnorm) --k0;
         if (k0 == k) continue;     // not in a cluster with a previous vector
         double *zk = Y + (size_t)k * n;
         for (int q = k0; q < k; ++q) {
@@ -445,21 +691,81 @@ backtransform_kernel(int n, int nvec, const cd *__restrict__ A, int lda, const c
 
 // Upper Cholesky factor of a Hermitian positive definite G (n x n, ldg), in
 // place in the upper triangle: G = R^H R.  A pivot below rel * (original
-// diagonal) stops the factorisation and sets *flag = column + 1.  L: G is
-// staged in LDS for the factorisation (n <= kCholLds).
-template <bool L>
-__global__ void __launch_bounds__(kT)
-cholesky_kernel(int n, cd *__restrict__ Gg, int ldgg, double rel, int *__restrict__ flag) {
+// diagonal) stops the factorisation and sets *flag = column + 1.
+//
+// n <= kCholLds: G is staged in LDS and factorised by one 256-thread
+// workgroup with one barrier per column.  Step j updates the trailing upper
+// triangle with the unscaled pivot row, g_pq -= conj(g_jp) g_jq / g_jj, and
+// leaves row j unscaled; the rows are divided by sqrt(g_jj) at the end.
+constexpr int kCT = 256;
+__global__ void __launch_bounds__(kCT)
+cholesky_lds_kernel(int n, cd *__restrict__ Gg, int ldgg, double rel, int *__restrict__ flag) {
     extern __shared__ __align__(16) unsigned char smem[];
     double *d0 = reinterpret_cast<double *>(smem);
-    cd *M = reinterpret_cast<cd *>(d0 + ((n + 1) & ~1));
+    cd *G = reinterpret_cast<cd *>(d0 + ((n + 1) & ~1));
     __shared__ int s_fail;
     const int tid = threadIdx.x;
-    cd *G = L ? M : Gg;
-    const int ldg = L ? n : ldgg;
-    if (L)
-        for (int idx = tid; idx < n * n; idx += kT) M[idx] = Gg[(idx % n) + (size_t)(idx / n) * ldgg];
+    for (int idx = tid; idx < n * n; idx += kCT) {
+        const int p = idx % n, q = idx / n;
+        if (p <= q) G[idx] = Gg[p + (size_t)q * ldgg];
+    }
+    for (int j = tid; j < n; j += kCT) d0[j] = Gg[j + (size_t)j * ldgg].x;
+    if (tid == 0) {
+        s_fail = 0;
+        *flag = 0;
+    }
     __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        const double piv = G[j + (size_t)j * n].x;
+        if (!(piv > rel * d0[j])) {
+            if (tid == 0) {
+                s_fail = j + 1;
+                *flag = j + 1;
+            }
+            break;      // every thread reads the same pivot and leaves together
+        }
+        const double ip = 1.0 / piv;
+        // upper triangle of the trailing block, p <= q, as (q, p) pairs
+        const int s = n - j - 1;
+        const int tri = s * (s + 1) / 2;
+        for (int t = tid; t < tri; t += kCT) {
+            // column q (0-based within the block) holds q+1 entries
+            int q = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+            while ((q + 1) * (q + 2) / 2 <= t) ++q;
+            while (q * (q + 1) / 2 > t) --q;
+            const int p = t - q * (q + 1) / 2;
+            const int P = j + 1 + p, Q = j + 1 + q;
+            const cd a = G[j + (size_t)P * n], b = G[j + (size_t)Q * n];
+            cd g = G[P + (size_t)Q * n];
+            // g -= conj(a) b / piv
+            g.x -= (a.x * b.x + a.y * b.y) * ip;
+            g.y -= (a.x * b.y - a.y * b.x) * ip;
+            G[P + (size_t)Q * n] = g;
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (s_fail) return;
+    // R: row j = (unscaled row j) / sqrt(g_jj), diagonal sqrt(g_jj)
+    for (int idx = tid; idx < n * n; idx += kCT) {
+        const int p = idx % n, q = idx / n;
+        if (p > q) continue;
+        const double r = sqrt(G[p + (size_t)p * n].x);
+        cd g = G[idx];
+        if (p == q) g = make_double2(r, 0.0);
+        else g = make_double2(g.x / r, g.y / r);
+        Gg[p + (size_t)q * ldgg] = g;
+    }
+}
+
+// n > kCholLds: right-looking factorisation in global memory, one
+// 1024-thread workgroup.
+__global__ void __launch_bounds__(kT)
+cholesky_kernel(int n, cd *__restrict__ G, int ldg, double rel, int *__restrict__ flag) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    double *d0 = reinterpret_cast<double *>(smem);
+    __shared__ int s_fail;
+    const int tid = threadIdx.x;
     for (int j = tid; j < n; j += kT) d0[j] = G[j + (size_t)j * ldg].x;
     if (tid == 0) {
         s_fail = 0;
@@ -496,12 +802,6 @@ cholesky_kernel(int n, cd *__restrict__ Gg, int ldgg, double rel, int *__restric
         }
         __syncthreads();
     }
-    if (L && !s_fail) {
-        for (int idx = tid; idx < n * n; idx += kT) {
-            const int p = idx % n, q = idx / n;
-            if (p <= q) Gg[p + (size_t)q * ldgg] = M[idx];
-        }
-    }
 }
 
 }  // namespace
@@ -512,10 +812,10 @@ int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel) {
     c.ws_flag.alloc(1);
     const size_t d0 = (size_t)((n + 1) & ~1) * sizeof(double);
     if (n <= kCholLds)
-        hipLaunchKernelGGL(cholesky_kernel<true>, dim3(1), dim3(kT), d0 + (size_t)n * n * sizeof(cd), c.stream, n, G,
+        hipLaunchKernelGGL(cholesky_lds_kernel, dim3(1), dim3(kCT), d0 + (size_t)n * n * sizeof(cd), c.stream, n, G,
                            ldg, rel, c.ws_flag.p);
     else
-        hipLaunchKernelGGL(cholesky_kernel<false>, dim3(1), dim3(kT), d0, c.stream, n, G, ldg, rel, c.ws_flag.p);
+        hipLaunchKernelGGL(cholesky_kernel, dim3(1), dim3(kT), d0, c.stream, n, G, ldg, rel, c.ws_flag.p);
     WTB_LAUNCH_CHECK();
     int flag = 0;
     WTB_HIP_CHECK(hipMemcpyAsync(&flag, c.ws_flag.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
@@ -529,18 +829,30 @@ void heev_lowest(Ctx &c, int n, int nvec, cd *A, int lda, double *W, cd *Z, int 
     if (n > kT) throw Error{WTB_ERR_UNSUPPORTED, "dense eigenproblem larger than 1024"};
     if (nvec > n) nvec = n;
     c.ws_heev_c.alloc((size_t)n);
-    c.ws_heev_d.alloc((size_t)2 * n + (size_t)n * nvec + (size_t)6 * n * nvec);
+    c.ws_heev_d.alloc((size_t)2 * n + (size_t)n * nvec);
     cd *tau = c.ws_heev_c.p;
-    double *d = c.ws_heev_d.p, *e = d + n, *Y = e + n, *work = Y + (size_t)n * nvec;
+    double *d = c.ws_heev_d.p, *e = d + n, *Y = e + n;
     if (n <= kTridLds)
-        hipLaunchKernelGGL(tridiag_kernel<true>, dim3(1), dim3(kT), (2 * (size_t)n + kT + (size_t)n * n) * sizeof(cd),
+        hipLaunchKernelGGL(tridiag_kernel, dim3(1), dim3(kT), (2 * (size_t)n + kT + (size_t)n * n) * sizeof(cd),
                            c.stream, n, A, lda, tau, d, e);
-    else
-        hipLaunchKernelGGL(tridiag_kernel<false>, dim3(1), dim3(kT), (2 * (size_t)n + kT) * sizeof(cd), c.stream, n, A, lda,
-                       tau, d, e);
+    else {
+        // panel width: V and W (2 nb n), y (n) and the partial sums (kT) in 156 KB
+        const size_t fixed = ((size_t)n + kT) * sizeof(cd);
+        const int nb = (int)std::min<size_t>(kMaxNb, (156 * 1024 - fixed) / (2 * (size_t)n * sizeof(cd)));
+        hipLaunchKernelGGL(tridiag_blocked_kernel, dim3(1), dim3(kT), fixed + 2 * (size_t)nb * n * sizeof(cd),
+                           c.stream, n, nb, A, lda, tau, d, e);
+    }
     WTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(tridiag_eig_kernel, dim3(1), dim3(kT), 4 * (size_t)n * sizeof(double), c.stream, n, nvec,
-                       d, e, W, Y, work);
+    hipLaunchKernelGGL(bisect_kernel, dim3(nvec), dim3(kBis), 3 * (size_t)n * sizeof(double),
+                       c.stream, n, nvec, d, e, W);
+    WTB_LAUNCH_CHECK();
+    // vectors per workgroup: d, e, e^2 and six n-vectors each in 150 KB
+    const int vpw = (int)std::max<size_t>(1, std::min<size_t>(16, (150 * 1024 / sizeof(double) - 3 * (size_t)n) /
+                                                                        (6 * (size_t)n)));
+    hipLaunchKernelGGL(inviter_kernel, dim3((nvec + vpw - 1) / vpw), dim3(64),
+                       (3 * (size_t)n + 6 * (size_t)n * vpw) * sizeof(double), c.stream, n, nvec, vpw, d, e, W, Y);
+    WTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(cluster_gs_kernel, dim3(1), dim3(kT), 0, c.stream, n, nvec, d, e, W, Y);
     WTB_LAUNCH_CHECK();
     hipLaunchKernelGGL(backtransform_kernel, dim3((nvec + 3) / 4), dim3(kBT), 0, c.stream, n, nvec, A, lda, tau,
                        Y, Z, ldz);
