@@ -29,8 +29,11 @@ namespace {
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int kRows = 64;      // output rows per workgroup (16 per wavefront)
 constexpr int kKT = 64;        // k per iteration of the split-FP16 kernel
+// wavefronts per workgroup (16 rows each): 8 puts two wavefronts on each SIMD
+// (a 256-register budget per wavefront), which keeps twice the H stream in
+// flight of a 4-wavefront workgroup
+constexpr int kWPB = 8;
 
 __device__ __forceinline__ f4 mfma16(h8 a, h8 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
@@ -70,29 +73,32 @@ __global__ void split_x_kernel(long N, long kpad, int ncol, const double *__rest
     P[3 * plane + o] = il;
 }
 
-// per-column max |x| (as float bits, atomicMax on non-negative floats)
-__global__ void colmax_kernel(long N, int ncol, const double *__restrict__ X, unsigned *__restrict__ mx) {
-    int col = blockIdx.y;
+// Per-column scales of X: max |x| over the column (one workgroup per column)
+// sets the power of two s with max |x| s < 2^13 (FP16 max 65504, room for the
+// products); xs = s scales the split operand, ysc = hscale / s undoes both
+// scalings on the output.
+__global__ void __launch_bounds__(256)
+col_scales_kernel(long N, const double *__restrict__ X, float hscale, float *__restrict__ xs,
+                  float *__restrict__ ysc) {
+    __shared__ float red[4];
+    const int col = blockIdx.x, tid = threadIdx.x;
+    const double2 *x = reinterpret_cast<const double2 *>(X) + (size_t)col * N;
     float m = 0.f;
-    for (long k = blockIdx.x * (long)blockDim.x + threadIdx.x; k < N; k += (long)gridDim.x * blockDim.x) {
-        m = fmaxf(m, fabsf((float)X[2 * (k + (size_t)col * N)]));
-        m = fmaxf(m, fabsf((float)X[2 * (k + (size_t)col * N) + 1]));
+    for (long k = tid; k < N; k += 256) {
+        const double2 v = x[k];
+        m = fmaxf(m, fmaxf(fabsf((float)v.x), fabsf((float)v.y)));
     }
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(mx + col, __float_as_uint(m));
-}
-
-__global__ void col_scales_kernel(int ncol, const unsigned *__restrict__ mx, float hscale, float *__restrict__ xs,
-                                  float *__restrict__ ysc) {
-    int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= ncol) return;
-    float m = __uint_as_float(mx[c]);
-    // scale so that max |x| s < 2^13 (FP16 max 65504, room for the products)
-    int e = 0;
-    if (m > 0.f) frexpf(m, &e);          // m = f 2^e, 0.5 <= f < 1
-    float s = ldexpf(1.0f, 13 - e);
-    xs[c] = s;
-    ysc[c] = hscale / s;
+    if ((tid & 63) == 0) red[tid >> 6] = m;
+    __syncthreads();
+    if (tid == 0) {
+        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        int e = 0;
+        if (m > 0.f) frexpf(m, &e);          // m = f 2^e, 0.5 <= f < 1
+        const float sc = ldexpf(1.0f, 13 - e);
+        xs[col] = sc;
+        ysc[col] = hscale / sc;
+    }
 }
 
 struct Split {
@@ -121,24 +127,27 @@ __device__ __forceinline__ Split split8(const float4 *h, float hinv) {
     return s;
 }
 
-template <int NCB, int KT>
-__global__ void __launch_bounds__(256)
+// WPB wavefronts per workgroup, each owning 16 output rows; the workgroup's X
+// slice is staged once in LDS for all of them.
+template <int NCB, int KT, int WPB>
+__global__ void __launch_bounds__(64 * WPB)
 hx_f16split_kernel(long N, long ld, int ncol, float hinv, const float2 *__restrict__ H,
                    const _Float16 *__restrict__ P, const float *__restrict__ ysc, double *__restrict__ Y) {
     constexpr int NC = NCB * 16;
     constexpr int NCH = KT / 32;                 // MFMA depths per iteration
     constexpr int XS = KT + 8;                   // halfs per staged column (pad against bank conflicts)
     constexpr int QN = KT / 8;                   // 16-byte chunks per column and plane
-    constexpr int XCH = 4 * NC * QN / 256;       // 16-byte X chunks per thread per iteration
-    static_assert(4 * NC * QN % 256 == 0, "X slice must split evenly over the workgroup");
+    constexpr int NT = 64 * WPB;                 // threads per workgroup
+    constexpr int XCH = 4 * NC * QN / NT;        // 16-byte X chunks per thread per iteration
+    static_assert(4 * NC * QN % NT == 0, "X slice must split evenly over the workgroup");
     __shared__ __align__(16) _Float16 xs[2][4][NC][XS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int blk = lane >> 4, li = lane & 15;
     // Rows past N (last workgroup) and X columns past ncol read clamped, valid
     // addresses: their products land in outputs that are never written.
-    // Rows N..ld-1 of H and of the X planes are zero, so every 64-row tile of
-    // a column is read whole without bounds checks.
-    const long row = min((long)blockIdx.x * kRows + wave * 16 + li, N - 1);
+    // Rows N..ld-1 of H and of the X planes are zero, so every k slice of a
+    // column is read whole without bounds checks.
+    const long row = min((long)blockIdx.x * (16 * WPB) + wave * 16 + li, N - 1);
     const float2 *hcol = H + (size_t)row * ld + 2 * blk;
     const size_t plane = (size_t)ncol * ld;
     const long niter = (N + KT - 1) / KT;
@@ -147,7 +156,7 @@ hx_f16split_kernel(long N, long ld, int ncol, float hinv, const float2 *__restri
         const long k0 = min(it, niter - 1) * KT;
 #pragma unroll
         for (int t = 0; t < XCH; ++t) {
-            const int c = tid + 256 * t;
+            const int c = tid + NT * t;
             const int q = c % QN, col = (c / QN) % NC, p = c / (QN * NC);
             r[t] = *reinterpret_cast<const h8 *>(P + p * plane + (size_t)min(col, ncol - 1) * ld + k0 + 8 * q);
         }
@@ -155,7 +164,7 @@ hx_f16split_kernel(long N, long ld, int ncol, float hinv, const float2 *__restri
     auto store_x = [&](int buf, const h8 (&r)[XCH]) {
 #pragma unroll
         for (int t = 0; t < XCH; ++t) {
-            const int c = tid + 256 * t;
+            const int c = tid + NT * t;
             const int q = c % QN, col = (c / QN) % NC, p = c / (QN * NC);
             *reinterpret_cast<h8 *>(&xs[buf][p][col][8 * q]) = r[t];
         }
@@ -235,7 +244,7 @@ hx_f16split_kernel(long N, long ld, int ncol, float hinv, const float2 *__restri
     if (it < niter) step(it, 0, ha, hb);
 
     // D layout (16x16 f32): register i of lane l -> row 4*(l/16)+i, column l%16
-    const long r0 = (long)blockIdx.x * kRows + wave * 16 + 4 * blk;
+    const long r0 = (long)blockIdx.x * (16 * WPB) + wave * 16 + 4 * blk;
 #pragma unroll
     for (int cb = 0; cb < NCB; ++cb) {
         const int col = cb * 16 + li;
@@ -252,13 +261,15 @@ hx_f16split_kernel(long N, long ld, int ncol, float hinv, const float2 *__restri
     }
 }
 
-template <int KT>
-void launch_f16split(hipStream_t st, int ncb, dim3 grid, long N, long ld, int nc, float hinv, const cf *H,
+template <int KT, int WPB>
+void launch_f16split(hipStream_t st, int ncb, long N, long ld, int nc, float hinv, const cf *H,
                      const _Float16 *P, const float *ysc, double *y) {
+    const dim3 grid((unsigned)((N + 16 * WPB - 1) / (16 * WPB)));
     switch (ncb) {
-#define WTB_F16_CASE(K)                                                                                       \
-    case K:                                                                                                   \
-        hipLaunchKernelGGL((hx_f16split_kernel<K, KT>), grid, dim3(256), 0, st, N, ld, nc, hinv, H, P, ysc, y); \
+#define WTB_F16_CASE(K)                                                                                        \
+    case K:                                                                                                    \
+        hipLaunchKernelGGL((hx_f16split_kernel<K, KT, WPB>), grid, dim3(64 * WPB), 0, st, N, ld, nc, hinv, H, P, \
+                           ysc, y);                                                                            \
         break;
         WTB_F16_CASE(1) WTB_F16_CASE(2) WTB_F16_CASE(3) WTB_F16_CASE(4)
         WTB_F16_CASE(5) WTB_F16_CASE(6) WTB_F16_CASE(7) WTB_F16_CASE(8)
@@ -279,21 +290,16 @@ void matvec_f16split(Ctx &c, int ncol, const cd *X, cd *Y) {
         const double *x = (const double *)X + 2 * (size_t)c0 * N;
         double *y = (double *)Y + 2 * (size_t)c0 * N;
         c.ws_split.alloc((size_t)4 * nc * ld);
-        c.ws_colmax.alloc(nc);
         c.ws_colscale.alloc(2 * (size_t)nc);
-        WTB_HIP_CHECK(hipMemsetAsync(c.ws_colmax.p, 0, sizeof(unsigned) * nc, c.stream));
-        unsigned gx = (unsigned)std::min<long>((N + 255) / 256, 64);
-        hipLaunchKernelGGL(colmax_kernel, dim3(gx, nc), dim3(256), 0, c.stream, N, nc, x, c.ws_colmax.p);
-        hipLaunchKernelGGL(col_scales_kernel, dim3((nc + 63) / 64), dim3(64), 0, c.stream, nc, c.ws_colmax.p,
-                           s.hscale, c.ws_colscale.p, c.ws_colscale.p + nc);
+        hipLaunchKernelGGL(col_scales_kernel, dim3(nc), dim3(256), 0, c.stream, N, x, s.hscale, c.ws_colscale.p,
+                           c.ws_colscale.p + nc);
         hipLaunchKernelGGL(split_x_kernel, dim3((unsigned)((ld + 255) / 256), nc), dim3(256), 0, c.stream, N, ld,
                            nc, x, c.ws_colscale.p, c.ws_split.p);
         WTB_LAUNCH_CHECK();
         const int ncb = (nc + 15) / 16;
-        dim3 grid((unsigned)((N + kRows - 1) / kRows));
         c.kclock.begin(KernelClock::F16SPLIT, c.stream);
-        launch_f16split<kKT>(c.stream, ncb, grid, N, ld, nc, 1.0f / s.hscale, s.h32.p, c.ws_split.p,
-                             c.ws_colscale.p + nc, y);
+        launch_f16split<kKT, kWPB>(c.stream, ncb, N, ld, nc, 1.0f / s.hscale, s.h32.p, c.ws_split.p,
+                                  c.ws_colscale.p + nc, y);
         c.kclock.end(KernelClock::F16SPLIT, c.stream, 8.0 * (double)ld * N, 8.0 * (double)N * N * nc);
         WTB_LAUNCH_CHECK();
     }

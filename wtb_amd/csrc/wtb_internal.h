@@ -187,7 +187,6 @@ struct Ctx {
     DBuf<int> ws_flag;   // device status word of the small factorisations
     DBuf<cd> ws_gram;    // split-K partial Gram products
     DBuf<_Float16> ws_split;   // FP16 split planes of X for the mixed matvec
-    DBuf<unsigned> ws_colmax;  // per-column max |x|
     DBuf<float> ws_colscale;   // per-column split / output scales
     KernelClock kclock;
 };
