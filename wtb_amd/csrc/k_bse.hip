@@ -8,8 +8,9 @@
 // is E_c - E_v + V(k,k).  Basis index i = k*nc*nv + c*nv + v.
 //
 // One 256-thread workgroup per tile of TK=64 k-points x TKP k'-points:
-//   phase 1: every (k,k') pair of the tile gets V (FP64) and its two overlap
-//            blocks (FP64 or FP32), staged in LDS; U(k') is staged too.
+//   phase 1: U(k') of the tile is staged in LDS; every (k,k') pair gets V
+//            (FP64) and its two overlap blocks (FP64 or FP32), which then
+//            replace the staged U(k') in LDS.
 //   phase 2: the 64*nc*nv x TKP*nc*nv output tile is written column by column,
 //            each store covering 256 consecutive rows of one column
 //            (2 KB FP32 / 4 KB FP64 contiguous): the stage is HBM-write bound.
@@ -88,8 +89,9 @@ template <class C>
 __device__ __forceinline__ C mk(double a, double b) { return C{(decltype(C{}.x))a, (decltype(C{}.x))b}; }
 
 // R: arithmetic/store precision of the overlaps and the matrix; NC, NV: bands;
-// S: k' pairs per thread (tile width TKP = 4*S).
-template <class R, int NC, int NV, int S>
+// TK k-points per tile (TK nc nv rows), S: k' pairs per thread (tile width
+// TKP = S kThreads / TK k'-points).
+template <class R, int NC, int NV, int S, int TK = kTK>
 __global__ void __launch_bounds__(kThreads)
 bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
                  const double *__restrict__ kpt, const double *__restrict__ esel,
@@ -103,29 +105,45 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
     using C = typename CT<R>::C;
     float vmax = 0.f;   // largest |Re|, |Im| written (scales the FP16 split of the matvec)
     constexpr int NB = NC + NV, NCV = NC * NV, NC2 = NC * NC, NV2 = NV * NV;
-    constexpr int TKP = 4 * S;
+    constexpr int TKP = S * kThreads / TK;
     extern __shared__ __align__(16) unsigned char smem[];
     const long N = (long)nk * NCV;
-    const int k0 = blockIdx.x * kTK, kp0 = blockIdx.y * TKP;
-    constexpr int NPAIR = kTK * TKP;
-    // LDS layout: pair Mc (C), pair Mv (C), staged U(k') (C), pair V (R)
+    const int k0 = blockIdx.x * TK, kp0 = blockIdx.y * TKP;
+    constexpr int NPAIR = TK * TKP;
+    // LDS layout: a union of {pair Mc, pair Mv} (written after phase 1) and
+    // the staged U(k') [kpl][j][o] (read in phase 1), then pair V (R).
+    const size_t pairs = (size_t)NPAIR * (NC2 + NV2), stage = (size_t)TKP * NB * norb;
     C *pmc = reinterpret_cast<C *>(smem);
     C *pmv = pmc + (size_t)NPAIR * NC2;
-    C *uks = pmv + (size_t)NPAIR * NV2;                  // [kpl][j][o]
-    R *pv = reinterpret_cast<R *>(uks + (size_t)TKP * NB * norb);
+    C *uks = pmc;
+    R *pv = reinterpret_cast<R *>(pmc + (pairs > stage ? pairs : stage));
     const int tid = threadIdx.x;
 
-    // stage U(k') for the tile's k' columns
-    for (int idx = tid; idx < TKP * NB * norb; idx += kThreads) {
-        int kpl = idx / (NB * norb), rem = idx % (NB * norb), j = rem / norb, o = rem % norb;
-        int kp = kp0 + kpl;
-        uks[idx] = (kp < nk) ? usel[((size_t)o * NB + j) * nk + kp] : mk<C>(0.0, 0.0);
+    // stage U(k') for the tile's k' columns, eight loads in flight per thread
+    {
+        constexpr int kL = 8;
+        const int tot = TKP * NB * norb;
+        for (int base = 0; base < tot; base += kL * kThreads) {
+            C v[kL];
+#pragma unroll
+            for (int t = 0; t < kL; ++t) {
+                const int idx = base + t * kThreads + tid;
+                const int kpl = idx / (NB * norb), rem = idx % (NB * norb), j = rem / norb, o = rem % norb;
+                const int kp = kp0 + kpl;
+                v[t] = (idx < tot && kp < nk) ? usel[((size_t)o * NB + j) * nk + kp] : mk<C>(0.0, 0.0);
+            }
+#pragma unroll
+            for (int t = 0; t < kL; ++t) {
+                const int idx = base + t * kThreads + tid;
+                if (idx < tot) uks[idx] = v[t];
+            }
+        }
     }
     __syncthreads();
 
-    // phase 1: pair products.  Thread -> kl = tid % 64, kpl = tid / 64 + 4 s
+    // phase 1: pair products.  Thread -> kl = tid % TK, kpl = tid / TK + (kThreads / TK) s
     {
-        const int kl = tid % kTK;
+        const int kl = tid % TK;
         const int k = k0 + kl;
         C mc[S][NC2], mv[S][NV2];
 #pragma unroll
@@ -136,43 +154,56 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
             for (int e = 0; e < NV2; ++e) mv[s][e] = mk<C>(0.0, 0.0);
         }
         if (k < nk) {
-            for (int o = 0; o < norb; ++o) {
-                C uk[NB];
+            // U(k) of OB orbitals is loaded before their products are formed
+            constexpr int OB = sizeof(R) == sizeof(float) ? 8 : 4;
+            for (int o0 = 0; o0 < norb; o0 += OB) {
+                C ukb[OB][NB];
 #pragma unroll
-                for (int j = 0; j < NB; ++j) uk[j] = usel[((size_t)o * NB + j) * nk + k];
+                for (int ob = 0; ob < OB; ++ob)
 #pragma unroll
-                for (int s = 0; s < S; ++s) {
-                    const int kpl = tid / kTK + 4 * s;
-                    const C *up = uks + (size_t)kpl * NB * norb + o;
+                    for (int j = 0; j < NB; ++j)
+                        ukb[ob][j] = (o0 + ob < norb) ? usel[((size_t)(o0 + ob) * NB + j) * nk + k]
+                                                      : mk<C>(0.0, 0.0);
 #pragma unroll
-                    for (int c = 0; c < NC; ++c) {
-                        const C a = uk[NV + c];
+                for (int ob = 0; ob < OB; ++ob) {
+                    const int o = o0 + ob;
+                    if (o >= norb) break;
+                    const C *uk = ukb[ob];
 #pragma unroll
-                        for (int c2 = 0; c2 < NC; ++c2) {
-                            const C b = up[(NV + c2) * norb];
-                            C &t = mc[s][c * NC + c2];
-                            t.x += a.x * b.x + a.y * b.y;
-                            t.y += a.x * b.y - a.y * b.x;
+                    for (int s = 0; s < S; ++s) {
+                        const int kpl = tid / TK + (kThreads / TK) * s;
+                        const C *up = uks + (size_t)kpl * NB * norb + o;
+#pragma unroll
+                        for (int c = 0; c < NC; ++c) {
+                            const C a = uk[NV + c];
+#pragma unroll
+                            for (int c2 = 0; c2 < NC; ++c2) {
+                                const C b = up[(NV + c2) * norb];
+                                C &t = mc[s][c * NC + c2];
+                                t.x += a.x * b.x + a.y * b.y;
+                                t.y += a.x * b.y - a.y * b.x;
+                            }
                         }
-                    }
 #pragma unroll
-                    for (int v = 0; v < NV; ++v) {
-                        const C a = uk[v];
+                        for (int v = 0; v < NV; ++v) {
+                            const C a = uk[v];
 #pragma unroll
-                        for (int v2 = 0; v2 < NV; ++v2) {
-                            const C b = up[v2 * norb];
-                            C &t = mv[s][v * NV + v2];
-                            t.x += a.x * b.x + a.y * b.y;
-                            t.y += a.x * b.y - a.y * b.x;
+                            for (int v2 = 0; v2 < NV; ++v2) {
+                                const C b = up[v2 * norb];
+                                C &t = mv[s][v * NV + v2];
+                                t.x += a.x * b.x + a.y * b.y;
+                                t.y += a.x * b.y - a.y * b.x;
+                            }
                         }
                     }
                 }
             }
         }
+        __syncthreads();    // the pair products overwrite the staged U
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            const int kpl = tid / kTK + 4 * s;
-            const int p = kpl * kTK + kl;
+            const int kpl = tid / TK + (kThreads / TK) * s;
+            const int p = kpl * TK + kl;
             const int kp = kp0 + kpl;
             double vq = 0.0;
             if (k < nk && kp < nk)
@@ -187,40 +218,53 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
     }
     __syncthreads();
 
-    // phase 2: coalesced column writes
-    constexpr int ROWS = kTK * NCV, COLS = TKP * NCV;
+    // phase 2: coalesced column writes.  Thread row il (k, c, v) takes one
+    // pair (k, k') at a time: the nc + nv overlap factors and V are read once
+    // from LDS for the nc nv outputs of the pair, each store covering 256
+    // consecutive rows of one column.
+    constexpr int ROWS = TK * NCV;
     const long I0 = (long)k0 * NCV, J0 = (long)kp0 * NCV;
     for (int il = tid; il < ROWS; il += kThreads) {
         const long I = I0 + il;
         if (I >= ld) break;     // rows N..ld-1 come out zero (k >= nk has zero V and overlaps)
         const int kl = il / NCV, ci = (il % NCV) / NV, vi = il % NV;
-        for (int jl = 0; jl < COLS; ++jl) {
-            const long J = J0 + jl;
-            if (J >= N) break;
-            const int kpl = jl / NCV, cj = (jl % NCV) / NV, vj = jl % NV;
-            const int p = kpl * kTK + kl;
-            C val;
-            if (I == J) {
-                // The FP32 store keeps a zero diagonal: E_c - E_v + V(0) lives
-                // in the FP64 diagonal array and is applied in FP64 by the
-                // matvec, so FP32 rounding only touches the Coulomb part.
-                if (sizeof(R) == sizeof(float)) {
-                    val = mk<C>(0.0, 0.0);
-                } else {
-                    const int k = k0 + kl;
-                    double de = esel[(size_t)k * NB + NV + ci] - esel[(size_t)k * NB + vi];
-                    val = mk<C>(de + (double)pv[p], 0.0);
-                }
-            } else {
-                const C a = pmc[(size_t)p * NC2 + ci * NC + cj];
-                C b = pmv[(size_t)p * NV2 + vi * NV + vj];
-                if (textbook) b.y = -b.y;
-                const R vq = pv[p];
-                val.x = vq * (a.x * b.x - a.y * b.y);
-                val.y = vq * (a.x * b.y + a.y * b.x);
+        for (int kpl = 0; kpl < TKP; ++kpl) {
+            const long Jb = J0 + (long)kpl * NCV;
+            if (Jb >= N) break;
+            const int p = kpl * TK + kl;
+            const R vq = pv[p];
+            C a[NC], b[NV];
+#pragma unroll
+            for (int cj = 0; cj < NC; ++cj) a[cj] = pmc[(size_t)p * NC2 + ci * NC + cj];
+#pragma unroll
+            for (int vj = 0; vj < NV; ++vj) {
+                b[vj] = pmv[(size_t)p * NV2 + vi * NV + vj];
+                if (textbook) b[vj].y = -b[vj].y;
             }
-            H[I + J * ld] = val;
-            vmax = fmaxf(vmax, fmaxf(fabsf((float)val.x), fabsf((float)val.y)));
+#pragma unroll
+            for (int cj = 0; cj < NC; ++cj)
+#pragma unroll
+                for (int vj = 0; vj < NV; ++vj) {
+                    const long J = Jb + cj * NV + vj;
+                    C val;
+                    if (I == J) {
+                        // The FP32 store keeps a zero diagonal: E_c - E_v + V(0)
+                        // lives in the FP64 diagonal array and is applied in FP64
+                        // by the matvec, so FP32 rounding only touches the Coulomb part.
+                        if (sizeof(R) == sizeof(float)) {
+                            val = mk<C>(0.0, 0.0);
+                        } else {
+                            const int k = k0 + kl;
+                            double de = esel[(size_t)k * NB + NV + ci] - esel[(size_t)k * NB + vi];
+                            val = mk<C>(de + (double)vq, 0.0);
+                        }
+                    } else {
+                        val.x = vq * (a[cj].x * b[vj].x - a[cj].y * b[vj].y);
+                        val.y = vq * (a[cj].x * b[vj].y + a[cj].y * b[vj].x);
+                    }
+                    H[I + J * ld] = val;
+                    vmax = fmaxf(vmax, fmaxf(fabsf((float)val.x), fabsf((float)val.y)));
+                }
         }
     }
     if (hmax) {
@@ -234,16 +278,19 @@ void launch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const typename CT<
                  typename CT<R>::C *H) {
     using C = typename CT<R>::C;
     constexpr int per = NC * NC + NV * NV;
-    constexpr int S = per <= 4 ? 4 : (per <= 8 ? 2 : 1);
-    constexpr int TKP = 4 * S;
+    // k' pairs per thread: the pair products of a tile stay within 64 KB
+    constexpr size_t kPairBytes = 64 * 1024;
+    constexpr int S0 = (int)(kPairBytes / ((size_t)kThreads * per * sizeof(C)));
+    constexpr int S = S0 >= 4 ? 4 : (S0 >= 2 ? 2 : 1);
+    constexpr int TK = kTK, TKP = S * kThreads / TK;
     const int nk = c.bands.nk, norb = c.model.norb;
-    size_t shmem = (size_t)kTK * TKP * per * sizeof(C) + (size_t)TKP * (NC + NV) * norb * sizeof(C) +
-                   (size_t)kTK * TKP * sizeof(R) + 16;
+    const size_t pairs = (size_t)TK * TKP * per, ukp = (size_t)TKP * (NC + NV) * norb;
+    const size_t shmem = std::max(pairs, ukp) * sizeof(C) + (size_t)TK * TKP * sizeof(R) + 16;
     if (shmem > 160 * 1024) throw Error{WTB_ERR_UNSUPPORTED, "assembly tile does not fit in LDS"};
-    dim3 grid((nk + kTK - 1) / kTK, (nk + TKP - 1) / TKP);
-    hipLaunchKernelGGL((bse_build_kernel<R, NC, NV, S>), grid, dim3(kThreads), shmem, c.stream, nk, norb,
-                       c.bse.ld, textbook, cd_, c.kpt.p, c.bands.esel.p, usel, H,
-                       sizeof(R) == sizeof(float) ? c.bse.hmax.p : nullptr);
+    dim3 grid((nk + TK - 1) / TK, (nk + TKP - 1) / TKP);
+    unsigned *hm = sizeof(R) == sizeof(float) ? c.bse.hmax.p : nullptr;
+    hipLaunchKernelGGL((bse_build_kernel<R, NC, NV, S, TK>), grid, dim3(kThreads), shmem, c.stream, nk, norb,
+                       c.bse.ld, textbook, cd_, c.kpt.p, c.bands.esel.p, usel, H, hm);
     WTB_LAUNCH_CHECK();
 }
 
