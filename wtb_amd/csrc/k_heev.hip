@@ -703,11 +703,13 @@ cholesky_lds_kernel(int n, cd *__restrict__ Gg, int ldgg, double rel, int *__res
     extern __shared__ __align__(16) unsigned char smem[];
     double *d0 = reinterpret_cast<double *>(smem);
     cd *G = reinterpret_cast<cd *>(d0 + ((n + 1) & ~1));
+    // odd leading dimension: row j of consecutive columns falls in different banks
+    const int ln = n | 1;
     __shared__ int s_fail;
     const int tid = threadIdx.x;
     for (int idx = tid; idx < n * n; idx += kCT) {
         const int p = idx % n, q = idx / n;
-        if (p <= q) G[idx] = Gg[p + (size_t)q * ldgg];
+        if (p <= q) G[p + q * ln] = Gg[p + (size_t)q * ldgg];
     }
     for (int j = tid; j < n; j += kCT) d0[j] = Gg[j + (size_t)j * ldgg].x;
     if (tid == 0) {
@@ -716,7 +718,7 @@ cholesky_lds_kernel(int n, cd *__restrict__ Gg, int ldgg, double rel, int *__res
     }
     __syncthreads();
     for (int j = 0; j < n; ++j) {
-        const double piv = G[j + (size_t)j * n].x;
+        const double piv = G[j + j * ln].x;
         if (!(piv > rel * d0[j])) {
             if (tid == 0) {
                 s_fail = j + 1;
@@ -735,12 +737,12 @@ cholesky_lds_kernel(int n, cd *__restrict__ Gg, int ldgg, double rel, int *__res
             while (q * (q + 1) / 2 > t) --q;
             const int p = t - q * (q + 1) / 2;
             const int P = j + 1 + p, Q = j + 1 + q;
-            const cd a = G[j + (size_t)P * n], b = G[j + (size_t)Q * n];
-            cd g = G[P + (size_t)Q * n];
+            const cd a = G[j + P * ln], b = G[j + Q * ln];
+            cd g = G[P + Q * ln];
             // g -= conj(a) b / piv
             g.x -= (a.x * b.x + a.y * b.y) * ip;
             g.y -= (a.x * b.y - a.y * b.x) * ip;
-            G[P + (size_t)Q * n] = g;
+            G[P + Q * ln] = g;
         }
         __syncthreads();
     }
@@ -750,8 +752,8 @@ cholesky_lds_kernel(int n, cd *__restrict__ Gg, int ldgg, double rel, int *__res
     for (int idx = tid; idx < n * n; idx += kCT) {
         const int p = idx % n, q = idx / n;
         if (p > q) continue;
-        const double r = sqrt(G[p + (size_t)p * n].x);
-        cd g = G[idx];
+        const double r = sqrt(G[p + p * ln].x);
+        cd g = G[p + q * ln];
         if (p == q) g = make_double2(r, 0.0);
         else g = make_double2(g.x / r, g.y / r);
         Gg[p + (size_t)q * ldgg] = g;
@@ -812,7 +814,7 @@ int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel) {
     c.ws_flag.alloc(1);
     const size_t d0 = (size_t)((n + 1) & ~1) * sizeof(double);
     if (n <= kCholLds)
-        hipLaunchKernelGGL(cholesky_lds_kernel, dim3(1), dim3(kCT), d0 + (size_t)n * n * sizeof(cd), c.stream, n, G,
+        hipLaunchKernelGGL(cholesky_lds_kernel, dim3(1), dim3(kCT), d0 + (size_t)(n | 1) * n * sizeof(cd), c.stream, n, G,
                            ldg, rel, c.ws_flag.p);
     else
         hipLaunchKernelGGL(cholesky_kernel, dim3(1), dim3(kT), d0, c.stream, n, G, ldg, rel, c.ws_flag.p);

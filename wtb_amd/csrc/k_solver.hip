@@ -213,27 +213,37 @@ __global__ void set_units(long N, int b, const int *__restrict__ idx, cd *__rest
 }
 
 // norm2[j] += sum_i |AX_ij - lam_j X_ij|^2   (lam may be null: plain column norms of AX)
-__global__ void resid_norms(long N, const cd *__restrict__ X, const cd *__restrict__ AX,
-                            const double *__restrict__ lam, double *__restrict__ norm2) {
-    __shared__ double red[4];
-    const int j = blockIdx.y;
+// Column norms^2 of AX - lam X (or of AX when lam is null), one workgroup per
+// column (a fixed reduction order, no atomics).
+constexpr int kNT = 1024;
+__global__ void __launch_bounds__(kNT)
+resid_norms(long N, const cd *__restrict__ X, const cd *__restrict__ AX, const double *__restrict__ lam,
+            double *__restrict__ norm2) {
+    __shared__ double red[kNT / 64];
+    const int j = blockIdx.x;
+    const cd *ax = AX + (size_t)j * N;
+    const cd *x = X + (size_t)j * N;
+    const double l = lam ? lam[j] : 0.0;
     double s = 0.0;
-    for (long i = blockIdx.x * (long)blockDim.x + threadIdx.x; i < N; i += (long)gridDim.x * blockDim.x) {
-        cd a = AX[i + (size_t)j * N];
+    for (long i = threadIdx.x; i < N; i += kNT) {
+        cd a = ax[i];
         if (lam) {
-            cd x = X[i + (size_t)j * N];
-            a.x -= lam[j] * x.x;
-            a.y -= lam[j] * x.y;
+            const cd xi = x[i];
+            a.x -= l * xi.x;
+            a.y -= l * xi.y;
         }
         s += a.x * a.x + a.y * a.y;
     }
     for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(&norm2[j], red[0] + red[1] + red[2] + red[3]);
+    if (threadIdx.x == 0) {
+        double t = 0.0;
+        for (int w = 0; w < kNT / 64; ++w) t += red[w];
+        norm2[j] = t;
+    }
 }
 
-// W[:, a] = (AX[:, j] - lam_j X[:, j]) / (d - lam_j), j = act[a]; scaled by sc[a].
 __global__ void precondition(long N, const cd *__restrict__ X, const cd *__restrict__ AX,
                              const double *__restrict__ lam, const double *__restrict__ diag,
                              const int *__restrict__ act, const double *__restrict__ sc, cd *__restrict__ W) {
@@ -263,11 +273,13 @@ __global__ void gather_scale(long N, const int *__restrict__ act, const double *
     APo[i + (size_t)a * N] = cscale(f, ap);
 }
 
-__global__ void scale_cols(long N, const double *__restrict__ sc, cd *__restrict__ W) {
+// W[:, a] /= sqrt(norm2[a]) (columns of zero norm are left alone)
+__global__ void unit_cols(long N, const double *__restrict__ norm2, cd *__restrict__ W) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     int a = blockIdx.y;
     if (i >= N) return;
-    W[i + (size_t)a * N] = cscale(sc[a], W[i + (size_t)a * N]);
+    const double n2 = norm2[a];
+    if (n2 > 0.0) W[i + (size_t)a * N] = cscale(1.0 / sqrt(n2), W[i + (size_t)a * N]);
 }
 
 // T = U[:, off:off+r] diag(d^-1/2)
@@ -287,46 +299,162 @@ inline rocblas_double_complex *rb(cd *p) { return reinterpret_cast<rocblas_doubl
 
 // C (m x n) = A^H B, A: K x m, B: K x n (column-major, ld = K)
 // sum of nsplit partial m x n products (ld m, stride m n) into C (ldc)
-__global__ void reduce_splitk(int m, int n, int nsplit, const cd *__restrict__ part, cd *__restrict__ C, int ldc) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y;
-    if (i >= m || j >= n) return;
-    double re = 0.0, im = 0.0;
-    for (int s = 0; s < nsplit; ++s) {
-        cd v = part[(size_t)s * m * n + i + (size_t)j * m];
-        re += v.x;
-        im += v.y;
+// ---------------------------------------------------------------------------
+// Tall-skinny Gram products C = A^H B (K >> m, n) on FP64 MFMA
+// (v_mfma_f64_16x16x4f64; operand lane l holds row/column l % 16 at k = l / 16,
+// result register r of lane l holds element (4 r + l / 16, l % 16)).
+// A workgroup owns an MT x NT block of C for one K chunk: each K-step of A
+// and B is staged in LDS once and shared by the four wavefronts, which split
+// the 16 x 16 sub-tiles; the chunk partials are summed by reduce_chunks in a
+// fixed order (no atomics, so the result does not depend on scheduling).
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+template <int MT, int NT>
+__global__ void __launch_bounds__(256)
+gram_block_kernel(int m, int n, long K, long kc, const cd *__restrict__ A, int lda, const cd *__restrict__ B,
+                  int ldb, cd *__restrict__ part) {
+    constexpr int KS = 16, TI = MT / 16, TJ = NT / 16, NTILE = TI * TJ, TPW = (NTILE + 3) / 4;
+    constexpr int AE = KS * MT / 256, BE = KS * NT / 256;       // staged elements per thread
+    static_assert(KS * MT % 256 == 0 && KS * NT % 256 == 0, "block must split evenly over the workgroup");
+    __shared__ cd sa[KS][MT + 1], sb[KS][NT + 1];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int i0 = blockIdx.x * MT, j0 = blockIdx.y * NT;
+    const long k0 = (long)blockIdx.z * kc, k1 = min(K, k0 + kc);
+    d4 cre[TPW], cim[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        cre[t] = d4{0.0, 0.0, 0.0, 0.0};
+        cim[t] = d4{0.0, 0.0, 0.0, 0.0};
     }
-    C[i + (size_t)j * ldc] = make_double2(re, im);
+    // columns past m (n) read the last valid one; rows past the chunk read zero
+    cd ra[AE], rb[BE];
+    auto load = [&](long s) {
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+            const int idx = tid + 256 * e, k = idx % KS, i = idx / KS;
+            const long kk = s + k;
+            ra[e] = kk < k1 ? A[kk + (size_t)min(i0 + i, m - 1) * lda] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            const int idx = tid + 256 * e, k = idx % KS, j = idx / KS;
+            const long kk = s + k;
+            rb[e] = kk < k1 ? B[kk + (size_t)min(j0 + j, n - 1) * ldb] : make_double2(0.0, 0.0);
+        }
+    };
+    load(k0);
+    for (long s = k0; s < k1; s += KS) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+            const int idx = tid + 256 * e;
+            sa[idx % KS][idx / KS] = ra[e];
+        }
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            const int idx = tid + 256 * e;
+            sb[idx % KS][idx / KS] = rb[e];
+        }
+        __syncthreads();
+        if (s + KS < k1) load(s + KS);      // next K-step in flight during the MFMAs
+#pragma unroll
+        for (int q = 0; q < KS / 4; ++q) {
+            const int k = 4 * q + (lane >> 4);
+#pragma unroll
+            for (int t = 0; t < TPW; ++t) {
+                const int tile = w + 4 * t;
+                if (tile < NTILE) {
+                    const int ti = tile % TI, tj = tile / TI;
+                    const cd a = sa[k][16 * ti + (lane & 15)], b = sb[k][16 * tj + (lane & 15)];
+                    // conj(a) b
+                    cre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, cre[t], 0, 0, 0);
+                    cre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, b.y, cre[t], 0, 0, 0);
+                    cim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.y, cim[t], 0, 0, 0);
+                    cim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, b.x, cim[t], 0, 0, 0);
+                }
+            }
+        }
+    }
+    cd *p = part + (size_t)blockIdx.z * m * n;
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        const int tile = w + 4 * t;
+        if (tile >= NTILE) continue;
+        const int ti = tile % TI, tj = tile / TI;
+        const int jj = j0 + 16 * tj + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int ii = i0 + 16 * ti + 4 * r + (lane >> 4);
+            if (ii < m && jj < n) p[ii + (size_t)jj * m] = make_double2(cre[t][r], cim[t][r]);
+        }
+    }
 }
 
-// C (m x n) = A^H B for tall A (K x m) and B (K x n): K is split into chunks
-// computed as one strided-batched GEMM (a plain GEMM here has only
-// (m/64)(n/64) output tiles and leaves most CUs idle), then reduced in FP64.
+// C[i + j ldc] = sum over chunks of part[s][i + j m]: lanes take 64 consecutive
+// elements, the four wavefronts every fourth chunk with eight loads in
+// flight, then an LDS sum in a fixed order.
+__global__ void __launch_bounds__(256)
+reduce_chunks(int m, int n, int nch, const cd *__restrict__ part, cd *__restrict__ C, int ldc) {
+    __shared__ cd red[4][64];
+    const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+    const long mn = (long)m * n, e = (long)blockIdx.x * 64 + lane;
+    const long ec = e < mn ? e : mn - 1;
+    cd acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = make_double2(0.0, 0.0);
+    for (int s0 = g; s0 < nch; s0 += 32) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int s = s0 + 4 * u;
+            if (s < nch) {
+                const cd v = part[(size_t)s * mn + ec];
+                acc[u].x += v.x;
+                acc[u].y += v.y;
+            }
+        }
+    }
+    cd t = acc[0];
+#pragma unroll
+    for (int u = 1; u < 8; ++u) {
+        t.x += acc[u].x;
+        t.y += acc[u].y;
+    }
+    red[g][lane] = t;
+    __syncthreads();
+    if (g == 0 && e < mn) {
+        cd r = red[0][lane];
+        for (int q = 1; q < 4; ++q) {
+            r.x += red[q][lane].x;
+            r.y += red[q][lane].y;
+        }
+        C[e % m + (size_t)(e / m) * ldc] = r;
+    }
+}
+
+// C (m x n) = A^H B for tall A (K x m) and B (K x n).  Output blocks of up to
+// 80 x 80; K is split so that about 256 workgroups run.
 void gram(Ctx &c, int m, int n, long K, const cd *A, int lda, const cd *B, int ldb, cd *C, int ldc) {
-    const long kc_min = 512;
-    int nsplit = (int)std::min<long>(64, std::max<long>(1, K / kc_min));
-    if (nsplit <= 1 || (long)m * n > 512L * 512L) {
-        WTB_BLAS_CHECK(rocblas_zgemm(c.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, n,
-                                     (rocblas_int)K, &kOne, rb(A), lda, rb(B), ldb, &kZero, rb(C), ldc));
-        return;
+    if (m <= 0 || n <= 0) return;
+    const int bs = std::min(80, std::max((m + 15) / 16, (n + 15) / 16) * 16);
+    const int bx = (m + bs - 1) / bs, by = (n + bs - 1) / bs;
+    int nsp = std::max(1, 256 / (bx * by));
+    long kc = (K + nsp - 1) / nsp;
+    kc = std::max<long>(16, (kc + 15) / 16 * 16);
+    nsp = (int)((K + kc - 1) / kc);
+    c.ws_gram.alloc((size_t)nsp * m * n);
+    const dim3 g(bx, by, nsp);
+    switch (bs) {
+#define WTB_GRAM_CASE(S)                                                                                     \
+    case S:                                                                                                  \
+        hipLaunchKernelGGL((gram_block_kernel<S, S>), g, dim3(256), 0, c.stream, m, n, K, kc, A, lda, B, ldb, \
+                           c.ws_gram.p);                                                                     \
+        break;
+        WTB_GRAM_CASE(16) WTB_GRAM_CASE(32) WTB_GRAM_CASE(48) WTB_GRAM_CASE(64) WTB_GRAM_CASE(80)
+#undef WTB_GRAM_CASE
     }
-    const long kc = (K + nsplit - 1) / nsplit;
-    nsplit = (int)((K + kc - 1) / kc);
-    const int full = (int)(K / kc);           // chunks of exactly kc rows
-    c.ws_gram.alloc((size_t)nsplit * m * n);
-    WTB_BLAS_CHECK(rocblas_zgemm_strided_batched(
-        c.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, n, (rocblas_int)kc, &kOne, rb(A),
-        lda, (rocblas_stride)kc, rb(B), ldb, (rocblas_stride)kc, &kZero, rb(c.ws_gram.p), m,
-        (rocblas_stride)((size_t)m * n), full));
-    if (full < nsplit) {   // the ragged last chunk
-        const long k0 = (long)full * kc;
-        WTB_BLAS_CHECK(rocblas_zgemm(c.blas, rocblas_operation_conjugate_transpose, rocblas_operation_none, m, n,
-                                     (rocblas_int)(K - k0), &kOne, rb(A + k0), lda, rb(B + k0), ldb, &kZero,
-                                     rb(c.ws_gram.p + (size_t)full * m * n), m));
-    }
-    hipLaunchKernelGGL(reduce_splitk, dim3((m + 63) / 64, n), dim3(64), 0, c.stream, m, n, nsplit, c.ws_gram.p, C,
-                       ldc);
+    WTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(reduce_chunks, dim3((unsigned)(((long)m * n + 63) / 64)), dim3(256), 0, c.stream, m, n, nsp,
+                       c.ws_gram.p, C, ldc);
     WTB_LAUNCH_CHECK();
 }
 
@@ -427,14 +555,18 @@ struct Lobpcg {
         tmv.end();
     }
 
-    // column norms^2 of (AX - lam X) or of AX when lam == nullptr
+    // column norms^2 of (AX - lam X) or of AX when lam == nullptr; with lam,
+    // the Ritz values of the last Rayleigh-Ritz step come back in the same
+    // round trip (h_lam)
     std::vector<double> norms(int ncol, const cd *X, const cd *AX, const double *lamd) {
-        WTB_HIP_CHECK(hipMemsetAsync(norm2.p, 0, sizeof(double) * ncol, c.stream));
-        unsigned gx = (unsigned)std::min<long>((N + 255) / 256, 1024);
-        hipLaunchKernelGGL(resid_norms, dim3(gx, ncol), dim3(256), 0, c.stream, N, X, AX, lamd, norm2.p);
+        hipLaunchKernelGGL(resid_norms, dim3(ncol), dim3(kNT), 0, c.stream, N, X, AX, lamd, norm2.p);
         WTB_LAUNCH_CHECK();
         std::vector<double> h(ncol);
         WTB_HIP_CHECK(hipMemcpyAsync(h.data(), norm2.p, sizeof(double) * ncol, hipMemcpyDeviceToHost, c.stream));
+        if (lamd) {
+            h_lam.resize(ncol);
+            WTB_HIP_CHECK(hipMemcpyAsync(h_lam.data(), lamd, sizeof(double) * ncol, hipMemcpyDeviceToHost, c.stream));
+        }
         WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
         return h;
     }
@@ -450,14 +582,11 @@ struct Lobpcg {
         trr.end();
     }
 
-    // scale columns of B (and B2 with the same factors) to unit norm
+    // scale columns of B (and B2 with the same factors) to unit norm, on the device
     void normalize(int nb, cd *B, cd *B2) {
-        std::vector<double> n2 = norms(nb, nullptr, B, nullptr);
-        std::vector<double> inv(nb);
-        for (int a = 0; a < nb; ++a) inv[a] = n2[a] > 0 ? 1.0 / std::sqrt(n2[a]) : 1.0;
-        WTB_HIP_CHECK(hipMemcpyAsync(sc.p, inv.data(), sizeof(double) * nb, hipMemcpyHostToDevice, c.stream));
-        hipLaunchKernelGGL(scale_cols, gN(nb), dim3(256), 0, c.stream, N, sc.p, B);
-        if (B2) hipLaunchKernelGGL(scale_cols, gN(nb), dim3(256), 0, c.stream, N, sc.p, B2);
+        hipLaunchKernelGGL(resid_norms, dim3(nb), dim3(kNT), 0, c.stream, N, nullptr, B, nullptr, norm2.p);
+        if (B2) hipLaunchKernelGGL(unit_cols, gN(nb), dim3(256), 0, c.stream, N, norm2.p, B2);
+        hipLaunchKernelGGL(unit_cols, gN(nb), dim3(256), 0, c.stream, N, norm2.p, B);
         WTB_LAUNCH_CHECK();
     }
 
@@ -546,9 +675,6 @@ struct Lobpcg {
         if (withP && m > b)
             WTB_HIP_CHECK(hipMemcpyAsync(AP.p, T1.p + (size_t)b * N, sizeof(cd) * N * b, hipMemcpyDeviceToDevice, c.stream));
         trr.end();
-        h_lam.resize(nb);
-        WTB_HIP_CHECK(hipMemcpyAsync(h_lam.data(), lam.p, sizeof(double) * nb, hipMemcpyDeviceToHost, c.stream));
-        WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
     }
 };
 
