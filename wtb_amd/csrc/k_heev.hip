@@ -808,8 +808,9 @@ cholesky_kernel(int n, cd *__restrict__ G, int ldg, double rel, int *__restrict_
 
 }  // namespace
 
-// Returns 0 on success, or 1 + the first failing column.
-int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel) {
+// Returns 0 on success, or 1 + the first failing column.  With check false the
+// pivot test is not read back (no host synchronisation) and 0 is returned.
+int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel, bool check) {
     if (n > kT * 8) throw Error{WTB_ERR_UNSUPPORTED, "Cholesky size"};
     c.ws_flag.alloc(1);
     const size_t d0 = (size_t)((n + 1) & ~1) * sizeof(double);
@@ -819,6 +820,7 @@ int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel) {
     else
         hipLaunchKernelGGL(cholesky_kernel, dim3(1), dim3(kT), d0, c.stream, n, G, ldg, rel, c.ws_flag.p);
     WTB_LAUNCH_CHECK();
+    if (!check) return 0;
     int flag = 0;
     WTB_HIP_CHECK(hipMemcpyAsync(&flag, c.ws_flag.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     WTB_HIP_CHECK(hipStreamSynchronize(c.stream));

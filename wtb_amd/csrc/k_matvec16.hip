@@ -77,22 +77,30 @@ __global__ void split_x_kernel(long N, long kpad, int ncol, const double *__rest
 // sets the power of two s with max |x| s < 2^13 (FP16 max 65504, room for the
 // products); xs = s scales the split operand, ysc = hscale / s undoes both
 // scalings on the output.
-__global__ void __launch_bounds__(256)
+constexpr int kCS = 1024;
+__global__ void __launch_bounds__(kCS)
 col_scales_kernel(long N, const double *__restrict__ X, float hscale, float *__restrict__ xs,
                   float *__restrict__ ysc) {
-    __shared__ float red[4];
+    __shared__ float red[kCS / 64];
     const int col = blockIdx.x, tid = threadIdx.x;
     const double2 *x = reinterpret_cast<const double2 *>(X) + (size_t)col * N;
     float m = 0.f;
-    for (long k = tid; k < N; k += 256) {
-        const double2 v = x[k];
-        m = fmaxf(m, fmaxf(fabsf((float)v.x), fabsf((float)v.y)));
+    // eight loads in flight per thread
+    for (long k0 = 0; k0 < N; k0 += 8 * kCS) {
+        double2 v[8];
+#pragma unroll
+        for (int t = 0; t < 8; ++t) {
+            const long k = k0 + t * kCS + tid;
+            v[t] = k < N ? x[k] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int t = 0; t < 8; ++t) m = fmaxf(m, fmaxf(fabsf((float)v[t].x), fabsf((float)v[t].y)));
     }
     for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_xor(m, off, 64));
     if ((tid & 63) == 0) red[tid >> 6] = m;
     __syncthreads();
     if (tid == 0) {
-        m = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        for (int w = 0; w < kCS / 64; ++w) m = fmaxf(m, red[w]);
         int e = 0;
         if (m > 0.f) frexpf(m, &e);          // m = f 2^e, 0.5 <= f < 1
         const float sc = ldexpf(1.0f, 13 - e);
@@ -291,7 +299,7 @@ void matvec_f16split(Ctx &c, int ncol, const cd *X, cd *Y) {
         double *y = (double *)Y + 2 * (size_t)c0 * N;
         c.ws_split.alloc((size_t)4 * nc * ld);
         c.ws_colscale.alloc(2 * (size_t)nc);
-        hipLaunchKernelGGL(col_scales_kernel, dim3(nc), dim3(256), 0, c.stream, N, x, s.hscale, c.ws_colscale.p,
+        hipLaunchKernelGGL(col_scales_kernel, dim3(nc), dim3(kCS), 0, c.stream, N, x, s.hscale, c.ws_colscale.p,
                            c.ws_colscale.p + nc);
         hipLaunchKernelGGL(split_x_kernel, dim3((unsigned)((ld + 255) / 256), nc), dim3(256), 0, c.stream, N, ld,
                            nc, x, c.ws_colscale.p, c.ws_split.p);

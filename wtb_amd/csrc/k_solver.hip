@@ -458,6 +458,101 @@ void gram(Ctx &c, int m, int n, long K, const cd *A, int lda, const cd *B, int l
     WTB_LAUNCH_CHECK();
 }
 
+// B <- B R^-1 in place for tall B (N x nb, ldb) and upper triangular R
+// (nb x nb, ldr): one thread per row of B solves y R = x by forward
+// substitution, 16 columns at a time (the solved columns are read back from
+// B, eight at a time).  R is staged in LDS, packed by columns (column j at
+// j (j + 1) / 2), with the reciprocals of its diagonal.  gridDim.y = 2
+// applies the same R to B2.
+constexpr int kTrsT = 64;
+__global__ void __launch_bounds__(kTrsT)
+trsm_ru_kernel(long N, int nb, const cd *__restrict__ R, int ldr, cd *__restrict__ B, cd *__restrict__ B2, long ldb) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cd *rp = reinterpret_cast<cd *>(smem);                       // packed upper R
+    double *rinv = reinterpret_cast<double *>(rp + (size_t)nb * (nb + 1) / 2);
+    {   // stage R: eight loads in flight per thread
+        const int tot = nb * nb;
+        for (int base = 0; base < tot; base += 8 * kTrsT) {
+            cd v[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int idx = min(base + t * kTrsT + (int)threadIdx.x, tot - 1);
+                v[t] = R[idx % nb + (size_t)(idx / nb) * ldr];
+            }
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int idx = base + t * kTrsT + threadIdx.x, i = idx % nb, j = idx / nb;
+                if (idx < tot && i <= j) {
+                    rp[(size_t)j * (j + 1) / 2 + i] = v[t];
+                    if (i == j) rinv[j] = 1.0 / v[t].x;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    cd *M = blockIdx.y ? B2 : B;
+    const long row = (long)blockIdx.x * kTrsT + threadIdx.x;
+    if (row >= N) return;
+    cd *x = M + row;
+    constexpr int JB = 16;
+    for (int j0 = 0; j0 < nb; j0 += JB) {
+        cd y[JB];
+#pragma unroll
+        for (int u = 0; u < JB; ++u) y[u] = j0 + u < nb ? x[(size_t)(j0 + u) * ldb] : make_double2(0.0, 0.0);
+        // x_J -= y_k R[k, J] for the solved columns k < j0 (read back eight at a time)
+        for (int k0 = 0; k0 < j0; k0 += 8) {
+            cd yk[8];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) yk[t] = x[(size_t)min(k0 + t, j0 - 1) * ldb];
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                const int k = k0 + t;
+                if (k < j0) {
+#pragma unroll
+                    for (int u = 0; u < JB; ++u) {
+                        const int j = min(j0 + u, nb - 1);
+                        const cd r = rp[(size_t)j * (j + 1) / 2 + k];
+                        y[u].x -= yk[t].x * r.x - yk[t].y * r.y;
+                        y[u].y -= yk[t].x * r.y + yk[t].y * r.x;
+                    }
+                }
+            }
+        }
+        // substitution inside the block
+#pragma unroll
+        for (int u = 0; u < JB; ++u) {
+            const int j = j0 + u;
+            if (j < nb) {
+#pragma unroll
+                for (int w = 0; w < u; ++w) {
+                    const cd r = rp[(size_t)j * (j + 1) / 2 + j0 + w];
+                    y[u].x -= y[w].x * r.x - y[w].y * r.y;
+                    y[u].y -= y[w].x * r.y + y[w].y * r.x;
+                }
+                const double ri = rinv[j];
+                y[u].x *= ri;
+                y[u].y *= ri;
+                x[(size_t)j * ldb] = y[u];
+            }
+        }
+    }
+}
+
+void trsm_right_upper(Ctx &c, long N, int nb, const cd *R, int ldr, cd *B, cd *B2) {
+    const size_t sh = (size_t)nb * (nb + 1) / 2 * sizeof(cd) + (size_t)nb * sizeof(double);
+    if (sh > 150 * 1024) {      // R too large to stage: library triangular solve
+        for (cd *M : {B, B2})
+            if (M)
+                WTB_BLAS_CHECK(rocblas_ztrsm(c.blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
+                                             rocblas_diagonal_non_unit, (rocblas_int)N, nb, &kOne, rb(R), ldr, rb(M),
+                                             (rocblas_int)N));
+        return;
+    }
+    hipLaunchKernelGGL(trsm_ru_kernel, dim3((unsigned)((N + kTrsT - 1) / kTrsT), B2 ? 2 : 1), dim3(kTrsT), sh,
+                       c.stream, N, nb, R, ldr, B, B2, N);
+    WTB_LAUNCH_CHECK();
+}
+
 // C (M x n) = A (M x k) B (k x n)
 void gemm_nn(Ctx &c, long M, int n, int k, const cd *A, long lda, const cd *B, int ldb, cd *C, long ldc) {
     WTB_BLAS_CHECK(rocblas_zgemm(c.blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, n, k,
@@ -604,18 +699,14 @@ struct Lobpcg {
             gram(c, nb, nb, N, B, (int)N, B, (int)N, Tmp.p, m3);
             hipLaunchKernelGGL(hermitize, dim3((nb + 63) / 64, nb), dim3(64), 0, c.stream, nb, Tmp.p, m3);
             WTB_LAUNCH_CHECK();
-            int fail = cholesky_upper(c, nb, Tmp.p, m3, piv);
+            // the second pass factors I + O(eps cond^2) and cannot fail once the
+            // first has passed: only the first pass's pivot test is read back
+            int fail = cholesky_upper(c, nb, Tmp.p, m3, piv, pass == 0);
             if (fail) {
                 trr.end();
                 return false;
             }
-            WTB_BLAS_CHECK(rocblas_ztrsm(c.blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                                         rocblas_diagonal_non_unit, (rocblas_int)N, nb, &kOne, rb(Tmp.p), m3, rb(B),
-                                         (rocblas_int)N));
-            if (B2)
-                WTB_BLAS_CHECK(rocblas_ztrsm(c.blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
-                                             rocblas_diagonal_non_unit, (rocblas_int)N, nb, &kOne, rb(Tmp.p), m3,
-                                             rb(B2), (rocblas_int)N));
+            trsm_right_upper(c, N, nb, Tmp.p, m3, B, B2);
             trr.end();
         }
         return true;
