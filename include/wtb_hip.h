@@ -182,6 +182,25 @@ typedef struct {
 } wtb_grad;
 int wtb_grad_energies(wtb_ctx *ctx, const double *w, wtb_grad *g, double *hop_re_grad, double *hop_im_grad);
 
+/* Gradients of the spectrum loss L = sum_j (eps2(omega[j]) - target[j])^2 over
+ * the nev states of the last wtb_bse_solve, with
+ *   eps2(w) = 180.90 / (N_k V_cell) sum_n f_n gamma / ((w - E_n)^2 + gamma^2)
+ * (the Lorentzian of optdiel, module_optics.F90:171-241, at a fixed width) and
+ * f_n = |sum_i x_n[i] d_i|^2 (reference convention, dielbsev,
+ * module_bse.F90:231-238) or |x_n^H d|^2 (textbook convention), d the dipoles
+ * of Cartesian component comp (0..2) with broadening cshift (wtb_optics).
+ * Eigenvector derivatives come from the other solved states plus a
+ * preconditioned CG solve on the complement of them, so the solved states
+ * must be the lowest nev with a gap above.  No counterpart in the reference;
+ * its callers are fits of the screening, the scissor and the hoppings to a
+ * measured absorption spectrum.  In the reference convention the strengths
+ * depend on the TB eigenvector phases, so only the Coulomb and scissor
+ * gradients are defined there (hop_re_grad / hop_im_grad must be NULL).
+ *   omega, target : double (nw);  loss : L;  eps2 : double (nw) or NULL */
+int wtb_grad_spectrum(wtb_ctx *ctx, int comp, double cshift, double gamma, int nw, const double *omega,
+                      const double *target, double *loss, double *eps2, wtb_grad *g, double *hop_re_grad,
+                      double *hop_im_grad);
+
 /* Drop-in for slepc_solver(HX,EX,HE,NX,NE) (module_slepc_solver.F90:12-97):
  * uploads the caller's full host matrix HX(NX,NX), returns the NE
  * smallest-magnitude eigenpairs.  EX(NE) ascending in |E|; HE(NX,NE). */

@@ -309,11 +309,14 @@ __device__ inline void atomic_max_nonneg(double *addr, double v) {
 
 // TB eigenvector adjoint, one wavefront per k.  W: [(j n + o) nk + k] (Ubar of
 // selected band j).  acc: [0] dL/dscissor, [1] dL/dE_F, [2] max |Im u_m^H ubar_m|.
+// ebx, ubx (optional): further dL/dE_j(k) [k nb + j] and Ubar contributions in
+// the layout of W (from the dipole adjoint).
 __global__ void __launch_bounds__(64)
 tb_adjoint_kernel(int nk, int n, int nv, int nc, int nst, double scissor, double degtol,
                   const double *__restrict__ eall, const cd *__restrict__ uall, const int *__restrict__ nocpk,
                   const cd *__restrict__ W, const cd *__restrict__ X, long N, const int *__restrict__ sidx,
-                  const double *__restrict__ sw, cd *__restrict__ Hbar, double *__restrict__ acc) {
+                  const double *__restrict__ sw, const double *__restrict__ ebx, const cd *__restrict__ ubx,
+                  cd *__restrict__ Hbar, double *__restrict__ acc) {
     constexpr int MO = kMaxOrb, MB = 8;
     __shared__ cd U[MO * MO];        // U[o * n + l]
     __shared__ cd Ub[MO * MB];       // Ubar[o * nb + j]
@@ -328,7 +331,9 @@ tb_adjoint_kernel(int nk, int n, int nv, int nc, int nst, double scissor, double
     for (int l = tid; l < n; l += 64) lam[l] = eall[(size_t)k * n + l] - (l >= nocp ? scissor : 0.0);
     for (int idx = tid; idx < n * nb; idx += 64) {
         const int o = idx / nb, j = idx % nb;
-        Ub[idx] = W[((size_t)j * n + o) * nk + k];
+        const size_t wi = ((size_t)j * n + o) * nk + k;
+        Ub[idx] = W ? W[wi] : make_double2(0.0, 0.0);
+        if (ubx) Ub[idx] = cadd(Ub[idx], ubx[wi]);
     }
     if (tid < nb) {
         // dL/dE_j: +sum over v for a conduction band, -sum over c for a valence band
@@ -344,6 +349,7 @@ tb_adjoint_kernel(int nk, int n, int nv, int nc, int nst, double scissor, double
             }
             s += sw[st] * t;
         }
+        if (ebx) s += ebx[(size_t)k * nb + tid];
         lbar[tid] = s;
     }
     __syncthreads();
@@ -403,21 +409,32 @@ tb_adjoint_kernel(int nk, int n, int nv, int nc, int nst, double scissor, double
 // g(R)_ab = sum_k Hbar(k)_ab e^{-i k.R} / deg(R), with the phase of the
 // forward Fourier sum (float32-rounded in compat mode).  Output in the
 // reference layout (nvec, norb, norb), column-major.
+// With Dbar (the adjoint of the dipole matrix dH/dk_alpha(k) = sum_R i R_alpha
+// e^{ik.R} H(R)/deg), the term sum_k Dbar(k) (-i R_alpha) e^{-ik.R}/deg is added,
+// R_alpha rounded as in the dipole kernel.
 __global__ void fourier_adjoint_kernel(int nk, int n, int nvec, int compat, const double *__restrict__ kpt,
                                        const double *__restrict__ rcart, const double *__restrict__ invdeg,
-                                       const cd *__restrict__ Hbar, double *__restrict__ gre,
-                                       double *__restrict__ gim) {
+                                       const cd *__restrict__ Hbar, const cd *__restrict__ Dbar, int alpha,
+                                       double *__restrict__ gre, double *__restrict__ gim) {
     const int nn = n * n;
     const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
     if (t >= (long)nvec * nn) return;
     const int r = (int)(t / nn), idx = (int)(t % nn), a = idx / n, b = idx % n;
+    double Ra = Dbar ? rcart[3 * r + alpha] : 0.0;
+    if (compat) Ra = (double)(float)Ra;
     double ar = 0.0, ai = 0.0;
     for (int k = 0; k < nk; ++k) {
         double x = kdotr(kpt[3 * k], kpt[3 * k + 1], kpt[3 * k + 2], rcart + 3 * r);
         if (compat) x = (double)(float)x;
         double sn, cs;
         sincos(x, &sn, &cs);
-        const cd h = Hbar[(size_t)k * nn + idx];
+        cd h = Hbar[(size_t)k * nn + idx];
+        if (Dbar) {
+            // + Dbar (-i R_a) = + R_a (Dbar.y, -Dbar.x)
+            const cd db = Dbar[(size_t)k * nn + idx];
+            h.x += Ra * db.y;
+            h.y -= Ra * db.x;
+        }
         ar += h.x * cs + h.y * sn;
         ai += h.y * cs - h.x * sn;
     }
@@ -427,21 +444,21 @@ __global__ void fourier_adjoint_kernel(int nk, int n, int nvec, int compat, cons
 }
 
 template <int NC, int NV>
-void launch_pair(Ctx &c, const CoulombDev &cd_, int nst, int accumulate, const int *sidx, const double *sw,
-                 cd *G, double *dcoul) {
+void launch_pair(Ctx &c, const CoulombDev &cd_, const cd *X, int nst, int accumulate, const int *sidx,
+                 const double *sw, cd *G, double *dcoul) {
     const int nk = c.bands.nk, norb = c.model.norb;
     const size_t shmem = ((size_t)norb * (NC + NV) + (size_t)nst * NC * NV) * sizeof(cd);
     hipLaunchKernelGGL((grad_pair_kernel<NC, NV>), dim3(nk), dim3(kPT), shmem, c.stream, nk, norb, nst,
-                       c.bse.convention == WTB_CONV_TEXTBOOK ? 1 : 0, accumulate, cd_, c.kpt.p, c.bands.usel.p,
-                       c.sol.evecs.p, c.bse.N, sidx, sw, G, dcoul);
+                       c.bse.convention == WTB_CONV_TEXTBOOK ? 1 : 0, accumulate, cd_, c.kpt.p, c.bands.usel.p, X,
+                       c.bse.N, sidx, sw, G, dcoul);
     WTB_LAUNCH_CHECK();
 }
 
-void dispatch_pair(Ctx &c, const CoulombDev &cd_, int nst, int accumulate, const int *sidx, const double *sw,
-                   cd *G, double *dcoul) {
+void dispatch_pair(Ctx &c, const CoulombDev &cd_, const cd *X, int nst, int accumulate, const int *sidx,
+                   const double *sw, cd *G, double *dcoul) {
     const int nc = c.bands.nc, nv = c.bands.nv;
 #define WTB_PAIR_CASE(A, B) \
-    if (nc == A && nv == B) return launch_pair<A, B>(c, cd_, nst, accumulate, sidx, sw, G, dcoul);
+    if (nc == A && nv == B) return launch_pair<A, B>(c, cd_, X, nst, accumulate, sidx, sw, G, dcoul);
     WTB_PAIR_CASE(1, 1) WTB_PAIR_CASE(1, 2) WTB_PAIR_CASE(2, 1) WTB_PAIR_CASE(2, 2)
     WTB_PAIR_CASE(1, 3) WTB_PAIR_CASE(3, 1) WTB_PAIR_CASE(2, 3) WTB_PAIR_CASE(3, 2) WTB_PAIR_CASE(3, 3)
     WTB_PAIR_CASE(1, 4) WTB_PAIR_CASE(4, 1) WTB_PAIR_CASE(2, 4) WTB_PAIR_CASE(4, 2) WTB_PAIR_CASE(3, 4)
@@ -452,15 +469,19 @@ void dispatch_pair(Ctx &c, const CoulombDev &cd_, int nst, int accumulate, const
 
 }  // namespace
 
-void grad_energies(Ctx &c, const double *w, wtb_grad *out, double *g_re, double *g_im) {
-    if (!c.sol.valid || !c.bse.valid || !c.bands.valid) throw Error{WTB_ERR_STATE, "wtb_bse_solve must run first"};
+// dL for L = sum_s w_s x_s^H H x_s over the columns of X (N x nx, ld N),
+// plus the optional dipole-adjoint terms (GradExtra).  grad_energies is the
+// case X = the solved eigenvectors.
+void grad_core(Ctx &c, const cd *X, int nx, const double *w, const GradExtra *ex, wtb_grad *out, double *g_re,
+               double *g_im) {
+    if (!c.bse.valid || !c.bands.valid) throw Error{WTB_ERR_STATE, "wtb_bse_build must run first"};
     if (!w || !out) throw Error{WTB_ERR_ARG, "null weights or output"};
     Bands &b = c.bands;
     const int nk = b.nk, nc = b.nc, nv = b.nv, nb = nc + nv, n = c.model.norb;
     if (nb > 8) throw Error{WTB_ERR_UNSUPPORTED, "nc + nv > 8 in the adjoint"};
     std::vector<int> idx;
     std::vector<double> ws;
-    for (int s = 0; s < c.sol.nev; ++s)
+    for (int s = 0; s < nx; ++s)
         if (w[s] != 0.0) {
             if (!std::isfinite(w[s])) throw Error{WTB_ERR_ARG, "non-finite weight"};
             idx.push_back(s);
@@ -475,49 +496,54 @@ void grad_energies(Ctx &c, const double *w, wtb_grad *out, double *g_re, double 
     scal.alloc(kNP + 3);
     WTB_HIP_CHECK(hipMemsetAsync(scal.p, 0, sizeof(double) * (kNP + 3), c.stream));
     Hbar.alloc((size_t)nk * n * n);
-    if (idx.empty()) {
+    const bool extra = ex && (ex->ebar || ex->ubar);
+    if (idx.empty() && !extra) {
         WTB_HIP_CHECK(hipMemsetAsync(Hbar.p, 0, sizeof(cd) * nk * n * n, c.stream));
     } else {
-        didx.alloc(idx.size());
-        dw.alloc(ws.size());
-        WTB_HIP_CHECK(hipMemcpyAsync(didx.p, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, c.stream));
-        WTB_HIP_CHECK(hipMemcpyAsync(dw.p, ws.data(), sizeof(double) * ws.size(), hipMemcpyHostToDevice, c.stream));
-        const CoulombDev cd_ = make_coulomb(c.bse.coul, c.model.rlat, c.ngrid);
-        G.alloc((size_t)npq * nk * nk);
-        for (size_t s0 = 0; s0 < idx.size(); s0 += kMaxStates) {
-            const int ns = (int)std::min<size_t>(kMaxStates, idx.size() - s0);
-            dispatch_pair(c, cd_, ns, s0 > 0, didx.p + s0, dw.p + s0, G.p, scal.p);
-        }
-        // Ubar_j(k) = sum_{j'} A_{jj'} U_{j'}  (textbook valence: sum_{j'} A_{j'j}^T U_{j'})
-        Wb.alloc((size_t)nb * n * nk);
-        const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
-        const int ldu = nb * nk;
-        const bool textbook = c.bse.convention == WTB_CONV_TEXTBOOK;
-        auto mat = [&](int p) { return (const rocblas_double_complex *)(G.p + (size_t)p * nk * nk); };
-        auto ucol = [&](int j) { return (const rocblas_double_complex *)(b.usel.p + (size_t)j * nk); };
-        for (int j = 0; j < nb; ++j) {
-            rocblas_double_complex *Wj = (rocblas_double_complex *)(Wb.p + (size_t)j * n * nk);
-            const bool cond = j >= nv;
-            const int m = cond ? nc : nv;
-            for (int j2 = 0; j2 < m; ++j2) {
-                int p;
-                rocblas_operation op = rocblas_operation_none;
-                if (cond) {
-                    p = (j - nv) * nc + j2;
-                } else if (!textbook) {
-                    p = nc * nc + j * nv + j2;
-                } else {
-                    p = nc * nc + j2 * nv + j;
-                    op = rocblas_operation_transpose;
+        if (!idx.empty()) {
+            didx.alloc(idx.size());
+            dw.alloc(ws.size());
+            WTB_HIP_CHECK(
+                hipMemcpyAsync(didx.p, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, c.stream));
+            WTB_HIP_CHECK(hipMemcpyAsync(dw.p, ws.data(), sizeof(double) * ws.size(), hipMemcpyHostToDevice, c.stream));
+            const CoulombDev cd_ = make_coulomb(c.bse.coul, c.model.rlat, c.ngrid);
+            G.alloc((size_t)npq * nk * nk);
+            for (size_t s0 = 0; s0 < idx.size(); s0 += kMaxStates) {
+                const int ns = (int)std::min<size_t>(kMaxStates, idx.size() - s0);
+                dispatch_pair(c, cd_, X, ns, s0 > 0, didx.p + s0, dw.p + s0, G.p, scal.p);
+            }
+            // Ubar_j(k) = sum_{j'} A_{jj'} U_{j'}  (textbook valence: sum_{j'} A_{j'j}^T U_{j'})
+            Wb.alloc((size_t)nb * n * nk);
+            const rocblas_double_complex one = {1.0, 0.0}, zero = {0.0, 0.0};
+            const int ldu = nb * nk;
+            const bool textbook = c.bse.convention == WTB_CONV_TEXTBOOK;
+            auto mat = [&](int p) { return (const rocblas_double_complex *)(G.p + (size_t)p * nk * nk); };
+            auto ucol = [&](int j) { return (const rocblas_double_complex *)(b.usel.p + (size_t)j * nk); };
+            for (int j = 0; j < nb; ++j) {
+                rocblas_double_complex *Wj = (rocblas_double_complex *)(Wb.p + (size_t)j * n * nk);
+                const bool cond = j >= nv;
+                const int m = cond ? nc : nv;
+                for (int j2 = 0; j2 < m; ++j2) {
+                    int p;
+                    rocblas_operation op = rocblas_operation_none;
+                    if (cond) {
+                        p = (j - nv) * nc + j2;
+                    } else if (!textbook) {
+                        p = nc * nc + j * nv + j2;
+                    } else {
+                        p = nc * nc + j2 * nv + j;
+                        op = rocblas_operation_transpose;
+                    }
+                    const int src = cond ? nv + j2 : j2;
+                    WTB_BLAS_CHECK(rocblas_zgemm(c.blas, op, rocblas_operation_none, nk, n, nk, &one, mat(p), nk,
+                                                 ucol(src), ldu, j2 == 0 ? &zero : &one, Wj, nk));
                 }
-                const int src = cond ? nv + j2 : j2;
-                WTB_BLAS_CHECK(rocblas_zgemm(c.blas, op, rocblas_operation_none, nk, n, nk, &one, mat(p), nk,
-                                             ucol(src), ldu, j2 == 0 ? &zero : &one, Wj, nk));
             }
         }
         hipLaunchKernelGGL(tb_adjoint_kernel, dim3(nk), dim3(64), 0, c.stream, nk, n, nv, nc, (int)idx.size(),
-                           c.model.scissor, 1e-8, b.eall.p, b.uall.p, b.nocp.p, Wb.p, c.sol.evecs.p, c.bse.N,
-                           didx.p, dw.p, Hbar.p, scal.p + kNP);
+                           c.model.scissor, 1e-8, b.eall.p, b.uall.p, b.nocp.p, idx.empty() ? nullptr : Wb.p, X,
+                           c.bse.N, didx.p, dw.p, ex ? ex->ebar : nullptr, ex ? ex->ubar : nullptr, Hbar.p,
+                           scal.p + kNP);
         WTB_LAUNCH_CHECK();
     }
     if (g_re || g_im) {
@@ -526,7 +552,8 @@ void grad_energies(Ctx &c, const double *w, wtb_grad *out, double *g_re, double 
         gdev.alloc(2 * tot);
         hipLaunchKernelGGL(fourier_adjoint_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, nk, n,
                            nvec, (c.model.flags & WTB_FLAG_COMPAT_F32) ? 1 : 0, c.kpt.p, c.model.rcart.p,
-                           c.model.invdeg.p, Hbar.p, gdev.p, gdev.p + tot);
+                           c.model.invdeg.p, Hbar.p, ex ? ex->dbar : nullptr, ex ? ex->alpha : 0, gdev.p,
+                           gdev.p + tot);
         WTB_LAUNCH_CHECK();
         if (g_re) WTB_HIP_CHECK(hipMemcpyAsync(g_re, gdev.p, sizeof(double) * tot, hipMemcpyDeviceToHost, c.stream));
         if (g_im)
@@ -539,6 +566,11 @@ void grad_energies(Ctx &c, const double *w, wtb_grad *out, double *g_re, double 
     out->scissor = h[kNP];
     out->efermi = h[kNP + 1];
     out->gauge_residual = h[kNP + 2];
+}
+
+void grad_energies(Ctx &c, const double *w, wtb_grad *out, double *g_re, double *g_im) {
+    if (!c.sol.valid) throw Error{WTB_ERR_STATE, "wtb_bse_solve must run first"};
+    grad_core(c, c.sol.evecs.p, c.sol.nev, w, nullptr, out, g_re, g_im);
 }
 
 }  // namespace wtb

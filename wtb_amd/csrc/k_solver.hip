@@ -1003,4 +1003,15 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     (void)hipEventDestroy(tr);
 }
 
+// Tall-skinny products for other files (the adjoint's linear solves).
+void gram_tall(Ctx &c, int m, int n, long K, const cd *A, int lda, const cd *B, int ldb, cd *C, int ldc) {
+    gram(c, m, n, K, A, lda, B, ldb, C, ldc);
+}
+void gemm_tall_sub(Ctx &c, long M, int n, int k, const cd *A, long lda, const cd *B, int ldb, cd *C, long ldc) {
+    gemm_nn_sub(c, M, n, k, A, lda, B, ldb, C, ldc);
+}
+void gemm_tall(Ctx &c, long M, int n, int k, const cd *A, long lda, const cd *B, int ldb, cd *C, long ldc) {
+    gemm_nn(c, M, n, k, A, lda, B, ldb, C, ldc);
+}
+
 }  // namespace wtb

@@ -328,6 +328,14 @@ int wtb_grad_energies(wtb_ctx *ctx, const double *w, wtb_grad *g, double *hop_re
     return guarded(ctx, [&](Ctx &c) { grad_energies(c, w, g, hop_re_grad, hop_im_grad); });
 }
 
+int wtb_grad_spectrum(wtb_ctx *ctx, int comp, double cshift, double gamma, int nw, const double *omega,
+                      const double *target, double *loss, double *eps2, wtb_grad *g, double *hop_re_grad,
+                      double *hop_im_grad) {
+    return guarded(ctx, [&](Ctx &c) {
+        grad_spectrum(c, comp, cshift, gamma, nw, omega, target, loss, eps2, g, hop_re_grad, hop_im_grad);
+    });
+}
+
 namespace {
 __global__ void upload_diag(long N, long ld, const cd *__restrict__ H, double *__restrict__ d) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;

@@ -233,11 +233,30 @@ int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel, bool check = true)
 
 // Copy H(i0:i0+ni, j0:j0+nj) of the stored matrix into out (ni x nj, ldo) as FP64.
 // gradients of sum_n w_n E_n (k_grad.hip)
+// Extra inputs of the quadratic-form adjoint from the dipole adjoint:
+// dL/dE_j(k) [k nb + j], Ubar in the layout [(j norb + o) nk + k], and the
+// adjoint Dbar(k) [k norb^2 + a norb + b] of dH/dk_alpha (any may be null).
+struct GradExtra {
+    const double *ebar = nullptr;
+    const cd *ubar = nullptr;
+    const cd *dbar = nullptr;
+    int alpha = 0;
+};
+void grad_core(Ctx &c, const cd *X, int nx, const double *w, const GradExtra *ex, wtb_grad *out, double *g_re,
+               double *g_im);
 void grad_energies(Ctx &c, const double *w, wtb_grad *out, double *g_re, double *g_im);
 void bse_block_to_c64(Ctx &c, long i0, long ni, long j0, long nj, cd *out, long ldo);
 
 // Block eigensolver driver (LOBPCG with soft locking, diagonal preconditioner).
 void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *rep);
+// C (m x n) = A^H B for tall A (K x m), B (K x n), FP64 MFMA
+void gram_tall(Ctx &c, int m, int n, long K, const cd *A, int lda, const cd *B, int ldb, cd *C, int ldc);
+// C (M x n) -= A (M x k) B (k x n);  C = A B
+void gemm_tall_sub(Ctx &c, long M, int n, int k, const cd *A, long lda, const cd *B, int ldb, cd *C, long ldc);
+void gemm_tall(Ctx &c, long M, int n, int k, const cd *A, long lda, const cd *B, int ldb, cd *C, long ldc);
+// gradient of the spectrum loss sum_j (eps2(w_j) - t_j)^2 (k_spec.hip)
+void grad_spectrum(Ctx &c, int comp, double cshift, double gamma, int nw, const double *omega, const double *target,
+                   double *loss, double *eps2, wtb_grad *out, double *g_re, double *g_im);
 
 float elapsed_ms(hipEvent_t a, hipEvent_t b);
 

@@ -4,7 +4,11 @@
 pipeline (H(k) eigensolves, BSE assembly, lowest eigenpairs) for given
 screening parameters, scissor shift and TB hoppings, and whose backward calls
 the library's Hellmann-Feynman adjoint (wtb_grad_energies) with the incoming
-gradient as the state weights.  `fit` is a small optimisation loop on top.
+gradient as the state weights.  `SpectrumLoss` is the same map followed by the
+Lorentzian absorption spectrum of the solved states and the squared misfit to
+a target spectrum (wtb_grad_spectrum returns the loss with its gradient, so the
+backward only scales it).  `fit` and `fit_spectrum` are small optimisation
+loops on top.
 
 Parameters of the differentiable map:
   coul    (7,)  r0, EDIEL_T, EDIEL, EDIEL_B, EDIEL_Z, W_COUL, LC
@@ -60,6 +64,12 @@ class ExcitonProblem:
     def grad(self, w):
         return self.ctx.grad_energies(np.asarray(w, np.float64))
 
+    def spectrum_grad(self, spec, hop=True):
+        """Loss, eps2 and gradients of the spectrum misfit for the held solve;
+        spec = dict(omega, target, comp, cshift, gamma)."""
+        return self.ctx.grad_spectrum(spec["omega"], spec["target"], spec.get("comp", "x"),
+                                      spec.get("cshift", 0.05), spec.get("gamma", 0.05), hop=hop)
+
     def close(self):
         self.ctx.close()
 
@@ -91,6 +101,71 @@ class ExcitonEnergies(torch.autograd.Function):
         gh = g["hop"]
         return (None, gc, gs, torch.as_tensor(gh.real.copy(), device=dev_h),
                 torch.as_tensor(gh.imag.copy(), device=dev_h))
+
+
+class SpectrumLoss(torch.autograd.Function):
+    """L = SpectrumLoss.apply(problem, spec, coul, scissor, hop_re, hop_im):
+    sum_j (eps2(omega_j) - target_j)^2 over the problem's nev states, spec as
+    in ExcitonProblem.spectrum_grad.  The H(R) gradient is computed only when
+    hop_re or hop_im requires it (the reference convention has none)."""
+
+    @staticmethod
+    def forward(ctx, problem, spec, coul, scissor, hop_re, hop_im):
+        c = coul.detach().cpu().double().numpy()
+        s = float(scissor.detach().cpu())
+        h = hop_re.detach().cpu().double().numpy() + 1j * hop_im.detach().cpu().double().numpy()
+        problem.solve(c, s, h)
+        need_hop = ctx.needs_input_grad[4] or ctx.needs_input_grad[5]
+        g = problem.spectrum_grad(spec, hop=need_hop)
+        ctx.g = g
+        ctx.meta = (coul.device, hop_re.device, need_hop)
+        return torch.tensor(g["loss"], dtype=torch.float64, device=coul.device)
+
+    @staticmethod
+    def backward(ctx, gL):
+        g = ctx.g
+        dev_c, dev_h, need_hop = ctx.meta
+        a = float(gL)
+        gc = a * torch.tensor([g["coulomb"][k] for k in hip.COULOMB_PARAMS], dtype=torch.float64, device=dev_c)
+        gs = a * torch.tensor(g["scissor"], dtype=torch.float64, device=dev_c)
+        ghr = ghi = None
+        if need_hop:
+            ghr = a * torch.as_tensor(g["hop"].real.copy(), device=dev_h)
+            ghi = a * torch.as_tensor(g["hop"].imag.copy(), device=dev_h)
+        return None, None, gc, gs, ghr, ghi
+
+
+def fit_spectrum(problem, spec, coul, scissor, hop_re, hop_im, fit_coul=(0,), fit_scissor=False, fit_hop=False,
+                 steps=30, lr=1.0):
+    """Least-squares fit of the absorption spectrum to spec["target"] over the
+    selected parameters with L-BFGS.  Returns the fitted tensors and the loss
+    history."""
+    coul = coul.clone().double()
+    scissor = scissor.clone().double()
+    hop_re, hop_im = hop_re.clone().double(), hop_im.clone().double()
+    mask = torch.zeros(7, dtype=torch.float64)
+    mask[list(fit_coul)] = 1.0
+    coul.requires_grad_(len(fit_coul) > 0)
+    scissor.requires_grad_(fit_scissor)
+    hop_re.requires_grad_(fit_hop)
+    hop_im.requires_grad_(fit_hop)
+    params = [p for p in (coul, scissor, hop_re, hop_im) if p.requires_grad]
+    opt = torch.optim.LBFGS(params, lr=lr, max_iter=steps, line_search_fn="strong_wolfe",
+                            tolerance_grad=1e-12, tolerance_change=1e-16)
+    history = []
+
+    def closure():
+        opt.zero_grad()
+        loss = SpectrumLoss.apply(problem, spec, coul, scissor, hop_re, hop_im)
+        loss.backward()
+        if coul.grad is not None:
+            coul.grad.mul_(mask)
+        history.append(float(loss))
+        return loss
+
+    opt.step(closure)
+    return dict(coul=coul.detach(), scissor=scissor.detach(), hop_re=hop_re.detach(), hop_im=hop_im.detach(),
+                history=history)
 
 
 def fit(problem, target, coul, scissor, hop_re, hop_im, fit_coul=(0,), fit_scissor=False, fit_hop=False,

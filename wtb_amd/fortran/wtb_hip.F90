@@ -158,6 +158,20 @@ module wtb_hip
             type(c_ptr), value :: hop_re_grad, hop_im_grad
         end function wtb_grad_energies
 
+        ! gradients of sum_j (eps2(omega(j)) - target(j))^2; eps2: c_loc of (nw) or c_null_ptr
+        integer(c_int) function wtb_grad_spectrum(ctx, comp, cshift, gamma, nw, omega, target, loss, eps2, g, &
+                                                  hop_re_grad, hop_im_grad) bind(C, name="wtb_grad_spectrum")
+            import :: c_int, c_ptr, c_double, wtb_grad
+            type(c_ptr), value :: ctx
+            integer(c_int), value :: comp, nw
+            real(c_double), value :: cshift, gamma
+            real(c_double), intent(in) :: omega(*), target(*)
+            real(c_double), intent(out) :: loss
+            type(c_ptr), value :: eps2
+            type(wtb_grad), intent(out) :: g
+            type(c_ptr), value :: hop_re_grad, hop_im_grad
+        end function wtb_grad_spectrum
+
         ! literal slepc_solver(HX,EX,HE,NX,NE) drop-in: uploads the host matrix
         integer(c_int) function wtb_slepc_solver_compat(hx, ex, he, nx, ne) bind(C, name="wtb_slepc_solver_compat")
             import :: c_int, c_double_complex, c_double

@@ -30,7 +30,7 @@ COMPONENTS = {"x": 0, "y": 1, "z": 2, "sp": 3, "sm": 4}
 EXPORTS = ["wtb_create", "wtb_destroy", "wtb_last_error", "wtb_set_tb_model", "wtb_set_kpoints",
            "wtb_hk_eig", "wtb_get_bands", "wtb_bse_build", "wtb_bse_solve", "wtb_optics",
            "wtb_oscillator", "wtb_bse_get_block", "wtb_bse_matvec", "wtb_bse_diag",
-           "wtb_slepc_solver_compat", "wtb_grad_energies"]
+           "wtb_slepc_solver_compat", "wtb_grad_energies", "wtb_grad_spectrum"]
 
 
 class WTBError(RuntimeError):
@@ -93,6 +93,7 @@ def load():
         "wtb_bse_diag": [P, P],
         "wtb_slepc_solver_compat": [P, P, P, I, I],
         "wtb_grad_energies": [P, P, C.POINTER(GradScalars), P, P],
+        "wtb_grad_spectrum": [P, I, D, D, I, P, P, P, P, C.POINTER(GradScalars), P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -242,6 +243,30 @@ class Context:
                                                _ptr(gre) if hop else None, _ptr(gim) if hop else None))
         out = {"coulomb": dict(zip(COULOMB_PARAMS, list(g.coulomb))), "scissor": g.scissor,
                "efermi": g.efermi, "gauge_residual": g.gauge_residual}
+        if hop:
+            out["hop"] = gre + 1j * gim
+        return out
+
+    def grad_spectrum(self, omega, target, comp="x", cshift=0.05, gamma=0.05, hop=True):
+        """Spectrum loss L = sum_j (eps2(omega_j) - target_j)^2 over the states
+        of the last solve and its gradient (the keys of grad_energies plus
+        'loss' and 'eps2')."""
+        omega = np.ascontiguousarray(omega, np.float64)
+        target = np.ascontiguousarray(target, np.float64)
+        if omega.shape != target.shape or omega.ndim != 1:
+            raise ValueError("omega and target must be 1-D arrays of one length")
+        g = GradScalars()
+        loss = C.c_double(0.0)
+        eps2 = np.zeros(len(omega))
+        gre = gim = None
+        if hop:
+            gre = np.zeros((self.nvec, self.norb, self.norb), order="F")
+            gim = np.zeros((self.nvec, self.norb, self.norb), order="F")
+        self._check(self.lib.wtb_grad_spectrum(self.h, COMPONENTS[comp], cshift, gamma, len(omega), _ptr(omega),
+                                               _ptr(target), C.byref(loss), _ptr(eps2), C.byref(g),
+                                               _ptr(gre) if hop else None, _ptr(gim) if hop else None))
+        out = {"loss": loss.value, "eps2": eps2, "coulomb": dict(zip(COULOMB_PARAMS, list(g.coulomb))),
+               "scissor": g.scissor, "efermi": g.efermi, "gauge_residual": g.gauge_residual}
         if hop:
             out["hop"] = gre + 1j * gim
         return out
