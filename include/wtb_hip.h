@@ -50,6 +50,9 @@ extern "C" {
                                     with FP32 accumulation, FP64 orthogonalisation /
                                     Rayleigh-Ritz / final residual refinement               */
 #define WTB_FP32   2             /* complex FP32 store, FP32 MFMA matvec (no refinement)     */
+#define WTB_INT8X  3             /* matvec only (wtb_bse_matvec on the FP32 store): exact int8
+                                    digit products with FP64 recombination, ~2^-28 relative to
+                                    max|H| max|X|; the refinement product of WTB_MIXED        */
 
 /* kernel overlap convention */
 #define WTB_CONV_REFERENCE 0     /* V <c_i|c_j><v_i|v_j>, module_bse.F90:77-81 */

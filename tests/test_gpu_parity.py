@@ -145,6 +145,54 @@ def test_matvec(precision, ncol, rng):
     ctx.close()
 
 
+@pytest.mark.parametrize("chunk", [None, 128])
+@pytest.mark.parametrize("ncol", [1, 5, 16, 40, 64, 130])
+def test_matvec_int8(ncol, chunk, rng, monkeypatch):
+    """Exact int8-digit product (the mixed solver's refinement product): the
+    error is the fixed-point rounding of H and X, ~2^-31 of max|H| per term,
+    which is the FP32 store's own rounding level.  chunk = 128 splits k into
+    four launches' worth of chunks (N = 400)."""
+    say("case matvec int8", ncol, chunk)
+    if chunk:
+        monkeypatch.setenv("WTB_INT8_CHUNK", str(chunk))
+    mesh = (10, 10, 1)
+    m = hex_model(mesh)
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, 2, 2)
+    ctx.bse_build(COUL, hip.MIXED)
+    H = ctx.get_block(0, ctx.N, 0, ctx.N)          # the FP32 store, exactly
+    X = rng.standard_normal((ctx.N, ncol)) + 1j * rng.standard_normal((ctx.N, ncol))
+    X[:, 0] *= 1e-3
+    if ncol > 2:
+        X[:, 2] *= 1e5
+    Y = ctx.matvec(X, hip.INT8X)
+    Yr = H @ X
+    rel = (np.linalg.norm(Y - Yr, axis=0) / np.linalg.norm(Yr, axis=0)).max()
+    # worst-case bound per column: fixed-point rounding of H (half a unit of
+    # hscale 2^-17 <= 2^-30 max|H|) and of X (<= 2^-30 max|X_c| per component),
+    # plus the dropped digit pairs (< 2^-34 max|H| max|X_c| per term)
+    off = H - np.diag(np.diag(H))
+    hm, xm = np.abs(off).max(), np.abs(X).max(axis=0)
+    bound = (2.0 ** -29 * (hm * np.abs(X).sum(axis=0) + xm * np.abs(off).sum(axis=1).max())
+             + ctx.N * 2.0 ** -32 * hm * xm)
+    err = np.abs(Y - Yr).max(axis=0)
+    say("rel error", rel, "max err / bound", (err / bound).max())
+    assert np.all(err <= bound)
+    assert rel < 5e-7
+    ctx.close()
+
+
+def test_matvec_int8_needs_fp32_store(rng):
+    say("case matvec int8 on FP64 store")
+    mesh = (6, 6, 1)
+    m = hex_model(mesh)
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, 2, 2)
+    ctx.bse_build(COUL, hip.FP64)
+    with pytest.raises(hip.WTBError) as e:
+        ctx.matvec(np.ones((ctx.N, 1), complex), hip.INT8X)
+    assert e.value.code == -7
+    ctx.close()
+
+
 @pytest.mark.parametrize("nk,nev", [(12, 144), (10, 100), (16, 96), (22, 200), (32, 64)])
 def test_solve_dense_small(nk, nev):
     """Dense path (N <= 1024): the tridiagonalisation runs LDS-resident for

@@ -19,6 +19,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <numeric>
 
 namespace wtb {
@@ -949,9 +950,12 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     WTB_HIP_CHECK(hipEventRecord(tr, c.stream));
     double max_rel = 0;
     if (precision != WTB_FP64) {
-        // FP64 residual refinement: H X in FP64 arithmetic on the nev wanted
-        // columns, then a Rayleigh-Ritz step on them
-        L.apply(WTB_FP64, nev, L.S.p, L.AS.p);
+        // residual refinement: H X on the nev wanted columns with exact int8
+        // digit products recombined in FP64 (or FP64 MFMA arithmetic with
+        // WTB_REFINE=fp64), then an FP64 Rayleigh-Ritz step on them
+        const char *renv = std::getenv("WTB_REFINE");
+        const bool ref64 = renv && std::strcmp(renv, "fp64") == 0;
+        L.apply(ref64 ? WTB_FP64 : WTB_INT8X, nev, L.S.p, L.AS.p);
         mvcols += nev;
         L.rayleigh_ritz(nev, false, true, nev);
     }
@@ -993,9 +997,11 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
         rep->kernel_bytes = k.slot[main].bytes;
         rep->kernel_flops = k.slot[main].flops;
         if (main != KernelClock::F64_ON_F32 && main != KernelClock::F64_ON_F64) {
-            rep->t_refine_kernel_ms = k.ms(KernelClock::F64_ON_F32);
-            rep->refine_launches = (int)k.slot[KernelClock::F64_ON_F32].ev.size();
-            rep->refine_flops = k.slot[KernelClock::F64_ON_F32].flops;
+            const KernelClock::Kind rk =
+                k.slot[KernelClock::INT8X].ev.empty() ? KernelClock::F64_ON_F32 : KernelClock::INT8X;
+            rep->t_refine_kernel_ms = k.ms(rk);
+            rep->refine_launches = (int)k.slot[rk].ev.size();
+            rep->refine_flops = k.slot[rk].flops;
         }
     }
     (void)hipEventDestroy(t0);

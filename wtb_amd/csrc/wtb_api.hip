@@ -303,8 +303,10 @@ int wtb_bse_matvec(wtb_ctx *ctx, int ncol, const double *X, double *Y, int preci
     return guarded(ctx, [&](Ctx &c) {
         if (!c.bse.valid) throw Error{WTB_ERR_STATE, "no BSE matrix"};
         if (ncol <= 0 || !X || !Y) throw Error{WTB_ERR_ARG, "invalid matvec arguments"};
-        if (precision != WTB_FP64 && precision != WTB_MIXED && precision != WTB_FP32)
+        if (precision != WTB_FP64 && precision != WTB_MIXED && precision != WTB_FP32 && precision != WTB_INT8X)
             throw Error{WTB_ERR_ARG, "unknown precision"};
+        if (precision == WTB_INT8X && c.bse.precision == WTB_FP64)
+            throw Error{WTB_ERR_UNSUPPORTED, "the int8-digit product runs on the FP32 store"};
         const long N = c.bse.N;
         DBuf<cd> x, y;
         x.alloc((size_t)N * ncol);

@@ -120,7 +120,7 @@ struct Solution {
 // algorithmic bytes (the store, read once) and flops (8 N^2 per column) are
 // added to the slot of its kind.
 struct KernelClock {
-    enum Kind { F16SPLIT = 0, F64_ON_F64 = 1, F64_ON_F32 = 2, F32_ON_F32 = 3, NKIND = 4 };
+    enum Kind { F16SPLIT = 0, F64_ON_F64 = 1, F64_ON_F32 = 2, F32_ON_F32 = 3, INT8X = 4, NKIND = 5 };
     struct Slot {
         std::vector<std::pair<hipEvent_t, hipEvent_t>> ev;
         double bytes = 0, flops = 0;
@@ -188,6 +188,8 @@ struct Ctx {
     DBuf<cd> ws_gram;    // split-K partial Gram products
     DBuf<_Float16> ws_split;   // FP16 split planes of X for the mixed matvec
     DBuf<float> ws_colscale;   // per-column split / output scales
+    DBuf<double> ws_scale8;    // per-column scales of the int8-digit matvec
+    DBuf<cd> ws_ypart;         // k-chunk partial products of the int8-digit matvec
     KernelClock kclock;
 };
 
@@ -216,6 +218,10 @@ void matvec(Ctx &c, int precision, int ncol, const cd *X, cd *Y);
 
 // Y = H_offdiag X on the FP32 store with split-FP16 MFMA (mixed precision).
 void matvec_f16split(Ctx &c, int ncol, const cd *X, cd *Y);
+
+// Y = H_offdiag X on the FP32 store with exact int8 digit products (the
+// refinement product of the mixed-precision solver).
+void matvec_int8(Ctx &c, int ncol, const cd *X, cd *Y);
 
 // Hermitian eigensolver for small dense matrices on the GPU (FP64 Jacobi).
 // A (m x m column-major, lda) is destroyed; W receives ascending eigenvalues,

@@ -21,7 +21,7 @@ ERRORS = {-1: "WTB_ERR_ARG", -2: "WTB_ERR_HIP", -3: "WTB_ERR_NOCONV", -4: "WTB_E
           -5: "WTB_ERR_NOMEM", -6: "WTB_ERR_STATE", -7: "WTB_ERR_UNSUPPORTED"}
 FLAG_COMPAT_F32 = 1
 FLAG_GAUGE_LAPACK = 2
-FP64, MIXED, FP32 = 0, 1, 2
+FP64, MIXED, FP32, INT8X = 0, 1, 2, 3
 CONV_REFERENCE, CONV_TEXTBOOK = 0, 1
 COULOMB = {"V2DK": 1, "V3D": 2, "V3DL": 3, "V2DT": 4, "V2DT2": 5, "V2DOH": 6, "V2DRK": 7,
            "V0DT": 8, "V1DT": 9}
