@@ -5,12 +5,14 @@
 // One 64-lane workgroup (a single wavefront) per k-point, norb <= 32:
 //   1. Fourier sum: each lane owns up to 16 matrix entries and accumulates
 //      over R in the reference's order; phases are staged in LDS per chunk.
-//   2. Parallel cyclic Jacobi (round-robin pairing, norb/2 rotations per step)
-//      with the matrix and the eigenvector accumulator in LDS.
-//   3. Optional gauge fix: each eigenvector is rotated so that Q^H z is real,
-//      Q being the unblocked lower Householder reduction of H(k) (LAPACK
-//      ZHETD2 with ZLARFG reflectors).  This reproduces the phases returned by
-//      ZHEEV('V','L') up to a real sign.
+//   2. Householder reduction to a real tridiagonal T = Q^H H(k) Q with the
+//      unblocked lower reflectors of LAPACK ZHETD2 (ZLARFG), Q accumulated in
+//      LDS.
+//   3. Implicit-shift QL on T (the tql2 recurrence), its rotations
+//      accumulated in a real Z by one lane per row.  Each column of Z gets its
+//      largest entry positive, and the eigenvectors are Q Z: they carry the
+//      phases returned by ZHEEV('V','L') (Q^H z real), with that sign, whatever
+//      the WTB_FLAG_GAUGE_LAPACK flag says.
 //   4. nocp (first band above 0), scissor, band selection.
 #include "wtb_internal.h"
 
@@ -18,7 +20,6 @@ namespace wtb {
 namespace {
 
 __device__ __forceinline__ cd cmul(cd a, cd b) { return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
-__device__ __forceinline__ cd cconjmul(cd a, cd b) { return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x); }  // conj(a) b
 __device__ __forceinline__ cd cadd(cd a, cd b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ cd csub(cd a, cd b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ cd cscale(double s, cd a) { return make_double2(s * a.x, s * a.y); }
@@ -27,21 +28,6 @@ __device__ __forceinline__ cd cconj(cd a) { return make_double2(a.x, -a.y); }
 __device__ __forceinline__ double wave_sum(double v) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     return v;
-}
-
-// Round-robin pairing (circle method) of m (even) indices; step t in [0, m-2].
-__device__ __forceinline__ void rr_pair(int m, int t, int i, int &p, int &q) {
-    int mm = m - 1;
-    int x, y;
-    if (i == 0) {
-        x = t % mm;
-        y = mm;
-    } else {
-        x = (t + i) % mm;
-        y = (t - i + mm) % mm;
-    }
-    p = min(x, y);
-    q = max(x, y);
 }
 
 constexpr int kLanes = 64;
@@ -57,18 +43,13 @@ hk_eig_kernel(int nk, int n, int nvec, int nv, int nc, int flags, double efermi,
     const int tid = threadIdx.x;
     const int nn = n * n;
     cd *A = reinterpret_cast<cd *>(smem);            // [r*n + s]
-    cd *V = A + nn;                                  // eigenvectors, [o*n + j]
-    cd *H0 = V + nn;                                 // copy of H(k) for the gauge fix
-    cd *ph = H0 + nn;                                // kLanes phases
-    cd *vb = ph + kLanes;                            // 32
-    cd *yb = vb + kMaxOrb;                           // 32
-    cd *rph = yb + kMaxOrb;                          // 16 rotation phases
-    double *rc = reinterpret_cast<double *>(rph + kMaxOrb / 2);
-    double *rs = rc + kMaxOrb / 2;
-    double *wv = rs + kMaxOrb / 2;                   // 32 eigenvalues
-    int *rp = reinterpret_cast<int *>(wv + kMaxOrb);
-    int *rq = rp + kMaxOrb / 2;
-    int *perm = rq + kMaxOrb / 2;
+    cd *V = A + nn;                                  // Q, then the eigenvectors, [o*n + j]
+    cd *ph = V + nn;                                 // kLanes phases
+    cd *vb = ph + kLanes;                            // 32: Householder vector
+    cd *yb = vb + kMaxOrb;                           // 32: A v
+    double *wv = reinterpret_cast<double *>(yb + kMaxOrb);   // 32: diagonal of T, eigenvalues
+    double *eo = wv + kMaxOrb;                       // 32: off-diagonal of T
+    int *perm = reinterpret_cast<int *>(eo + kMaxOrb);       // 32
 
     const bool compat = flags & WTB_FLAG_COMPAT_F32;
     const double kx = kpt[3 * k], ky = kpt[3 * k + 1], kz = kpt[3 * k + 2];
@@ -107,169 +88,170 @@ hk_eig_kernel(int nk, int n, int nvec, int nv, int nc, int flags, double efermi,
             cd h = acc[s];
             if (idx / n == idx % n) h.x -= ef;
             A[idx] = h;
-            H0[idx] = h;
-            V[idx] = make_double2((idx / n == idx % n) ? 1.0 : 0.0, 0.0);
         }
     }
     __syncthreads();
 
-    // ---- 2. parallel cyclic Jacobi ----
-    const int m = (n + 1) & ~1;
-    const int npair = m / 2;
-    double fro = 0.0;
-    for (int idx = tid; idx < nn; idx += kLanes) fro += A[idx].x * A[idx].x + A[idx].y * A[idx].y;
-    fro = wave_sum(fro);
-    for (int sweep = 0; sweep < 40; ++sweep) {
-        double off = 0.0;
-        for (int idx = tid; idx < nn; idx += kLanes)
-            if (idx / n != idx % n) off += A[idx].x * A[idx].x + A[idx].y * A[idx].y;
-        off = wave_sum(off);
-        if (off <= 1e-30 * fro || off < 1e-300) break;
-        for (int t = 0; t < m - 1; ++t) {
-            if (tid < npair) {
-                int p, q;
-                rr_pair(m, t, tid, p, q);
-                double c = 1.0, s = 0.0;
-                cd e = make_double2(1.0, 0.0);
-                if (q < n) {
-                    cd b = A[p * n + q];
-                    double ab = sqrt(b.x * b.x + b.y * b.y);
-                    if (ab > 0.0) {
-                        double theta = (A[q * n + q].x - A[p * n + p].x) / (2.0 * ab);
-                        double tt = (theta >= 0.0 ? 1.0 : -1.0) / (fabs(theta) + sqrt(theta * theta + 1.0));
-                        c = 1.0 / sqrt(tt * tt + 1.0);
-                        s = tt * c;
-                        e = make_double2(b.x / ab, b.y / ab);
-                    }
+    // ---- 2. Householder reduction T = Q^H H Q (ZHETD2, lower, ZLARFG reflectors) ----
+    // T is real tridiagonal (diagonal d, sub-diagonal beta); Q accumulates in V.
+    for (int idx = tid; idx < nn; idx += kLanes)
+        V[idx] = make_double2((idx / n == idx % n) ? 1.0 : 0.0, 0.0);
+    __syncthreads();
+    for (int i = 0; i < n - 1; ++i) {
+        double xn2 = 0.0;
+        if (tid >= i + 2 && tid < n) {
+            const cd h = A[tid * n + i];
+            xn2 = h.x * h.x + h.y * h.y;
+        }
+        xn2 = wave_sum(xn2);
+        const cd alpha = A[(i + 1) * n + i];
+        if (xn2 == 0.0 && alpha.y == 0.0) {    // H(i) = I
+            __syncthreads();
+            continue;
+        }
+        const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+        const double beta = alpha.x >= 0.0 ? -nrm : nrm;
+        const cd tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
+        const cd den = make_double2(alpha.x - beta, alpha.y);          // v = x / (alpha - beta)
+        const double dd = den.x * den.x + den.y * den.y;
+        const cd scal = make_double2(den.x / dd, -den.y / dd);
+        if (tid < n) {
+            cd v = make_double2(0.0, 0.0);
+            if (tid == i + 1) v = make_double2(1.0, 0.0);
+            else if (tid > i + 1) v = cmul(A[tid * n + i], scal);
+            vb[tid] = v;
+        }
+        __syncthreads();
+        if (tid < n) {
+            cd y = make_double2(0.0, 0.0);
+            for (int c = i + 1; c < n; ++c) y = cadd(y, cmul(A[tid * n + c], vb[c]));
+            yb[tid] = y;
+        }
+        __syncthreads();
+        double mu = 0.0;
+        if (tid > i && tid < n) mu = vb[tid].x * yb[tid].x + vb[tid].y * yb[tid].y;
+        mu = wave_sum(mu);
+        const double t2 = tau.x * tau.x + tau.y * tau.y;
+        // A <- H(i)^H A H(i):  - tau y_r conj(v_c) - conj(tau) v_r conj(y_c) + |tau|^2 mu v_r conj(v_c)
+        for (int idx = tid; idx < nn; idx += kLanes) {
+            const int r = idx / n, c = idx % n;
+            const cd vr = vb[r], vc = vb[c], yr = yb[r], yc = yb[c];
+            const cd t1 = cmul(tau, cmul(yr, cconj(vc)));
+            const cd t2c = cmul(cconj(tau), cmul(vr, cconj(yc)));
+            const cd t3 = cscale(t2 * mu, cmul(vr, cconj(vc)));
+            A[idx] = cadd(csub(csub(A[idx], t1), t2c), t3);
+        }
+        // Q <- Q H(i) (row tid)
+        if (tid < n) {
+            cd sacc = make_double2(0.0, 0.0);
+            for (int c = i + 1; c < n; ++c) sacc = cadd(sacc, cmul(V[tid * n + c], vb[c]));
+            const cd ts = cmul(tau, sacc);
+            for (int c = i + 1; c < n; ++c) V[tid * n + c] = csub(V[tid * n + c], cmul(ts, cconj(vb[c])));
+        }
+        __syncthreads();
+    }
+    double *dg = wv;     // eo[j] couples j and j + 1
+    if (tid < n) {
+        dg[tid] = A[tid * n + tid].x;
+        eo[tid] = tid + 1 < n ? A[(tid + 1) * n + tid].x : 0.0;
+    }
+    __syncthreads();
+    double *Z = reinterpret_cast<double *>(A);     // real rotations of T, [o*n + j]
+    for (int idx = tid; idx < nn; idx += kLanes) Z[idx] = (idx / n == idx % n) ? 1.0 : 0.0;
+    __syncthreads();
+
+    // ---- 3. implicit QL on T (tql2), the rotations applied to the columns of V ----
+    // Every lane runs the same scalar recurrence on dg / eo (identical values,
+    // so the shared writes agree); lane o < n rotates row o of Z.
+    const double eps = 0x1p-52;
+    bool failed = false;
+    for (int l = 0; l < n && !failed; ++l) {
+        for (int iter = 0;; ++iter) {
+            int m = l;
+            for (; m < n - 1; ++m)
+                if (fabs(eo[m]) <= eps * (fabs(dg[m]) + fabs(dg[m + 1]))) break;
+            if (m == l) break;
+            if (iter == 60) {
+                failed = true;
+                break;
+            }
+            double g = (dg[l + 1] - dg[l]) / (2.0 * eo[l]);
+            double r = sqrt(g * g + 1.0);
+            g = dg[m] - dg[l] + eo[l] / (g + (g >= 0.0 ? r : -r));
+            double sn = 1.0, cs = 1.0, p = 0.0;
+            bool deflated = false;
+            for (int i = m - 1; i >= l; --i) {
+                const double f = sn * eo[i], b = cs * eo[i];
+                r = sqrt(f * f + g * g);
+                eo[i + 1] = r;
+                if (r == 0.0) {
+                    dg[i + 1] -= p;
+                    eo[m] = 0.0;
+                    deflated = true;
+                    break;
                 }
-                rp[tid] = p; rq[tid] = q; rc[tid] = c; rs[tid] = s; rph[tid] = e;
+                sn = f / r;
+                cs = g / r;
+                g = dg[i + 1] - p;
+                r = (dg[i] - g) * sn + 2.0 * cs * b;
+                p = sn * r;
+                dg[i + 1] = g + p;
+                g = cs * r - b;
+                if (tid < n) {
+                    const double zi = Z[tid * n + i], zf = Z[tid * n + i + 1];
+                    Z[tid * n + i + 1] = sn * zi + cs * zf;
+                    Z[tid * n + i] = cs * zi - sn * zf;
+                }
             }
-            __syncthreads();
-            // column pass: A <- A J, V <- V J
-            for (int task = tid; task < n * npair; task += kLanes) {
-                int r = task % n, a = task / n;
-                int p = rp[a], q = rq[a];
-                if (q >= n) continue;
-                double c = rc[a], s = rs[a];
-                cd ec = cconj(rph[a]);
-                cd x = A[r * n + p], y = A[r * n + q];
-                cd ey = cmul(ec, y);
-                A[r * n + p] = csub(cscale(c, x), cscale(s, ey));
-                A[r * n + q] = cadd(cscale(s, x), cscale(c, ey));
-                x = V[r * n + p]; y = V[r * n + q];
-                ey = cmul(ec, y);
-                V[r * n + p] = csub(cscale(c, x), cscale(s, ey));
-                V[r * n + q] = cadd(cscale(s, x), cscale(c, ey));
-            }
-            __syncthreads();
-            // row pass: A <- J^H A
-            for (int task = tid; task < n * npair; task += kLanes) {
-                int sc = task % n, a = task / n;
-                int p = rp[a], q = rq[a];
-                if (q >= n) continue;
-                double c = rc[a], s = rs[a];
-                cd e = rph[a];
-                cd x = A[p * n + sc], y = A[q * n + sc];
-                cd ey = cmul(e, y);
-                cd np = csub(cscale(c, x), cscale(s, ey));
-                cd nq = cadd(cscale(s, x), cscale(c, ey));
-                if (sc == p) { np.y = 0.0; nq = make_double2(0.0, 0.0); }
-                if (sc == q) { np = make_double2(0.0, 0.0); nq.y = 0.0; }
-                A[p * n + sc] = np;
-                A[q * n + sc] = nq;
-            }
-            __syncthreads();
+            if (deflated) continue;
+            dg[l] -= p;
+            eo[l] = g;
+            eo[m] = 0.0;
         }
     }
+    if (failed && tid == 0) atomicOr(status, 4);
+    __syncthreads();
+    // sign: the largest |Z| entry of each column (Z = Q^H z) positive, the
+    // first one on ties
+    if (tid < n) {
+        double best = -1.0, sg = 1.0;
+        for (int r = 0; r < n; ++r) {
+            const double z = Z[r * n + tid];
+            if (fabs(z) > best) {
+                best = fabs(z);
+                sg = z < 0.0 ? -1.0 : 1.0;
+            }
+        }
+        if (sg < 0.0)
+            for (int r = 0; r < n; ++r) Z[r * n + tid] = -Z[r * n + tid];
+    }
+    __syncthreads();
+    // eigenvectors V = Q Z, lane o forms row o
+    if (tid < n) {
+        cd q[kMaxOrb];
+#pragma unroll
+        for (int mm = 0; mm < kMaxOrb; ++mm) q[mm] = mm < n ? V[tid * n + mm] : make_double2(0.0, 0.0);
+        for (int j = 0; j < n; ++j) {
+            cd acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int mm = 0; mm < kMaxOrb; ++mm)
+                if (mm < n) acc = cadd(acc, cscale(Z[mm * n + j], q[mm]));
+            V[tid * n + j] = acc;
+        }
+    }
+    __syncthreads();
 
     // ---- sort ascending ----
-    if (tid < n) wv[tid] = A[tid * n + tid].x;
-    __syncthreads();
     if (tid < n) {
-        double w = wv[tid];
+        const double w = wv[tid];
         int rank = 0;
         for (int i = 0; i < n; ++i) {
-            double u = wv[i];
+            const double u = wv[i];
             rank += (u < w) || (u == w && i < tid);
         }
         perm[rank] = tid;
     }
     __syncthreads();
-
-    // ---- 3. LAPACK phase gauge ----
-    if (flags & WTB_FLAG_GAUGE_LAPACK) {
-        cd *Q = A;  // reuse
-        for (int idx = tid; idx < nn; idx += kLanes)
-            Q[idx] = make_double2((idx / n == idx % n) ? 1.0 : 0.0, 0.0);
-        __syncthreads();
-        for (int i = 0; i < n - 1; ++i) {
-            double xn2 = 0.0;
-            if (tid >= i + 2 && tid < n) {
-                cd h = H0[tid * n + i];
-                xn2 = h.x * h.x + h.y * h.y;
-            }
-            xn2 = wave_sum(xn2);
-            cd alpha = H0[(i + 1) * n + i];
-            if (xn2 == 0.0 && alpha.y == 0.0) {
-                __syncthreads();
-                continue;
-            }
-            double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-            double beta = alpha.x >= 0.0 ? -nrm : nrm;
-            cd tau = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-            cd den = make_double2(alpha.x - beta, alpha.y);          // 1/(alpha - beta)
-            double dd = den.x * den.x + den.y * den.y;
-            cd scal = make_double2(den.x / dd, -den.y / dd);
-            if (tid < n) {
-                cd v = make_double2(0.0, 0.0);
-                if (tid == i + 1) v = make_double2(1.0, 0.0);
-                else if (tid > i + 1) v = cmul(H0[tid * n + i], scal);
-                vb[tid] = v;
-            }
-            __syncthreads();
-            if (tid < n) {
-                cd y = make_double2(0.0, 0.0);
-                for (int c = i + 1; c < n; ++c) y = cadd(y, cmul(H0[tid * n + c], vb[c]));
-                yb[tid] = y;
-            }
-            __syncthreads();
-            double mu = 0.0;
-            if (tid > i && tid < n) mu = vb[tid].x * yb[tid].x + vb[tid].y * yb[tid].y;
-            mu = wave_sum(mu);
-            double t2 = tau.x * tau.x + tau.y * tau.y;
-            for (int idx = tid; idx < nn; idx += kLanes) {
-                int r = idx / n, c = idx % n;
-                cd vr = vb[r], vc = vb[c], yr = yb[r], yc = yb[c];
-                // - tau y_r conj(v_c) - conj(tau) v_r conj(y_c) + |tau|^2 mu v_r conj(v_c)
-                cd t1 = cmul(tau, cmul(yr, cconj(vc)));
-                cd t2c = cmul(cconj(tau), cmul(vr, cconj(yc)));
-                cd t3 = cscale(t2 * mu, cmul(vr, cconj(vc)));
-                H0[idx] = cadd(csub(csub(H0[idx], t1), t2c), t3);
-            }
-            if (tid < n) {
-                cd s = make_double2(0.0, 0.0);
-                for (int c = i + 1; c < n; ++c) s = cadd(s, cmul(Q[tid * n + c], vb[c]));
-                cd ts = cmul(tau, s);
-                for (int c = i + 1; c < n; ++c) Q[tid * n + c] = csub(Q[tid * n + c], cmul(ts, cconj(vb[c])));
-            }
-            __syncthreads();
-        }
-        // rotate eigenvector columns (thread j owns column j)
-        if (tid < n) {
-            double best = -1.0;
-            cd bph = make_double2(1.0, 0.0);
-            for (int mm2 = 0; mm2 < n; ++mm2) {
-                cd s = make_double2(0.0, 0.0);
-                for (int r = 0; r < n; ++r) s = cadd(s, cconjmul(Q[r * n + mm2], V[r * n + tid]));
-                double a2 = s.x * s.x + s.y * s.y;
-                if (a2 > best) { best = a2; bph = s; }
-            }
-            double ab = sqrt(best);
-            cd f = make_double2(bph.x / ab, -bph.y / ab);
-            for (int r = 0; r < n; ++r) V[r * n + tid] = cmul(V[r * n + tid], f);
-        }
-        __syncthreads();
-    }
 
     // ---- 4. nocp, scissor, outputs ----
     __shared__ int s_nocp;
@@ -326,8 +308,8 @@ void launch_hk_eig(Ctx &c, int nv, int nc) {
     b.usel32.alloc((size_t)nk * nb * n);
     b.status.alloc(1);
     WTB_HIP_CHECK(hipMemsetAsync(b.status.p, 0, sizeof(int), c.stream));
-    size_t shmem = 3 * (size_t)n * n * sizeof(cd) + (kLanes + 2 * kMaxOrb + kMaxOrb / 2) * sizeof(cd) +
-                   (kMaxOrb + kMaxOrb) * sizeof(double) + (kMaxOrb + kMaxOrb) * sizeof(int);
+    size_t shmem = 2 * (size_t)n * n * sizeof(cd) + (kLanes + 2 * kMaxOrb) * sizeof(cd) +
+                   2 * kMaxOrb * sizeof(double) + kMaxOrb * sizeof(int);
     hipLaunchKernelGGL(hk_eig_kernel, dim3(nk), dim3(kLanes), shmem, c.stream, nk, n, m.nvec, nv, nc,
                        m.flags, m.efermi, m.scissor, c.kpt.p, m.hop.p, m.rcart.p, m.invdeg.p,
                        b.eall.p, b.uall.p, b.nocp.p, b.esel.p, b.usel.p, b.status.p);
@@ -341,6 +323,7 @@ void launch_hk_eig(Ctx &c, int nv, int nc) {
     WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
     if (st & 1) throw Error{WTB_ERR_NOCP, "no eigenvalue above the Fermi level at some k (nocp undefined)"};
     if (st & 2) throw Error{WTB_ERR_NOCP, "Number of valence/conduction states higher than occupied/unoccupied states"};
+    if (st & 4) throw Error{WTB_ERR_NOCONV, "H(k) eigensolver: QL iteration did not converge at some k"};
     b.valid = true;
 }
 

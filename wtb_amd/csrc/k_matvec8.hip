@@ -146,9 +146,12 @@ __global__ void split_x8_kernel(long N, long ld, int ncol, const double *__restr
 
 // RT row tiles of 16 per workgroup; WPR wavefronts share a row tile, each
 // owning CPW blocks of 16 columns.  gridDim.y: chunks of kc_len values of k.
+// One wavefront per SIMD (the 80 KB LDS tile allows one workgroup per CU):
+// the MFMA accumulators take the AGPRs, the rest of the 512 registers hold
+// the digit operands of two steps and the H prefetch.
 // Yp: [chunk][col][N] complex FP64 partial products.
 template <int CPW, int WPR, int RT, int NL>
-__global__ void __launch_bounds__(64 * WPR * RT)
+__global__ void __launch_bounds__(64 * WPR * RT) __attribute__((amdgpu_waves_per_eu(1, 1)))
 hx_int8_kernel(long N, long ld, int ncol, long kc_len, float hmul, const float2 *__restrict__ H,
                const int8_t *__restrict__ P, const double *__restrict__ ysc, double *__restrict__ Yp) {
     constexpr int NCB = CPW * WPR;
@@ -227,46 +230,60 @@ hx_int8_kernel(long N, long ld, int ncol, long kc_len, float hmul, const float2 
             acc[L][0][cb] = i4{0, 0, 0, 0};
             acc[L][1][cb] = i4{0, 0, 0, 0};
         }
+    // X digit major: the B fragments of one X digit for every column block,
+    // then every H digit paired with it; the column blocks innermost keep
+    // 4 CPW independent accumulator chains between reuses
     auto compute = [&](int buf, const i4 (&a)[3][4]) {
 #pragma unroll
-        for (int cb = 0; cb < CPW; ++cb) {
-            const int col = (cp * CPW + cb) * 16 + li;
-            i4 b[2][4];
+        for (int sx = 0; sx < 4; ++sx) {
+            i4 b[CPW][2];
 #pragma unroll
-            for (int c = 0; c < 2; ++c)
+            for (int cb = 0; cb < CPW; ++cb) {
+                const int col = (cp * CPW + cb) * 16 + li;
 #pragma unroll
-                for (int s = 0; s < 4; ++s) b[c][s] = *reinterpret_cast<const i4 *>(&xs[buf][4 * c + s][col][16 * blk]);
+                for (int c = 0; c < 2; ++c) b[cb][c] = *reinterpret_cast<const i4 *>(&xs[buf][4 * c + sx][col][16 * blk]);
+            }
 #pragma unroll
-            for (int L = 0; L < NL; ++L)
+            for (int sh = 3; sh >= 0; --sh) {
+                const int L = 6 - sh - sx;
+                if (L >= NL) continue;
 #pragma unroll
-                for (int sh = min(3, 6 - L); sh >= max(0, 3 - L); --sh) {
-                    const int sx = 6 - L - sh;
-                    acc[L][0][cb] = mfma8(a[0][sh], b[0][sx], acc[L][0][cb]);   // c_r X_r
-                    acc[L][1][cb] = mfma8(a[0][sh], b[1][sx], acc[L][1][cb]);   // c_r X_i
-                    acc[L][0][cb] = mfma8(a[1][sh], b[1][sx], acc[L][0][cb]);   // c_i X_i
-                    acc[L][1][cb] = mfma8(a[2][sh], b[0][sx], acc[L][1][cb]);   // -c_i X_r
+                for (int cb = 0; cb < CPW; ++cb) {
+                    acc[L][0][cb] = mfma8(a[0][sh], b[cb][0], acc[L][0][cb]);   // c_r X_r
+                    acc[L][1][cb] = mfma8(a[0][sh], b[cb][1], acc[L][1][cb]);   // c_r X_i
                 }
+#pragma unroll
+                for (int cb = 0; cb < CPW; ++cb) {
+                    acc[L][0][cb] = mfma8(a[1][sh], b[cb][1], acc[L][0][cb]);   // c_i X_i
+                    acc[L][1][cb] = mfma8(a[2][sh], b[cb][0], acc[L][1][cb]);   // -c_i X_r
+                }
+            }
         }
     };
 
+    // Step it runs its MFMAs on digits made in step it - 1 (from H fetched in
+    // step it - 2): the next step's digits and the H two steps ahead are in
+    // flight while the MFMAs run.
     i4 xr[XCH];
     float4 h[8];
-    i4 a[3][4];
+    i4 a[3][4], an[3][4];
     load_x(0, xr);
     load_h(0, h);
     store_x(0, xr);
+    make_digits(h, a);
+    load_h(1, h);
     __syncthreads();
-    // the digits of step it are made before the loads of step it + 1 reuse the
-    // H registers; the loads stay in flight across the MFMA phase
     for (long it = 0; it < niter; ++it) {
         const int buf = (int)(it & 1);
-        make_digits(h, a);
         load_x(it + 1, xr);
-        load_h(it + 1, h);
-        __builtin_amdgcn_sched_barrier(0);
+        make_digits(h, an);
+        load_h(it + 2, h);
         compute(buf, a);
-        __builtin_amdgcn_sched_barrier(0);
         store_x(buf ^ 1, xr);
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+#pragma unroll
+            for (int s2 = 0; s2 < 4; ++s2) a[c][s2] = an[c][s2];
         __syncthreads();
     }
 
@@ -305,7 +322,7 @@ __global__ void sum_chunks8(size_t n, int nch, const double *__restrict__ Yp, do
     Y[i] = s;
 }
 
-constexpr int kRT8 = 4, kWPR8 = 2, kCPW8 = 2;   // 8 wavefronts: 64 rows x 64 columns per workgroup
+constexpr int kRT8 = 4, kWPR8 = 1, kCPW8 = 4;   // 4 wavefronts of 16 rows x 64 columns per workgroup
 constexpr int kNL8 = 5;                          // significance levels kept (13 digit pairs)
 
 }  // namespace
