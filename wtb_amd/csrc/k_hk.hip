@@ -32,6 +32,16 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 constexpr int kLanes = 64;
 
+// value of lane i (uniform i) of a per-lane double
+__device__ __forceinline__ double lane_get(double v, int i) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, i);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), i);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+// v with lane i replaced by x
+__device__ __forceinline__ double lane_set(double v, int i, double x) { return (int)threadIdx.x == i ? x : v; }
+
 __global__ void __launch_bounds__(kLanes)
 hk_eig_kernel(int nk, int n, int nvec, int nv, int nc, int flags, double efermi, double scissor,
               const double *__restrict__ kpt, const cd *__restrict__ hop,
@@ -160,42 +170,48 @@ hk_eig_kernel(int nk, int n, int nvec, int nv, int nc, int flags, double efermi,
     for (int idx = tid; idx < nn; idx += kLanes) Z[idx] = (idx / n == idx % n) ? 1.0 : 0.0;
     __syncthreads();
 
-    // ---- 3. implicit QL on T (tql2), the rotations applied to the columns of V ----
-    // Every lane runs the same scalar recurrence on dg / eo (identical values,
-    // so the shared writes agree); lane o < n rotates row o of Z.
+    // ---- 3. implicit QL on T (tql2), the rotations accumulated in Z ----
+    // The diagonal and off-diagonal live in registers, lane j holding d_j and
+    // e_j; the recurrence is uniform across the wavefront and reads / writes
+    // them with readlane and a per-lane select.  Lane o < n rotates row o of Z.
+    double dj = tid < n ? dg[tid] : 0.0, ej = tid < n ? eo[tid] : 0.0;
     const double eps = 0x1p-52;
     bool failed = false;
     for (int l = 0; l < n && !failed; ++l) {
         for (int iter = 0;; ++iter) {
-            int m = l;
-            for (; m < n - 1; ++m)
-                if (fabs(eo[m]) <= eps * (fabs(dg[m]) + fabs(dg[m + 1]))) break;
+            // first m >= l with a negligible e_m (m = n - 1 at the end)
+            const double dn = __shfl_down(dj, 1, 64);
+            const bool small = tid >= n - 1 || fabs(ej) <= eps * (fabs(dj) + fabs(dn));
+            const unsigned long long bal = __ballot(small) >> l;
+            const int m = l + __builtin_ctzll(bal);
             if (m == l) break;
             if (iter == 60) {
                 failed = true;
                 break;
             }
-            double g = (dg[l + 1] - dg[l]) / (2.0 * eo[l]);
+            const double dl = lane_get(dj, l), el = lane_get(ej, l);
+            double g = (lane_get(dj, l + 1) - dl) / (2.0 * el);
             double r = sqrt(g * g + 1.0);
-            g = dg[m] - dg[l] + eo[l] / (g + (g >= 0.0 ? r : -r));
+            g = lane_get(dj, m) - dl + el / (g + (g >= 0.0 ? r : -r));
             double sn = 1.0, cs = 1.0, p = 0.0;
             bool deflated = false;
             for (int i = m - 1; i >= l; --i) {
-                const double f = sn * eo[i], b = cs * eo[i];
+                const double ei = lane_get(ej, i);
+                const double f = sn * ei, b = cs * ei;
                 r = sqrt(f * f + g * g);
-                eo[i + 1] = r;
+                ej = lane_set(ej, i + 1, r);
                 if (r == 0.0) {
-                    dg[i + 1] -= p;
-                    eo[m] = 0.0;
+                    dj = lane_set(dj, i + 1, lane_get(dj, i + 1) - p);
+                    ej = lane_set(ej, m, 0.0);
                     deflated = true;
                     break;
                 }
                 sn = f / r;
                 cs = g / r;
-                g = dg[i + 1] - p;
-                r = (dg[i] - g) * sn + 2.0 * cs * b;
+                g = lane_get(dj, i + 1) - p;
+                r = (lane_get(dj, i) - g) * sn + 2.0 * cs * b;
                 p = sn * r;
-                dg[i + 1] = g + p;
+                dj = lane_set(dj, i + 1, g + p);
                 g = cs * r - b;
                 if (tid < n) {
                     const double zi = Z[tid * n + i], zf = Z[tid * n + i + 1];
@@ -204,11 +220,12 @@ hk_eig_kernel(int nk, int n, int nvec, int nv, int nc, int flags, double efermi,
                 }
             }
             if (deflated) continue;
-            dg[l] -= p;
-            eo[l] = g;
-            eo[m] = 0.0;
+            dj = lane_set(dj, l, dl - p);
+            ej = lane_set(ej, l, g);
+            ej = lane_set(ej, m, 0.0);
         }
     }
+    if (tid < n) dg[tid] = dj;
     if (failed && tid == 0) atomicOr(status, 4);
     __syncthreads();
     // sign: the largest |Z| entry of each column (Z = Q^H z) positive, the

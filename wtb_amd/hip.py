@@ -169,13 +169,18 @@ class Context:
         self._check(self.lib.wtb_set_kpoints(self.h, k.shape[0], _ptr(k), _ptr(g)))
         self.nk = k.shape[0]
 
-    def hk_eig(self, nv, nc):
+    def hk_eig(self, nv, nc, host=True):
+        """Bands of every k.  host=False keeps them on the device only (they
+        feed the assembly there) and returns None."""
         nb = nv + nc
+        self.N = self.nk * nc * nv
+        if not host:
+            self._check(self.lib.wtb_hk_eig(self.h, nv, nc, None, None, None))
+            return None
         eig = np.zeros((self.nk, nb), order="F")
         vec = np.zeros((self.nk, nb, self.norb), np.complex128, order="F")
         nocp = np.zeros(self.nk, np.int32)
         self._check(self.lib.wtb_hk_eig(self.h, nv, nc, _ptr(eig), _ptr(vec), _ptr(nocp)))
-        self.N = self.nk * nc * nv
         return eig, vec, nocp
 
     def get_bands(self):
