@@ -228,3 +228,43 @@ def test_spectrum_torch_function_and_fit():
     say("spectrum fit history", out["history"][:3], out["history"][-3:], "r0", float(out["coul"][0]))
     assert abs(float(out["coul"][0]) - COUL[0]) < 1e-3 * COUL[0]
     prob.close()
+
+
+def oracle_loss(m, k, coul, omega, target, cpre, scissor=0.0):
+    """The reference-convention spectrum loss entirely from the oracle:
+    bands, BSE matrix, dense eigenpairs, dipoles, strengths |x^T d|^2."""
+    mm = T.TBModel(**{**m.__dict__})
+    mm.scissor = scissor
+    om = O.Model(mm)
+    ref = O.hk_eig(om, k, 2, 2)
+    Hm = O.bse_matrix(k, ref["eig"], ref["vec"], O.coulomb(**coul_dict(coul)), MESH, mm.rlat, 2, 2)
+    E, X = np.linalg.eigh(Hm)
+    E, X = E[:NEV], X[:, :NEV]
+    d = O.dipoles(om, k, ref["eig"], ref["vec"], 2, 2, CSHIFT)[0]
+    f = np.abs(X.T @ d) ** 2
+    x = omega[:, None] - E[None, :]
+    eps = cpre * (f[None, :] * GAMMA / (x * x + GAMMA * GAMMA)).sum(axis=1)
+    return ((eps - target) ** 2).sum()
+
+
+def test_spectrum_grad_against_oracle_fd():
+    """Reference convention: the GPU adjoint against finite differences of
+    the oracle's own pipeline (r0 and the scissor, five-point stencil)."""
+    say("case spectrum grad vs oracle fd")
+    p, omega, target, E, f = make_case(hip.CONV_REFERENCE)
+    g = p.ctx.grad_spectrum(omega, target, "x", CSHIFT, GAMMA, hop=False)
+    L0 = oracle_loss(p.m, p.k, COUL, omega, target, p.cpre)
+    assert abs(L0 - g["loss"]) <= 1e-9 * L0
+
+    def at_r0(dv):
+        c = COUL.copy()
+        c[0] += dv
+        return oracle_loss(p.m, p.k, c, omega, target, p.cpre)
+
+    fd = fd5(at_r0, 1e-4 * COUL[0])
+    say("r0 oracle fd", fd, "adjoint", g["coulomb"]["r0"])
+    assert abs(g["coulomb"]["r0"] - fd) <= 1e-5 * abs(fd)
+    fd = fd5(lambda dv: oracle_loss(p.m, p.k, COUL, omega, target, p.cpre, scissor=dv), 1e-4)
+    say("scissor oracle fd", fd, "adjoint", g["scissor"])
+    assert abs(g["scissor"] - fd) <= 1e-5 * abs(fd)
+    p.ctx.close()
