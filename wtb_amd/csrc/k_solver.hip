@@ -206,6 +206,11 @@ __global__ void hermitize(int m, cd *A, int lda) {
     A[s + (size_t)r * lda] = cconj(h);
 }
 
+// p[0..n) = v
+__global__ void fill_value(int n, double v, double *__restrict__ p) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
 __global__ void set_units(long N, int b, const int *__restrict__ idx, cd *__restrict__ X) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     int j = blockIdx.y;
@@ -613,14 +618,20 @@ struct Lobpcg {
     Ctx &c;
     long N;
     int b, nev, m3;
-    DBuf<cd> S, AS, T1, P, AP;      // N x cols
-    DBuf<cd> G, A, U, T, Tmp, Cm;   // small m3 x m3
-    DBuf<double> D, lam, lamr, norm2, sc;
-    DBuf<int> idx, act;
+    DBuf<cd> &S, &AS, &T1, &P, &AP;      // N x cols
+    DBuf<cd> &G, &A, &U, &T, &Tmp, &Cm;  // small m3 x m3
+    DBuf<double> &D, &lam, &lamr, &norm2, &sc;
+    DBuf<int> &idx, &act;
     Timer tmv, trr;
     std::vector<double> h_lam;
 
-    Lobpcg(Ctx &cx, int nev_, int b_) : c(cx), N(cx.bse.N), b(b_), nev(nev_), m3(3 * b_) {
+    Lobpcg(Ctx &cx, int nev_, int b_)
+        : c(cx), N(cx.bse.N), b(b_), nev(nev_), m3(3 * b_), S(cx.lws.S), AS(cx.lws.AS), T1(cx.lws.T1),
+          P(cx.lws.P), AP(cx.lws.AP), G(cx.lws.G), A(cx.lws.A), U(cx.lws.U), T(cx.lws.T), Tmp(cx.lws.Tmp),
+          Cm(cx.lws.Cm), D(cx.lws.D), lam(cx.lws.lam), lamr(cx.lws.lamr), norm2(cx.lws.norm2), sc(cx.lws.sc),
+          idx(cx.lws.idx), act(cx.lws.act) {
+        c.lws.hd.alloc(std::max<size_t>((size_t)N, 2 * (size_t)m3));
+        c.lws.hi.alloc((size_t)m3);
         tmv.st = c.stream;
         trr.st = c.stream;
         S.alloc((size_t)N * m3);
@@ -657,14 +668,13 @@ struct Lobpcg {
     std::vector<double> norms(int ncol, const cd *X, const cd *AX, const double *lamd) {
         hipLaunchKernelGGL(resid_norms, dim3(ncol), dim3(kNT), 0, c.stream, N, X, AX, lamd, norm2.p);
         WTB_LAUNCH_CHECK();
-        std::vector<double> h(ncol);
-        WTB_HIP_CHECK(hipMemcpyAsync(h.data(), norm2.p, sizeof(double) * ncol, hipMemcpyDeviceToHost, c.stream));
-        if (lamd) {
-            h_lam.resize(ncol);
-            WTB_HIP_CHECK(hipMemcpyAsync(h_lam.data(), lamd, sizeof(double) * ncol, hipMemcpyDeviceToHost, c.stream));
-        }
+        double *hp = c.lws.hd.p;
+        WTB_HIP_CHECK(hipMemcpyAsync(hp, norm2.p, sizeof(double) * ncol, hipMemcpyDeviceToHost, c.stream));
+        if (lamd)
+            WTB_HIP_CHECK(hipMemcpyAsync(hp + ncol, lamd, sizeof(double) * ncol, hipMemcpyDeviceToHost, c.stream));
         WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
-        return h;
+        if (lamd) h_lam.assign(hp + ncol, hp + 2 * ncol);
+        return std::vector<double>(hp, hp + ncol);
     }
 
     // B -= Q (Q^H B) for Q (N x q) with orthonormal columns, B (N x nb); the
@@ -838,14 +848,15 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     const double piv_p = (precision == WTB_FP64) ? 1e-12 : 1e-2;
     const double p_keep = (precision == WTB_FP64) ? 1e-6 : 3e-2;
     // initial block: unit vectors on the b smallest diagonal entries
-    std::vector<double> d(N);
-    WTB_HIP_CHECK(hipMemcpyAsync(d.data(), c.bse.diag.p, sizeof(double) * N, hipMemcpyDeviceToHost, c.stream));
+    const double *d = c.lws.hd.p;
+    WTB_HIP_CHECK(hipMemcpyAsync(c.lws.hd.p, c.bse.diag.p, sizeof(double) * N, hipMemcpyDeviceToHost, c.stream));
     WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
     std::vector<int> order(N);
     std::iota(order.begin(), order.end(), 0);
     std::partial_sort(order.begin(), order.begin() + b, order.end(),
                       [&](int x, int y) { return d[x] < d[y] || (d[x] == d[y] && x < y); });
-    WTB_HIP_CHECK(hipMemcpyAsync(L.idx.p, order.data(), sizeof(int) * b, hipMemcpyHostToDevice, c.stream));
+    std::copy(order.begin(), order.begin() + b, c.lws.hi.p);
+    WTB_HIP_CHECK(hipMemcpyAsync(L.idx.p, c.lws.hi.p, sizeof(int) * b, hipMemcpyHostToDevice, c.stream));
     hipLaunchKernelGGL(set_units, L.gN(b), dim3(256), 0, c.stream, N, b, L.idx.p, L.S.p);
     WTB_LAUNCH_CHECK();
     const int mv_prec = precision;   // FP64 / split-FP16 (mixed) / FP32 products
@@ -891,10 +902,11 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
         if (worst > 10.0 * prev_best[0]) haveP = false;
         prev_best[0] = std::min(prev_best[0], worst);
         const int na = (int)active.size();
-        WTB_HIP_CHECK(hipMemcpyAsync(L.act.p, active.data(), sizeof(int) * na, hipMemcpyHostToDevice, c.stream));
+        // (the pinned index buffer is rewritten only after the next norms() sync)
+        std::copy(active.begin(), active.end(), c.lws.hi.p);
+        WTB_HIP_CHECK(hipMemcpyAsync(L.act.p, c.lws.hi.p, sizeof(int) * na, hipMemcpyHostToDevice, c.stream));
         // W = T R (unit scale first, normalised below)
-        std::vector<double> ones(na, 1.0);
-        WTB_HIP_CHECK(hipMemcpyAsync(L.sc.p, ones.data(), sizeof(double) * na, hipMemcpyHostToDevice, c.stream));
+        hipLaunchKernelGGL(fill_value, dim3(1), dim3(256), 0, c.stream, na, 1.0, L.sc.p);
         cd *W = L.S.p + (size_t)b * N;
         cd *AW = L.AS.p + (size_t)b * N;
         hipLaunchKernelGGL(precondition, L.gN(na), dim3(256), 0, c.stream, N, L.S.p, L.AS.p, L.lam.p,

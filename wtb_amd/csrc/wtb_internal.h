@@ -66,6 +66,43 @@ struct DBuf {
     ~DBuf() { free(); }
 };
 
+// Pinned host buffer for the solver's small device <-> host transfers: a
+// copy from or to it is a plain DMA, queued without a host-side stage.
+template <class T>
+struct HBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t count) {
+        if (count <= n && p) return;
+        free();
+        if (count == 0) return;
+        hipError_t e = hipHostMalloc(reinterpret_cast<void **>(&p), count * sizeof(T), hipHostMallocDefault);
+        if (e != hipSuccess) {
+            p = nullptr;
+            throw Error{WTB_ERR_NOMEM, "hipHostMalloc of " + std::to_string(count * sizeof(T)) +
+                                           " bytes failed: " + hipGetErrorString(e)};
+        }
+        n = count;
+    }
+    void free() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~HBuf() { free(); }
+};
+
+// Block eigensolver workspace, kept across solves (the Krylov block alone is
+// 6 N b complex values; allocating it per solve costs ~0.3 ms of hipMalloc).
+struct SolverWs {
+    DBuf<cd> S, AS, T1, P, AP;      // N x cols
+    DBuf<cd> G, A, U, T, Tmp, Cm;   // small (3b) x (3b)
+    DBuf<double> D, lam, lamr, norm2, sc;
+    DBuf<int> idx, act;
+    HBuf<double> hd;                // pinned: diagonal, residual norms, Ritz values
+    HBuf<int> hi;                   // pinned: active column indices
+};
+
 struct Model {
     int norb = 0, nvec = 0;
     double rlat[9] = {0};        // rows a_i
@@ -190,6 +227,7 @@ struct Ctx {
     DBuf<float> ws_colscale;   // per-column split / output scales
     DBuf<double> ws_scale8;    // per-column scales of the int8-digit matvec
     DBuf<cd> ws_ypart;         // k-chunk partial products of the int8-digit matvec
+    SolverWs lws;
     KernelClock kclock;
 };
 
