@@ -193,11 +193,11 @@ def test_matvec_int8_needs_fp32_store(rng):
     ctx.close()
 
 
-@pytest.mark.parametrize("nk,nev", [(12, 144), (10, 100), (16, 96), (22, 200), (32, 64)])
+@pytest.mark.parametrize("nk,nev", [(9, 81), (8, 20), (12, 144), (10, 100), (16, 96), (22, 200), (32, 64)])
 def test_solve_dense_small(nk, nev):
     """Dense path (N <= 1024): the tridiagonalisation runs LDS-resident for
-    N <= 90 and blocked with panel widths 16 (N <= 256), 8 (N = 484) and 3
-    (N = 1024) above."""
+    N <= 90 (N = 81, 64) and blocked with panel widths 16 (N <= 256), 8
+    (N = 484) and 3 (N = 1024) above."""
     say("case dense honeycomb", nk, nev)
     m = T.honeycomb_two_band()
     mesh = (nk, nk, 1)

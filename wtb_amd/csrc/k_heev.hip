@@ -25,6 +25,7 @@ constexpr int kT = 1024;
 // tridiagonalisation and the Cholesky factorisation (with their work arrays
 // it stays under the 160 KB of a CU)
 constexpr int kTridLds = 90;
+constexpr int kTridT = 256;     // threads of the LDS-resident tridiagonalisation
 constexpr int kCholLds = 96;
 // Eigenvalues closer than this (relative to |T|) are treated as one cluster
 // and their inverse-iteration vectors are re-orthogonalised.  Outside a
@@ -32,6 +33,7 @@ constexpr int kCholLds = 96;
 // eps |T| / gap <= 2e-9.
 constexpr double kClusterRel = 1e-7;
 
+template <int NT = kT>
 __device__ __forceinline__ double block_reduce(double v, double *red) {
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
@@ -40,7 +42,7 @@ __device__ __forceinline__ double block_reduce(double v, double *red) {
     __syncthreads();
     double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < kT / 64; ++i) s += red[i];
+    for (int i = 0; i < NT / 64; ++i) s += red[i];
     return s;
 }
 
@@ -49,31 +51,33 @@ __device__ __forceinline__ double block_reduce(double v, double *red) {
 // A (n x n, column-major, lda): full Hermitian matrix, destroyed.  On exit
 // A[r + i lda] (r >= i+2) holds v_i(r) of reflector i (v_i(i+1) = 1), tau[i]
 // its scalar, d/e the real tridiagonal matrix T = Q^H A Q.
-// The whole matrix is staged in LDS (n <= kTridLds).
-__global__ void __launch_bounds__(kT)
+// The whole matrix is staged in LDS (n <= kTridLds); NT threads (256 keeps
+// the nine barriers of a step cheap at these sizes).
+template <int NT>
+__global__ void __launch_bounds__(NT)
 tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double *__restrict__ d,
                double *__restrict__ e) {
     extern __shared__ __align__(16) unsigned char smem[];
     cd *v = reinterpret_cast<cd *>(smem);     // n
     cd *y = v + n;                            // n
-    cd *part = y + n;                         // kT partial sums
-    cd *M = part + kT;                        // n x n
-    __shared__ double red[kT / 64];
+    cd *part = y + n;                         // NT partial sums
+    cd *M = part + NT;                        // n x n
+    __shared__ double red[NT / 64];
     __shared__ cd s_tau;
     const int tid = threadIdx.x;
     cd *A = M;
     const int lda = n;
-    for (int idx = tid; idx < n * n; idx += kT) M[idx] = Ag[(idx % n) + (size_t)(idx / n) * ldg];
+    for (int idx = tid; idx < n * n; idx += NT) M[idx] = Ag[(idx % n) + (size_t)(idx / n) * ldg];
     __syncthreads();
     for (int i = 0; i < n - 1; ++i) {
         const int s = n - i - 1;                               // trailing size
         // column norm below the subdiagonal
         double xn2 = 0.0;
-        for (int r = i + 2 + tid; r < n; r += kT) {
+        for (int r = i + 2 + tid; r < n; r += NT) {
             cd a = A[r + (size_t)i * lda];
             xn2 += a.x * a.x + a.y * a.y;
         }
-        xn2 = block_reduce(xn2, red);
+        xn2 = block_reduce<NT>(xn2, red);
         const cd alpha = A[(i + 1) + (size_t)i * lda];
         d[i] = A[i + (size_t)i * lda].x;
         if (xn2 == 0.0 && alpha.y == 0.0) {
@@ -90,7 +94,7 @@ tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double
         const cd t = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
         const double dd = (alpha.x - beta) * (alpha.x - beta) + alpha.y * alpha.y;
         const cd scal = make_double2((alpha.x - beta) / dd, -alpha.y / dd);   // 1/(alpha - beta)
-        for (int r = i + 1 + tid; r < n; r += kT) {
+        for (int r = i + 1 + tid; r < n; r += NT) {
             cd vr;
             if (r == i + 1) {
                 vr = make_double2(1.0, 0.0);
@@ -107,14 +111,15 @@ tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double
             s_tau = t;
         }
         __syncthreads();
-        // y = A_trail v  (rows i+1..n-1), threads split the columns in 4 groups
+        // y = A_trail v  (rows i+1..n-1), threads split the columns in NT/64 groups
         {
-            const int g = tid / 256, lr = tid % 256;
-            for (int r0 = 0; r0 < s; r0 += 256) {
+            constexpr int G = NT / 64;
+            const int g = tid / 64, lr = tid % 64;
+            for (int r0 = 0; r0 < s; r0 += 64) {
                 const int r = i + 1 + r0 + lr;
                 cd acc = make_double2(0.0, 0.0);
                 if (r < n) {
-                    for (int c = i + 1 + g; c < n; c += 4) {
+                    for (int c = i + 1 + g; c < n; c += G) {
                         cd a = A[r + (size_t)c * lda], vc = v[c];
                         acc.x += a.x * vc.x - a.y * vc.y;
                         acc.y += a.x * vc.y + a.y * vc.x;
@@ -124,9 +129,9 @@ tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double
                 __syncthreads();
                 if (g == 0 && r < n) {
                     cd sum = part[lr];
-                    for (int q = 1; q < 4; ++q) {
-                        sum.x += part[q * 256 + lr].x;
-                        sum.y += part[q * 256 + lr].y;
+                    for (int q = 1; q < G; ++q) {
+                        sum.x += part[q * 64 + lr].x;
+                        sum.y += part[q * 64 + lr].y;
                     }
                     y[r] = sum;
                 }
@@ -135,17 +140,17 @@ tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double
         }
         // mu = Re(v^H y), w = tau y - |tau|^2 mu / 2 v
         double mu = 0.0;
-        for (int r = i + 1 + tid; r < n; r += kT) mu += v[r].x * y[r].x + v[r].y * y[r].y;
-        mu = block_reduce(mu, red);
+        for (int r = i + 1 + tid; r < n; r += NT) mu += v[r].x * y[r].x + v[r].y * y[r].y;
+        mu = block_reduce<NT>(mu, red);
         const double t2 = t.x * t.x + t.y * t.y;
-        for (int r = i + 1 + tid; r < n; r += kT) {
+        for (int r = i + 1 + tid; r < n; r += NT) {
             cd yr = y[r], vr = v[r];
             y[r] = make_double2(t.x * yr.x - t.y * yr.y - 0.5 * t2 * mu * vr.x,
                                 t.x * yr.y + t.y * yr.x - 0.5 * t2 * mu * vr.y);
         }
         __syncthreads();
         // A_trail -= v w^H + w v^H
-        for (int idx = tid; idx < s * s; idx += kT) {
+        for (int idx = tid; idx < s * s; idx += NT) {
             const int r = i + 1 + idx % s, c = i + 1 + idx / s;
             cd vr = v[r], wr = y[r], vc = v[c], wc = y[c];
             cd a = A[r + (size_t)c * lda];
@@ -161,7 +166,7 @@ tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double
         if (n >= 1) tau[n > 1 ? n - 1 : 0] = make_double2(0.0, 0.0);
     }
     // the reflectors (strict lower part) are read by the back-transformation
-    for (int idx = tid; idx < n * n; idx += kT) Ag[(idx % n) + (size_t)(idx / n) * ldg] = M[idx];
+    for (int idx = tid; idx < n * n; idx += NT) Ag[(idx % n) + (size_t)(idx / n) * ldg] = M[idx];
 }
 
 // Blocked form of the same reduction for n > kTridLds (the LATRD scheme).  The
@@ -837,8 +842,8 @@ void heev_lowest(Ctx &c, int n, int nvec, cd *A, int lda, double *W, cd *Z, int 
     cd *tau = c.ws_heev_c.p;
     double *d = c.ws_heev_d.p, *e = d + n, *Y = e + n;
     if (n <= kTridLds)
-        hipLaunchKernelGGL(tridiag_kernel, dim3(1), dim3(kT), (2 * (size_t)n + kT + (size_t)n * n) * sizeof(cd),
-                           c.stream, n, A, lda, tau, d, e);
+        hipLaunchKernelGGL(tridiag_kernel<kTridT>, dim3(1), dim3(kTridT),
+                           (2 * (size_t)n + kTridT + (size_t)n * n) * sizeof(cd), c.stream, n, A, lda, tau, d, e);
     else {
         // panel width: V and W (2 nb n), y (n) and the partial sums (kT) in 156 KB
         const size_t fixed = ((size_t)n + kT) * sizeof(cd);
