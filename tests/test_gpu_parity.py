@@ -121,6 +121,40 @@ def test_bse_matrix_cubic_v3dl():
     ctx.close()
 
 
+TILE_CASES = {
+    # name: (model, mesh, nv, nc)
+    "honeycomb-7x10": ("honeycomb", (7, 10, 1), 1, 1),     # N = 70, 32-k tiles: blocks 32, 32, 6
+    "hex22-5x7-3x3": ("hex", (5, 7, 1), 3, 3),              # N = 315, 16-k tiles: blocks 16, 16, 3
+    "hex22-9x9-1x2": ("hex", (9, 9, 1), 1, 2),              # N = 162, 32-k tiles: blocks 32, 32, 17
+}
+
+
+@pytest.mark.parametrize("precision", [hip.FP64, hip.MIXED])
+@pytest.mark.parametrize("case", list(TILE_CASES))
+def test_bse_matrix_partial_tiles(case, precision, rng):
+    """Assembly on meshes whose k count is no multiple of the square tile and
+    whose N is no multiple of 64: the partial last block row, the mirrored
+    off-diagonal tiles and the zero rows N..ld-1 of the store (read by the
+    matvecs without bounds checks)."""
+    say("case partial tiles", case, precision)
+    name, mesh, nv, nc = TILE_CASES[case]
+    m = T.honeycomb_two_band() if name == "honeycomb" else hex_model(mesh)
+    c = dict(COUL, r0=10.0) if name == "honeycomb" else COUL
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, nv, nc)
+    ctx.bse_build(c, precision)
+    H = ctx.get_block(0, ctx.N, 0, ctx.N)
+    Hr = O.bse_matrix(k, ref["eig"], ref["vec"], O.coulomb(**c), mesh, m.rlat, nc, nv)
+    scale = np.abs(Hr).max()
+    assert np.abs(H - Hr).max() < (1e-12 * max(1.0, scale) if precision == hip.FP64 else 2e-6 * scale)
+    assert np.array_equal(H, H.conj().T)
+    X = rng.standard_normal((ctx.N, 24)) + 1j * rng.standard_normal((ctx.N, 24))
+    Y = ctx.matvec(X, precision)
+    rel = (np.linalg.norm(Y - H @ X, axis=0) / np.linalg.norm(H @ X, axis=0)).max()
+    say("matvec rel error", rel)
+    assert rel < (1e-13 if precision == hip.FP64 else 2e-6)
+    ctx.close()
+
+
 @pytest.mark.parametrize("precision", [hip.FP64, hip.MIXED, hip.FP32])
 @pytest.mark.parametrize("ncol", [1, 5, 16, 40, 130])
 def test_matvec(precision, ncol, rng):

@@ -7,13 +7,20 @@
 // (reference convention; the textbook one uses conj(M^v)), and the diagonal
 // is E_c - E_v + V(k,k).  Basis index i = k*nc*nv + c*nv + v.
 //
-// One 256-thread workgroup per tile of TK=64 k-points x TKP k'-points:
+// H is Hermitian, so only the lower triangle of square tiles of TK x TK
+// k-points is computed: one 256-thread workgroup per tile (bk, bk'), bk' <= bk.
 //   phase 1: U(k') of the tile is staged in LDS; every (k,k') pair gets V
 //            (FP64) and its two overlap blocks (FP64 or FP32), which then
 //            replace the staged U(k') in LDS.
-//   phase 2: the 64*nc*nv x TKP*nc*nv output tile is written column by column,
-//            each store covering 256 consecutive rows of one column
-//            (2 KB FP32 / 4 KB FP64 contiguous): the stage is HBM-write bound.
+//   phase 2: the TK*nc*nv square tile is written column by column (each store
+//            covering consecutive rows of one column) and, off the diagonal,
+//            its conjugate transpose into the mirrored tile: the stored
+//            matrix is exactly Hermitian by construction.  Off-diagonal tiles
+//            form their overlaps with FMAs; the diagonal tiles, which hold
+//            both triangles of their block, use uncontracted products whose
+//            rounding is symmetric under k <-> k'.
+// Half the pair products of a full sweep, so the kernel runs near the HBM
+// write rate of the 8 N^2 (FP32) or 16 N^2 (FP64) byte store.
 #include "coulomb.h"
 #include "wtb_internal.h"
 
@@ -75,7 +82,6 @@ CoulombDev make_coulomb(const wtb_coulomb &in, const double *rlat, const int *ng
 
 namespace {
 
-constexpr int kTK = 64;
 constexpr int kThreads = 256;
 
 template <class R>
@@ -88,33 +94,107 @@ struct CT<float> { using C = float2; };
 template <class C>
 __device__ __forceinline__ C mk(double a, double b) { return C{(decltype(C{}.x))a, (decltype(C{}.x))b}; }
 
-// R: arithmetic/store precision of the overlaps and the matrix; NC, NV: bands;
-// TK k-points per tile (TK nc nv rows), S: k' pairs per thread (tile width
-// TKP = S kThreads / TK k'-points).
-template <class R, int NC, int NV, int S, int TK = kTK>
+__device__ __forceinline__ float fma_r(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fma_r(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
+// t += conj(a) b.  SYM: no contraction, so the sums for (a, b) and for (b, a)
+// are exact conjugates of each other (a diagonal tile holds both triangles
+// of its block); otherwise two FMAs per component.
+template <bool SYM, class C>
+__device__ __forceinline__ void cmac(C &t, C a, C b) {
+    if constexpr (SYM) {
+#pragma clang fp contract(off)
+        t.x += a.x * b.x + a.y * b.y;
+        t.y += a.x * b.y - a.y * b.x;
+    } else {
+        t.x = fma_r(a.y, b.y, fma_r(a.x, b.x, t.x));
+        t.y = fma_r(-a.y, b.x, fma_r(a.x, b.y, t.y));
+    }
+}
+
+// V a b without contraction: the value for the pair (k', k) of a diagonal tile
+// is then the exact conjugate of the one for (k, k').
+template <class C, class R>
+__device__ __forceinline__ C pair_value(R vq, C a, C b) {
+#pragma clang fp contract(off)
+    C v;
+    v.x = vq * (a.x * b.x - a.y * b.y);
+    v.y = vq * (a.x * b.y + a.y * b.x);
+    return v;
+}
+
+// Overlap blocks M^c, M^v of thread's S pairs (k, k'): U(k) from global,
+// OB orbitals at a time, U(k') from the staged copy in LDS.
+template <bool SYM, class C, int NC, int NV, int S, int TK>
+__device__ __forceinline__ void pair_products(int k, int nk, int norb, const C *__restrict__ usel,
+                                              const C *uks, int tid, C (&mc)[S][NC * NC],
+                                              C (&mv)[S][NV * NV]) {
+    constexpr int NB = NC + NV;
+    constexpr int OB = sizeof(C) == 8 ? 8 : 4;
+    for (int o0 = 0; o0 < norb; o0 += OB) {
+        C ukb[OB][NB];
+#pragma unroll
+        for (int ob = 0; ob < OB; ++ob)
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                ukb[ob][j] = (o0 + ob < norb) ? usel[((size_t)(o0 + ob) * NB + j) * nk + k] : mk<C>(0.0, 0.0);
+#pragma unroll
+        for (int ob = 0; ob < OB; ++ob) {
+            const int o = o0 + ob;
+            if (o >= norb) break;
+            const C *uk = ukb[ob];
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int kpl = tid / TK + (kThreads / TK) * s;
+                const C *up = uks + (size_t)kpl * NB * norb + o;
+#pragma unroll
+                for (int c = 0; c < NC; ++c)
+#pragma unroll
+                    for (int c2 = 0; c2 < NC; ++c2) cmac<SYM>(mc[s][c * NC + c2], uk[NV + c], up[(NV + c2) * norb]);
+#pragma unroll
+                for (int v = 0; v < NV; ++v)
+#pragma unroll
+                    for (int v2 = 0; v2 < NV; ++v2) cmac<SYM>(mv[s][v * NV + v2], uk[v], up[v2 * norb]);
+            }
+        }
+    }
+}
+
+// Pair slots of one plane of pair data in LDS: pair (kl, kpl) sits at
+// kpl (TK + 1) + kl, so that lanes running over kl (row writes) and lanes
+// running over kpl (mirror writes) both hit distinct banks.
+template <int TK>
+constexpr int pair_slots() { return TK * (TK + 1) + 1; }
+
+// R: arithmetic/store precision of the overlaps and the matrix; NC, NV:
+// bands; TK: k-points per side of the square tile (TK nc nv rows and columns).
+// Workgroup t covers the tile (bk, bkp) of the lower block triangle,
+// t = bk (bk + 1) / 2 + bkp with bkp <= bk.
+template <class R, int NC, int NV, int TK>
 __global__ void __launch_bounds__(kThreads)
 bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
                  const double *__restrict__ kpt, const double *__restrict__ esel,
                  const typename CT<R>::C *__restrict__ usel, typename CT<R>::C *__restrict__ H,
                  unsigned *__restrict__ hmax) {
-    // No FMA contraction: every product and sum is then rounded the same way
-    // for the pair (k,k') as for (k',k) with the roles of the factors swapped,
-    // so the stored matrix is exactly Hermitian, H[j][i] == conj(H[i][j]) bit
-    // for bit (the matvec reads row i as the conjugate of stored column i).
-#pragma clang fp contract(off)
     using C = typename CT<R>::C;
-    float vmax = 0.f;   // largest |Re|, |Im| written (scales the FP16 split of the matvec)
     constexpr int NB = NC + NV, NCV = NC * NV, NC2 = NC * NC, NV2 = NV * NV;
-    constexpr int TKP = S * kThreads / TK;
+    constexpr int S = TK * TK / kThreads;              // pairs per thread
+    constexpr int NPP = pair_slots<TK>();
+    static_assert(kThreads % TK == 0 && S >= 1, "tile must split evenly over the workgroup");
     extern __shared__ __align__(16) unsigned char smem[];
     const long N = (long)nk * NCV;
-    const int k0 = blockIdx.x * TK, kp0 = blockIdx.y * TKP;
-    constexpr int NPAIR = TK * TKP;
-    // LDS layout: a union of {pair Mc, pair Mv} (written after phase 1) and
-    // the staged U(k') [kpl][j][o] (read in phase 1), then pair V (R).
-    const size_t pairs = (size_t)NPAIR * (NC2 + NV2), stage = (size_t)TKP * NB * norb;
+    const long t = blockIdx.x;
+    int bk = (int)((sqrt(8.0 * (double)t + 1.0) - 1.0) * 0.5);
+    while ((long)bk * (bk + 1) / 2 > t) --bk;
+    while ((long)(bk + 1) * (bk + 2) / 2 <= t) ++bk;
+    const int bkp = (int)(t - (long)bk * (bk + 1) / 2);
+    const bool dtile = bk == bkp;
+    const int k0 = bk * TK, kp0 = bkp * TK;
+    // LDS: the pair planes (NC2 of M^c, NV2 of M^v) share their space with
+    // the staged U(k') [kpl][j][o] read before them; then pair V (R).
+    const size_t pairs = (size_t)(NC2 + NV2) * NPP, stage = (size_t)TK * NB * norb;
     C *pmc = reinterpret_cast<C *>(smem);
-    C *pmv = pmc + (size_t)NPAIR * NC2;
+    C *pmv = pmc + (size_t)NC2 * NPP;
     C *uks = pmc;
     R *pv = reinterpret_cast<R *>(pmc + (pairs > stage ? pairs : stage));
     const int tid = threadIdx.x;
@@ -122,20 +202,20 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
     // stage U(k') for the tile's k' columns, eight loads in flight per thread
     {
         constexpr int kL = 8;
-        const int tot = TKP * NB * norb;
+        const int tot = TK * NB * norb;
         for (int base = 0; base < tot; base += kL * kThreads) {
             C v[kL];
 #pragma unroll
-            for (int t = 0; t < kL; ++t) {
-                const int idx = base + t * kThreads + tid;
+            for (int u = 0; u < kL; ++u) {
+                const int idx = base + u * kThreads + tid;
                 const int kpl = idx / (NB * norb), rem = idx % (NB * norb), j = rem / norb, o = rem % norb;
                 const int kp = kp0 + kpl;
-                v[t] = (idx < tot && kp < nk) ? usel[((size_t)o * NB + j) * nk + kp] : mk<C>(0.0, 0.0);
+                v[u] = (idx < tot && kp < nk) ? usel[((size_t)o * NB + j) * nk + kp] : mk<C>(0.0, 0.0);
             }
 #pragma unroll
-            for (int t = 0; t < kL; ++t) {
-                const int idx = base + t * kThreads + tid;
-                if (idx < tot) uks[idx] = v[t];
+            for (int u = 0; u < kL; ++u) {
+                const int idx = base + u * kThreads + tid;
+                if (idx < tot) uks[idx] = v[u];
             }
         }
     }
@@ -154,56 +234,14 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
             for (int e = 0; e < NV2; ++e) mv[s][e] = mk<C>(0.0, 0.0);
         }
         if (k < nk) {
-            // U(k) of OB orbitals is loaded before their products are formed
-            constexpr int OB = sizeof(R) == sizeof(float) ? 8 : 4;
-            for (int o0 = 0; o0 < norb; o0 += OB) {
-                C ukb[OB][NB];
-#pragma unroll
-                for (int ob = 0; ob < OB; ++ob)
-#pragma unroll
-                    for (int j = 0; j < NB; ++j)
-                        ukb[ob][j] = (o0 + ob < norb) ? usel[((size_t)(o0 + ob) * NB + j) * nk + k]
-                                                      : mk<C>(0.0, 0.0);
-#pragma unroll
-                for (int ob = 0; ob < OB; ++ob) {
-                    const int o = o0 + ob;
-                    if (o >= norb) break;
-                    const C *uk = ukb[ob];
-#pragma unroll
-                    for (int s = 0; s < S; ++s) {
-                        const int kpl = tid / TK + (kThreads / TK) * s;
-                        const C *up = uks + (size_t)kpl * NB * norb + o;
-#pragma unroll
-                        for (int c = 0; c < NC; ++c) {
-                            const C a = uk[NV + c];
-#pragma unroll
-                            for (int c2 = 0; c2 < NC; ++c2) {
-                                const C b = up[(NV + c2) * norb];
-                                C &t = mc[s][c * NC + c2];
-                                t.x += a.x * b.x + a.y * b.y;
-                                t.y += a.x * b.y - a.y * b.x;
-                            }
-                        }
-#pragma unroll
-                        for (int v = 0; v < NV; ++v) {
-                            const C a = uk[v];
-#pragma unroll
-                            for (int v2 = 0; v2 < NV; ++v2) {
-                                const C b = up[v2 * norb];
-                                C &t = mv[s][v * NV + v2];
-                                t.x += a.x * b.x + a.y * b.y;
-                                t.y += a.x * b.y - a.y * b.x;
-                            }
-                        }
-                    }
-                }
-            }
+            if (dtile) pair_products<true, C, NC, NV, S, TK>(k, nk, norb, usel, uks, tid, mc, mv);
+            else pair_products<false, C, NC, NV, S, TK>(k, nk, norb, usel, uks, tid, mc, mv);
         }
         __syncthreads();    // the pair products overwrite the staged U
 #pragma unroll
         for (int s = 0; s < S; ++s) {
             const int kpl = tid / TK + (kThreads / TK) * s;
-            const int p = kpl * TK + kl;
+            const int p = kpl * (TK + 1) + kl;
             const int kp = kp0 + kpl;
             double vq = 0.0;
             if (k < nk && kp < nk)
@@ -211,59 +249,85 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
                                kpt[3 * k + 2] - kpt[3 * kp + 2]);
             pv[p] = (R)vq;
 #pragma unroll
-            for (int e = 0; e < NC2; ++e) pmc[(size_t)p * NC2 + e] = mc[s][e];
+            for (int e = 0; e < NC2; ++e) pmc[(size_t)e * NPP + p] = mc[s][e];
 #pragma unroll
-            for (int e = 0; e < NV2; ++e) pmv[(size_t)p * NV2 + e] = mv[s][e];
+            for (int e = 0; e < NV2; ++e) pmv[(size_t)e * NPP + p] = mv[s][e];
         }
     }
     __syncthreads();
 
-    // phase 2: coalesced column writes.  Thread row il (k, c, v) takes one
-    // pair (k, k') at a time: the nc + nv overlap factors and V are read once
-    // from LDS for the nc nv outputs of the pair, each store covering 256
-    // consecutive rows of one column.
+    // phase 2: the tile, column by column.  Thread -> (row il, k' kpl): the
+    // nc + nv overlap factors and V are read once from LDS for the nc nv
+    // outputs of the pair; each store covers consecutive rows of one column.
+    float vmax = 0.f;   // largest |Re|, |Im| written (scales the FP16 split of the matvec)
     constexpr int ROWS = TK * NCV;
     const long I0 = (long)k0 * NCV, J0 = (long)kp0 * NCV;
-    for (int il = tid; il < ROWS; il += kThreads) {
-        const long I = I0 + il;
-        if (I >= ld) break;     // rows N..ld-1 come out zero (k >= nk has zero V and overlaps)
+    for (int idx = tid; idx < ROWS * TK; idx += kThreads) {
+        const int il = idx % ROWS, kpl = idx / ROWS;
+        const long I = I0 + il, Jb = J0 + (long)kpl * NCV;
+        if (I >= N || Jb >= N) continue;       // rows N..ld-1 are cleared separately
         const int kl = il / NCV, ci = (il % NCV) / NV, vi = il % NV;
-        for (int kpl = 0; kpl < TKP; ++kpl) {
-            const long Jb = J0 + (long)kpl * NCV;
-            if (Jb >= N) break;
-            const int p = kpl * TK + kl;
+        const int p = kpl * (TK + 1) + kl;
+        const R vq = pv[p];
+        C a[NC], b[NV];
+#pragma unroll
+        for (int cj = 0; cj < NC; ++cj) a[cj] = pmc[(size_t)(ci * NC + cj) * NPP + p];
+#pragma unroll
+        for (int vj = 0; vj < NV; ++vj) {
+            b[vj] = pmv[(size_t)(vi * NV + vj) * NPP + p];
+            if (textbook) b[vj].y = -b[vj].y;
+        }
+#pragma unroll
+        for (int cj = 0; cj < NC; ++cj)
+#pragma unroll
+            for (int vj = 0; vj < NV; ++vj) {
+                const long J = Jb + cj * NV + vj;
+                C val;
+                if (I == J) {
+                    // The FP32 store keeps a zero diagonal: E_c - E_v + V(0)
+                    // lives in the FP64 diagonal array and is applied in FP64
+                    // by the matvec, so FP32 rounding only touches the Coulomb part.
+                    if (sizeof(R) == sizeof(float)) {
+                        val = mk<C>(0.0, 0.0);
+                    } else {
+                        const int k = k0 + kl;
+                        double de = esel[(size_t)k * NB + NV + ci] - esel[(size_t)k * NB + vi];
+                        val = mk<C>(de + (double)vq, 0.0);
+                    }
+                } else {
+                    val = pair_value(vq, a[cj], b[vj]);
+                }
+                H[I + J * ld] = val;
+                vmax = fmaxf(vmax, fmaxf(fabsf((float)val.x), fabsf((float)val.y)));
+            }
+    }
+    // the mirror of an off-diagonal tile: rows J (k'), columns I (k), the
+    // conjugates of the values above, so the store is exactly Hermitian.
+    // Thread -> (k' kpl, column il): the nc nv rows of the pair are
+    // consecutive, and consecutive threads continue them down the column.
+    if (!dtile) {
+        for (int idx = tid; idx < ROWS * TK; idx += kThreads) {
+            const int kpl = idx % TK, il = idx / TK;
+            const long I = I0 + il, Jb = J0 + (long)kpl * NCV;
+            if (I >= N || Jb >= N) continue;
+            const int kl = il / NCV, ci = (il % NCV) / NV, vi = il % NV;
+            const int p = kpl * (TK + 1) + kl;
             const R vq = pv[p];
             C a[NC], b[NV];
 #pragma unroll
-            for (int cj = 0; cj < NC; ++cj) a[cj] = pmc[(size_t)p * NC2 + ci * NC + cj];
+            for (int cj = 0; cj < NC; ++cj) a[cj] = pmc[(size_t)(ci * NC + cj) * NPP + p];
 #pragma unroll
             for (int vj = 0; vj < NV; ++vj) {
-                b[vj] = pmv[(size_t)p * NV2 + vi * NV + vj];
+                b[vj] = pmv[(size_t)(vi * NV + vj) * NPP + p];
                 if (textbook) b[vj].y = -b[vj].y;
             }
+            C *col = H + (size_t)I * ld + Jb;
 #pragma unroll
             for (int cj = 0; cj < NC; ++cj)
 #pragma unroll
                 for (int vj = 0; vj < NV; ++vj) {
-                    const long J = Jb + cj * NV + vj;
-                    C val;
-                    if (I == J) {
-                        // The FP32 store keeps a zero diagonal: E_c - E_v + V(0)
-                        // lives in the FP64 diagonal array and is applied in FP64
-                        // by the matvec, so FP32 rounding only touches the Coulomb part.
-                        if (sizeof(R) == sizeof(float)) {
-                            val = mk<C>(0.0, 0.0);
-                        } else {
-                            const int k = k0 + kl;
-                            double de = esel[(size_t)k * NB + NV + ci] - esel[(size_t)k * NB + vi];
-                            val = mk<C>(de + (double)vq, 0.0);
-                        }
-                    } else {
-                        val.x = vq * (a[cj].x * b[vj].x - a[cj].y * b[vj].y);
-                        val.y = vq * (a[cj].x * b[vj].y + a[cj].y * b[vj].x);
-                    }
-                    H[I + J * ld] = val;
-                    vmax = fmaxf(vmax, fmaxf(fabsf((float)val.x), fabsf((float)val.y)));
+                    const C val = pair_value(vq, a[cj], b[vj]);
+                    col[cj * NV + vj] = C{val.x, -val.y};
                 }
         }
     }
@@ -273,25 +337,41 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
     }
 }
 
+// rows N..ld-1 of every stored column are zero (the matvecs read whole
+// k slices without bounds checks)
+template <class C>
+__global__ void zero_pad_rows_kernel(long N, long ld, C *__restrict__ H) {
+    const long pad = ld - N;
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (i >= pad * N) return;
+    H[N + i % pad + (i / pad) * ld] = C{0, 0};
+}
+
 template <class R, int NC, int NV>
 void launch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const typename CT<R>::C *usel,
                  typename CT<R>::C *H) {
     using C = typename CT<R>::C;
     constexpr int per = NC * NC + NV * NV;
-    // k' pairs per thread: the pair products of a tile stay within 64 KB
-    constexpr size_t kPairBytes = 64 * 1024;
-    constexpr int S0 = (int)(kPairBytes / ((size_t)kThreads * per * sizeof(C)));
-    constexpr int S = S0 >= 4 ? 4 : (S0 >= 2 ? 2 : 1);
-    constexpr int TK = kTK, TKP = S * kThreads / TK;
+    // square tiles of 32 k-points where their pair planes fit in 72 KB (two
+    // workgroups per CU), else of 16
+    constexpr int TK = (size_t)pair_slots<32>() * per * sizeof(C) <= 72 * 1024 ? 32 : 16;
     const int nk = c.bands.nk, norb = c.model.norb;
-    const size_t pairs = (size_t)TK * TKP * per, ukp = (size_t)TKP * (NC + NV) * norb;
-    const size_t shmem = std::max(pairs, ukp) * sizeof(C) + (size_t)TK * TKP * sizeof(R) + 16;
+    const long N = (long)nk * NC * NV, ld = c.bse.ld;
+    const size_t pairs = (size_t)pair_slots<TK>() * per, stage = (size_t)TK * (NC + NV) * norb;
+    const size_t shmem = std::max(pairs, stage) * sizeof(C) + (size_t)pair_slots<TK>() * sizeof(R) + 16;
     if (shmem > 160 * 1024) throw Error{WTB_ERR_UNSUPPORTED, "assembly tile does not fit in LDS"};
-    dim3 grid((nk + TK - 1) / TK, (nk + TKP - 1) / TKP);
+    const long nblk = (nk + TK - 1) / TK;
+    const long ntile = nblk * (nblk + 1) / 2;
     unsigned *hm = sizeof(R) == sizeof(float) ? c.bse.hmax.p : nullptr;
-    hipLaunchKernelGGL((bse_build_kernel<R, NC, NV, S, TK>), grid, dim3(kThreads), shmem, c.stream, nk, norb,
-                       c.bse.ld, textbook, cd_, c.kpt.p, c.bands.esel.p, usel, H, hm);
+    hipLaunchKernelGGL((bse_build_kernel<R, NC, NV, TK>), dim3((unsigned)ntile), dim3(kThreads), shmem, c.stream,
+                       nk, norb, ld, textbook, cd_, c.kpt.p, c.bands.esel.p, usel, H, hm);
     WTB_LAUNCH_CHECK();
+    if (ld > N) {
+        const long tot = (ld - N) * N;
+        hipLaunchKernelGGL(zero_pad_rows_kernel<C>, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, N,
+                           ld, H);
+        WTB_LAUNCH_CHECK();
+    }
 }
 
 template <class R>

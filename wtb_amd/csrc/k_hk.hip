@@ -1,8 +1,11 @@
 // Batched H(k) = sum_R e^{ik.R} H(R)/deg(R) - E_F build and Hermitian
-// eigensolve for every k in one launch (replaces the eigsys k-loop of
+// eigensolve for every k in three launches (replaces the eigsys k-loop of
 // bsesolver, src/module_bse.F90:2230-2245 and src/module_hamiltonian.F90:200-320).
 //
-// One 64-lane workgroup (a single wavefront) per k-point, norb <= 32:
+// One 64-lane workgroup (a single wavefront) per k-point, norb <= 32; steps
+// 1-2, 3 and 4 are separate kernels, so that the latency-bound QL recurrence
+// runs with only its real Z in LDS (four times the wavefronts per CU of the
+// complex A and Q of steps 1-2):
 //   1. Fourier sum: each lane owns up to 16 matrix entries and accumulates
 //      over R in the reference's order; phases are staged in LDS per chunk.
 //   2. Householder reduction to a real tridiagonal T = Q^H H(k) Q with the
@@ -42,25 +45,21 @@ __device__ __forceinline__ double lane_get(double v, int i) {
 // v with lane i replaced by x
 __device__ __forceinline__ double lane_set(double v, int i, double x) { return (int)threadIdx.x == i ? x : v; }
 
+// 1 + 2: H(k) and its reduction T = Q^H H(k) Q.  Q -> qout[k] ([o*n + j]);
+// the diagonal and the off-diagonal of T -> tri[k] at 0 and kMaxOrb.
 __global__ void __launch_bounds__(kLanes)
-hk_eig_kernel(int nk, int n, int nvec, int nv, int nc, int flags, double efermi, double scissor,
-              const double *__restrict__ kpt, const cd *__restrict__ hop,
-              const double *__restrict__ rcart, const double *__restrict__ invdeg,
-              double *__restrict__ eall, cd *__restrict__ uall, int *__restrict__ nocp_out,
-              double *__restrict__ esel, cd *__restrict__ usel, int *__restrict__ status) {
+hk_house_kernel(int n, int nvec, int flags, double efermi, const double *__restrict__ kpt,
+                const cd *__restrict__ hop, const double *__restrict__ rcart, const double *__restrict__ invdeg,
+                cd *__restrict__ qout, double *__restrict__ tri) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int k = blockIdx.x;
     const int tid = threadIdx.x;
     const int nn = n * n;
     cd *A = reinterpret_cast<cd *>(smem);            // [r*n + s]
-    cd *V = A + nn;                                  // Q, then the eigenvectors, [o*n + j]
+    cd *V = A + nn;                                  // Q, [o*n + j]
     cd *ph = V + nn;                                 // kLanes phases
     cd *vb = ph + kLanes;                            // 32: Householder vector
     cd *yb = vb + kMaxOrb;                           // 32: A v
-    double *wv = reinterpret_cast<double *>(yb + kMaxOrb);   // 32: diagonal of T, eigenvalues
-    double *eo = wv + kMaxOrb;                       // 32: off-diagonal of T
-    int *perm = reinterpret_cast<int *>(eo + kMaxOrb);       // 32
-
     const bool compat = flags & WTB_FLAG_COMPAT_F32;
     const double kx = kpt[3 * k], ky = kpt[3 * k + 1], kz = kpt[3 * k + 2];
 
@@ -160,16 +159,30 @@ hk_eig_kernel(int nk, int n, int nvec, int nv, int nc, int flags, double efermi,
         }
         __syncthreads();
     }
-    double *dg = wv;     // eo[j] couples j and j + 1
+    for (int idx = tid; idx < nn; idx += kLanes) qout[(size_t)k * nn + idx] = V[idx];
     if (tid < n) {
-        dg[tid] = A[tid * n + tid].x;
-        eo[tid] = tid + 1 < n ? A[(tid + 1) * n + tid].x : 0.0;
+        double *t = tri + (size_t)k * 3 * kMaxOrb;
+        t[tid] = A[tid * n + tid].x;                                   // e[j] couples j and j + 1
+        t[kMaxOrb + tid] = tid + 1 < n ? A[(tid + 1) * n + tid].x : 0.0;
     }
-    __syncthreads();
-    double *Z = reinterpret_cast<double *>(A);     // real rotations of T, [o*n + j]
+}
+
+// 3: implicit QL on T (tql2) for every k, the rotations accumulated in a real
+// Z ([o*n + j] -> zout[k]) whose columns get their largest entry positive;
+// the eigenvalues (unsorted) -> tri[k] at 2 kMaxOrb.  The recurrence is a
+// sequential chain per k, so only Z sits in LDS and many wavefronts share a
+// CU to hide its latency.
+__global__ void __launch_bounds__(kLanes)
+hk_ql_kernel(int n, double *__restrict__ tri, double *__restrict__ zout, int *__restrict__ status) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int k = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nn = n * n;
+    double *Z = reinterpret_cast<double *>(smem);     // [o*n + j]
+    const double *dg = tri + (size_t)k * 3 * kMaxOrb, *eo = dg + kMaxOrb;
+    double *wg = tri + (size_t)k * 3 * kMaxOrb + 2 * kMaxOrb;
     for (int idx = tid; idx < nn; idx += kLanes) Z[idx] = (idx / n == idx % n) ? 1.0 : 0.0;
     __syncthreads();
-
     // ---- 3. implicit QL on T (tql2), the rotations accumulated in Z ----
     // The diagonal and off-diagonal live in registers, lane j holding d_j and
     // e_j; the recurrence is uniform across the wavefront and reads / writes
@@ -225,7 +238,7 @@ hk_eig_kernel(int nk, int n, int nvec, int nv, int nc, int flags, double efermi,
             ej = lane_set(ej, m, 0.0);
         }
     }
-    if (tid < n) dg[tid] = dj;
+    if (tid < n) wg[tid] = dj;
     if (failed && tid == 0) atomicOr(status, 4);
     __syncthreads();
     // sign: the largest |Z| entry of each column (Z = Q^H z) positive, the
@@ -242,6 +255,30 @@ hk_eig_kernel(int nk, int n, int nvec, int nv, int nc, int flags, double efermi,
         if (sg < 0.0)
             for (int r = 0; r < n; ++r) Z[r * n + tid] = -Z[r * n + tid];
     }
+    __syncthreads();
+    for (int idx = tid; idx < nn; idx += kLanes) zout[(size_t)k * nn + idx] = Z[idx];
+}
+
+// 4: eigenvectors V = Q Z (Q read back from uall[k], where hk_house_kernel
+// left it), ascending order, nocp, scissor and the outputs
+__global__ void __launch_bounds__(kLanes)
+hk_vec_kernel(int nk, int n, int nv, int nc, double scissor, const double *__restrict__ tri,
+              const double *__restrict__ zin, double *__restrict__ eall, cd *__restrict__ uall,
+              int *__restrict__ nocp_out, double *__restrict__ esel, cd *__restrict__ usel,
+              int *__restrict__ status) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int k = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nn = n * n;
+    cd *V = reinterpret_cast<cd *>(smem);                     // Q, then the eigenvectors, [o*n + j]
+    double *Z = reinterpret_cast<double *>(V + nn);           // [o*n + j]
+    double *wv = Z + nn;                                      // 32: eigenvalues
+    int *perm = reinterpret_cast<int *>(wv + kMaxOrb);        // 32
+    for (int idx = tid; idx < nn; idx += kLanes) {
+        V[idx] = uall[(size_t)k * nn + idx];
+        Z[idx] = zin[(size_t)k * nn + idx];
+    }
+    if (tid < n) wv[tid] = tri[(size_t)k * 3 * kMaxOrb + 2 * kMaxOrb + tid];
     __syncthreads();
     // eigenvectors V = Q Z, lane o forms row o
     if (tid < n) {
@@ -325,11 +362,18 @@ void launch_hk_eig(Ctx &c, int nv, int nc) {
     b.usel32.alloc((size_t)nk * nb * n);
     b.status.alloc(1);
     WTB_HIP_CHECK(hipMemsetAsync(b.status.p, 0, sizeof(int), c.stream));
-    size_t shmem = 2 * (size_t)n * n * sizeof(cd) + (kLanes + 2 * kMaxOrb) * sizeof(cd) +
-                   2 * kMaxOrb * sizeof(double) + kMaxOrb * sizeof(int);
-    hipLaunchKernelGGL(hk_eig_kernel, dim3(nk), dim3(kLanes), shmem, c.stream, nk, n, m.nvec, nv, nc,
-                       m.flags, m.efermi, m.scissor, c.kpt.p, m.hop.p, m.rcart.p, m.invdeg.p,
-                       b.eall.p, b.uall.p, b.nocp.p, b.esel.p, b.usel.p, b.status.p);
+    b.zrot.alloc((size_t)nk * n * n);
+    b.tri.alloc((size_t)nk * 3 * kMaxOrb);
+    const size_t sh_house = 2 * (size_t)n * n * sizeof(cd) + (kLanes + 2 * kMaxOrb) * sizeof(cd);
+    hipLaunchKernelGGL(hk_house_kernel, dim3(nk), dim3(kLanes), sh_house, c.stream, n, m.nvec, m.flags, m.efermi,
+                       c.kpt.p, m.hop.p, m.rcart.p, m.invdeg.p, b.uall.p, b.tri.p);
+    WTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(hk_ql_kernel, dim3(nk), dim3(kLanes), (size_t)n * n * sizeof(double), c.stream, n, b.tri.p,
+                       b.zrot.p, b.status.p);
+    WTB_LAUNCH_CHECK();
+    const size_t sh_vec = (size_t)n * n * (sizeof(cd) + sizeof(double)) + kMaxOrb * (sizeof(double) + sizeof(int));
+    hipLaunchKernelGGL(hk_vec_kernel, dim3(nk), dim3(kLanes), sh_vec, c.stream, nk, n, nv, nc, m.scissor, b.tri.p,
+                       b.zrot.p, b.eall.p, b.uall.p, b.nocp.p, b.esel.p, b.usel.p, b.status.p);
     WTB_LAUNCH_CHECK();
     size_t tot = (size_t)nk * nb * n;
     hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream,

@@ -6,13 +6,15 @@
 // preconditioner W = (diag(H) - l)^-1 R.  The BSE matrix is strongly
 // diagonally dominant (diag = E_c - E_v), so the block converges in a handful
 // of iterations and the cost is dominated by the H*X products (k_matvec).
-// The Rayleigh-Ritz step is done in FP64 on the basis S = [X W P] without
-// explicit orthogonalisation: G = S^H S is diagonalised (SVQB, small
-// directions dropped), the projected problem T^H (S^H H S) T is solved with a
-// GPU Jacobi eigensolver, and the Ritz block is formed with tall-skinny GEMMs.
-// Mixed precision: the products run on the FP32 store; a final FP64
-// refinement applies H in FP64 arithmetic to the converged block, redoes the
-// Rayleigh-Ritz step and reports FP64 residuals.
+// The Rayleigh-Ritz step is done in FP64 on the basis S = [X W P]: W and P
+// are made orthogonal to what precedes them (two classical Gram-Schmidt
+// passes) and orthonormal (CholQR2), so the projected problem S^H H S is a
+// standard Hermitian eigenproblem solved on the GPU (tridiagonalisation,
+// bisection, inverse iteration); a rank-deficient W falls back to SVQB on the
+// Gram matrix.  The Ritz block is formed with tall-skinny GEMMs.
+// Mixed precision: the products run on the FP32 store; a final refinement
+// applies H to the converged block with exact int8-digit products recombined
+// in FP64, redoes the Rayleigh-Ritz step and reports FP64 residuals.
 #include "wtb_internal.h"
 
 #include <algorithm>
@@ -397,21 +399,23 @@ gram_block_kernel(int m, int n, long K, long kc, const cd *__restrict__ A, int l
 }
 
 // C[i + j ldc] = sum over chunks of part[s][i + j m]: lanes take 64 consecutive
-// elements, the four wavefronts every fourth chunk with eight loads in
-// flight, then an LDS sum in a fixed order.
-__global__ void __launch_bounds__(256)
+// elements, the kRedG wavefronts every kRedG-th chunk with eight loads in
+// flight, then an LDS sum in a fixed order.  Sixteen wavefronts per block keep
+// enough loads in flight to stream the partials near the HBM rate.
+constexpr int kRedG = 16;
+__global__ void __launch_bounds__(64 * kRedG)
 reduce_chunks(int m, int n, int nch, const cd *__restrict__ part, cd *__restrict__ C, int ldc) {
-    __shared__ cd red[4][64];
+    __shared__ cd red[kRedG][64];
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     const long mn = (long)m * n, e = (long)blockIdx.x * 64 + lane;
     const long ec = e < mn ? e : mn - 1;
     cd acc[8];
 #pragma unroll
     for (int u = 0; u < 8; ++u) acc[u] = make_double2(0.0, 0.0);
-    for (int s0 = g; s0 < nch; s0 += 32) {
+    for (int s0 = g; s0 < nch; s0 += 8 * kRedG) {
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int s = s0 + 4 * u;
+            const int s = s0 + kRedG * u;
             if (s < nch) {
                 const cd v = part[(size_t)s * mn + ec];
                 acc[u].x += v.x;
@@ -429,7 +433,7 @@ reduce_chunks(int m, int n, int nch, const cd *__restrict__ part, cd *__restrict
     __syncthreads();
     if (g == 0 && e < mn) {
         cd r = red[0][lane];
-        for (int q = 1; q < 4; ++q) {
+        for (int q = 1; q < kRedG; ++q) {
             r.x += red[q][lane].x;
             r.y += red[q][lane].y;
         }
@@ -459,35 +463,42 @@ void gram(Ctx &c, int m, int n, long K, const cd *A, int lda, const cd *B, int l
 #undef WTB_GRAM_CASE
     }
     WTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(reduce_chunks, dim3((unsigned)(((long)m * n + 63) / 64)), dim3(256), 0, c.stream, m, n, nsp,
+    hipLaunchKernelGGL(reduce_chunks, dim3((unsigned)(((long)m * n + 63) / 64)), dim3(64 * kRedG), 0, c.stream, m, n, nsp,
                        c.ws_gram.p, C, ldc);
     WTB_LAUNCH_CHECK();
 }
 
 // B <- B R^-1 in place for tall B (N x nb, ldb) and upper triangular R
-// (nb x nb, ldr): one thread per row of B solves y R = x by forward
-// substitution, 16 columns at a time (the solved columns are read back from
-// B, eight at a time).  R is staged in LDS, packed by columns (column j at
+// (nb x nb, ldr): each row of B solves y R = x by forward substitution, 16
+// columns at a time.  R is staged in LDS, packed by columns (column j at
 // j (j + 1) / 2), with the reciprocals of its diagonal.  gridDim.y = 2
 // applies the same R to B2.
-constexpr int kTrsT = 64;
-__global__ void __launch_bounds__(kTrsT)
-trsm_ru_kernel(long N, int nb, const cd *__restrict__ R, int ldr, cd *__restrict__ B, cd *__restrict__ B2, long ldb) {
+// Four lanes per row.  Per 16-column block J: the update x_J -= y_k R[k, J] by
+// the solved columns is split over the lanes by k mod 4 and summed with lane
+// shuffles in a fixed order; the substitution inside the block then runs in
+// all four lanes alike, and lane q writes the columns j = q mod 4.  The
+// update of lane q reads only columns it wrote itself (k = q mod 4), so its
+// own program order makes them visible.
+constexpr int kTr4 = 256;
+__global__ void __launch_bounds__(kTr4)
+trsm_ru4_kernel(long N, int nb, const cd *__restrict__ R, int ldr, cd *__restrict__ B, cd *__restrict__ B2,
+                long ldb) {
+    constexpr int JB = 16;
     extern __shared__ __align__(16) unsigned char smem[];
     cd *rp = reinterpret_cast<cd *>(smem);                       // packed upper R
     double *rinv = reinterpret_cast<double *>(rp + (size_t)nb * (nb + 1) / 2);
     {   // stage R: eight loads in flight per thread
         const int tot = nb * nb;
-        for (int base = 0; base < tot; base += 8 * kTrsT) {
+        for (int base = 0; base < tot; base += 8 * kTr4) {
             cd v[8];
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
-                const int idx = min(base + t * kTrsT + (int)threadIdx.x, tot - 1);
+                const int idx = min(base + t * kTr4 + (int)threadIdx.x, tot - 1);
                 v[t] = R[idx % nb + (size_t)(idx / nb) * ldr];
             }
 #pragma unroll
             for (int t = 0; t < 8; ++t) {
-                const int idx = base + t * kTrsT + threadIdx.x, i = idx % nb, j = idx / nb;
+                const int idx = base + t * kTr4 + threadIdx.x, i = idx % nb, j = idx / nb;
                 if (idx < tot && i <= j) {
                     rp[(size_t)j * (j + 1) / 2 + i] = v[t];
                     if (i == j) rinv[j] = 1.0 / v[t].x;
@@ -497,34 +508,57 @@ trsm_ru_kernel(long N, int nb, const cd *__restrict__ R, int ldr, cd *__restrict
     }
     __syncthreads();
     cd *M = blockIdx.y ? B2 : B;
-    const long row = (long)blockIdx.x * kTrsT + threadIdx.x;
-    if (row >= N) return;
-    cd *x = M + row;
-    constexpr int JB = 16;
+    const int q = threadIdx.x & 3;
+    const long row = (long)blockIdx.x * (kTr4 / 4) + (threadIdx.x >> 2);
+    const bool own = row < N;
+    cd *x = M + (own ? row : N - 1);      // rows past N read a valid row and write nothing
     for (int j0 = 0; j0 < nb; j0 += JB) {
-        cd y[JB];
+        cd acc[JB];
 #pragma unroll
-        for (int u = 0; u < JB; ++u) y[u] = j0 + u < nb ? x[(size_t)(j0 + u) * ldb] : make_double2(0.0, 0.0);
-        // x_J -= y_k R[k, J] for the solved columns k < j0 (read back eight at a time)
-        for (int k0 = 0; k0 < j0; k0 += 8) {
+        for (int u = 0; u < JB; ++u) acc[u] = make_double2(0.0, 0.0);
+        // this lane's share of the update: solved columns k = 4 t + q < j0,
+        // read back eight at a time
+        const int nt = j0 / 4;
+        for (int t0 = 0; t0 < nt; t0 += 8) {
             cd yk[8];
 #pragma unroll
-            for (int t = 0; t < 8; ++t) yk[t] = x[(size_t)min(k0 + t, j0 - 1) * ldb];
+            for (int tt = 0; tt < 8; ++tt)
+                yk[tt] = x[(size_t)(4 * min(t0 + tt, nt - 1) + q) * ldb];
 #pragma unroll
-            for (int t = 0; t < 8; ++t) {
-                const int k = k0 + t;
-                if (k < j0) {
+            for (int tt = 0; tt < 8; ++tt) {
+                if (t0 + tt < nt) {
+                    const int k = 4 * (t0 + tt) + q;
 #pragma unroll
                     for (int u = 0; u < JB; ++u) {
                         const int j = min(j0 + u, nb - 1);
                         const cd r = rp[(size_t)j * (j + 1) / 2 + k];
-                        y[u].x -= yk[t].x * r.x - yk[t].y * r.y;
-                        y[u].y -= yk[t].x * r.y + yk[t].y * r.x;
+                        acc[u].x += yk[tt].x * r.x - yk[tt].y * r.y;
+                        acc[u].y += yk[tt].x * r.y + yk[tt].y * r.x;
                     }
                 }
             }
         }
-        // substitution inside the block
+        // sum over the four lanes of the row (xor 1, then xor 2: the same
+        // order in every lane, so all four hold identical sums)
+#pragma unroll
+        for (int u = 0; u < JB; ++u) {
+#pragma unroll
+            for (int o = 1; o <= 2; o <<= 1) {
+                acc[u].x += __shfl_xor(acc[u].x, o, 64);
+                acc[u].y += __shfl_xor(acc[u].y, o, 64);
+            }
+        }
+        cd y[JB];
+#pragma unroll
+        for (int u = 0; u < JB; ++u) {
+            const int j = j0 + u;
+            y[u] = make_double2(0.0, 0.0);
+            if (j < nb) {
+                const cd xv = x[(size_t)j * ldb];
+                y[u] = make_double2(xv.x - acc[u].x, xv.y - acc[u].y);
+            }
+        }
+        // substitution inside the block (every lane)
 #pragma unroll
         for (int u = 0; u < JB; ++u) {
             const int j = j0 + u;
@@ -538,8 +572,14 @@ trsm_ru_kernel(long N, int nb, const cd *__restrict__ R, int ldr, cd *__restrict
                 const double ri = rinv[j];
                 y[u].x *= ri;
                 y[u].y *= ri;
-                x[(size_t)j * ldb] = y[u];
             }
+        }
+#pragma unroll
+        for (int s4 = 0; s4 < JB / 4; ++s4) {
+            const cd v0 = y[4 * s4], v1 = y[4 * s4 + 1], v2 = y[4 * s4 + 2], v3 = y[4 * s4 + 3];
+            const cd v = q == 0 ? v0 : (q == 1 ? v1 : (q == 2 ? v2 : v3));
+            const int j = j0 + 4 * s4 + q;
+            if (own && j < nb) x[(size_t)j * ldb] = v;
         }
     }
 }
@@ -554,8 +594,8 @@ void trsm_right_upper(Ctx &c, long N, int nb, const cd *R, int ldr, cd *B, cd *B
                                              (rocblas_int)N));
         return;
     }
-    hipLaunchKernelGGL(trsm_ru_kernel, dim3((unsigned)((N + kTrsT - 1) / kTrsT), B2 ? 2 : 1), dim3(kTrsT), sh,
-                       c.stream, N, nb, R, ldr, B, B2, N);
+    hipLaunchKernelGGL(trsm_ru4_kernel, dim3((unsigned)((N + kTr4 / 4 - 1) / (kTr4 / 4)), B2 ? 2 : 1), dim3(kTr4),
+                       sh, c.stream, N, nb, R, ldr, B, B2, N);
     WTB_LAUNCH_CHECK();
 }
 
@@ -636,7 +676,7 @@ struct Lobpcg {
         trr.st = c.stream;
         S.alloc((size_t)N * m3);
         AS.alloc((size_t)N * m3);
-        T1.alloc((size_t)N * 2 * b);
+        T1.alloc((size_t)N * m3);       // trades places with S and AS
         P.alloc((size_t)N * b);
         AP.alloc((size_t)N * b);
         G.alloc((size_t)m3 * m3);
@@ -766,16 +806,20 @@ struct Lobpcg {
         }
         WTB_HIP_CHECK(hipMemcpyAsync(lam.p, lamr.p, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));
         // new X, AX (and P, AP)
+        // P and H P come straight out of the products (they read only S and AS);
+        // the new X (H X) is formed in T1, which then trades places with S
+        // (AS): the rest of S is rebuilt every iteration.  A partial update
+        // (nb < b) copies its columns back instead.
+        if (withP && m > b) {
+            gemm_nn(c, N, b, m - b, S.p + (size_t)b * N, N, Cm.p + b, m3, P.p, N);
+            gemm_nn(c, N, b, m - b, AS.p + (size_t)b * N, N, Cm.p + b, m3, AP.p, N);
+        }
         gemm_nn(c, N, nb, m, S.p, N, Cm.p, m3, T1.p, N);
-        if (withP && m > b) gemm_nn(c, N, b, m - b, S.p + (size_t)b * N, N, Cm.p + b, m3, T1.p + (size_t)b * N, N);
-        WTB_HIP_CHECK(hipMemcpyAsync(S.p, T1.p, sizeof(cd) * N * nb, hipMemcpyDeviceToDevice, c.stream));
-        if (withP && m > b)
-            WTB_HIP_CHECK(hipMemcpyAsync(P.p, T1.p + (size_t)b * N, sizeof(cd) * N * b, hipMemcpyDeviceToDevice, c.stream));
+        if (nb == b) swap_bufs(S, T1);
+        else WTB_HIP_CHECK(hipMemcpyAsync(S.p, T1.p, sizeof(cd) * N * nb, hipMemcpyDeviceToDevice, c.stream));
         gemm_nn(c, N, nb, m, AS.p, N, Cm.p, m3, T1.p, N);
-        if (withP && m > b) gemm_nn(c, N, b, m - b, AS.p + (size_t)b * N, N, Cm.p + b, m3, T1.p + (size_t)b * N, N);
-        WTB_HIP_CHECK(hipMemcpyAsync(AS.p, T1.p, sizeof(cd) * N * nb, hipMemcpyDeviceToDevice, c.stream));
-        if (withP && m > b)
-            WTB_HIP_CHECK(hipMemcpyAsync(AP.p, T1.p + (size_t)b * N, sizeof(cd) * N * b, hipMemcpyDeviceToDevice, c.stream));
+        if (nb == b) swap_bufs(AS, T1);
+        else WTB_HIP_CHECK(hipMemcpyAsync(AS.p, T1.p, sizeof(cd) * N * nb, hipMemcpyDeviceToDevice, c.stream));
         trr.end();
     }
 };
@@ -933,8 +977,14 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             WTB_LAUNCH_CHECK();
             L.normalize(na, Pd, APd);
             for (int pass = 0; pass < 2; ++pass) {
-                L.project_out(b, L.S.p, L.AS.p, na, Pd, APd);
-                L.project_out(na, W, AW, na, Pd, APd);
+                if (orth) {
+                    // W = S[:, b:b+na] follows X with [X W] orthonormal: one
+                    // projection against both (P and H P read once per pass)
+                    L.project_out(b + na, L.S.p, L.AS.p, na, Pd, APd);
+                } else {
+                    L.project_out(b, L.S.p, L.AS.p, na, Pd, APd);
+                    L.project_out(na, W, AW, na, Pd, APd);
+                }
             }
             // H P is carried by linear combinations, not recomputed: a column
             // whose projection kept only a fraction f of its norm has its

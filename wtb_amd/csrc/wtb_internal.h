@@ -6,6 +6,7 @@
 
 #include <cstdio>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/wtb_hip.h"
@@ -94,6 +95,13 @@ struct HBuf {
 
 // Block eigensolver workspace, kept across solves (the Krylov block alone is
 // 6 N b complex values; allocating it per solve costs ~0.3 ms of hipMalloc).
+// exchange two buffers (the solver's basis and its scratch trade places)
+template <class T>
+inline void swap_bufs(DBuf<T> &a, DBuf<T> &b) {
+    std::swap(a.p, b.p);
+    std::swap(a.n, b.n);
+}
+
 struct SolverWs {
     DBuf<cd> S, AS, T1, P, AP;      // N x cols
     DBuf<cd> G, A, U, T, Tmp, Cm;   // small (3b) x (3b)
@@ -122,6 +130,8 @@ struct Bands {
     DBuf<double> esel;   // [k][j] selected bands
     DBuf<cd> usel;       // [(o*nb + j)*nk + k]  (k fastest, as the reference's vector(k,j,o))
     DBuf<cf> usel32;     // FP32 copy for the mixed-precision assembly
+    DBuf<double> zrot;   // [k][o][j] real QL rotations of the batched eigensolver
+    DBuf<double> tri;    // [k][3][32] diagonal, off-diagonal and eigenvalues of T(k)
     DBuf<int> status;    // device error flag
     bool valid = false;
 };
