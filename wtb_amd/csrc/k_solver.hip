@@ -220,6 +220,23 @@ __global__ void set_units(long N, int b, const int *__restrict__ idx, cd *__rest
     X[i + (size_t)j * N] = make_double2(i == idx[j] ? 1.0 : 0.0, 0.0);
 }
 
+// H X for the unit block X = [e_idx[0] ... e_idx[b-1]]: column j of H is
+// stored column j (the store is exactly Hermitian), so AX[:, c] is stored
+// column idx[c] converted to FP64, plus the FP64 diagonal at row idx[c] when
+// the store keeps a zero diagonal (diag != null).  Exact: no product rounding.
+template <class T>
+__global__ void unit_images(long N, long ld, int b, const int *__restrict__ idx, const T *__restrict__ H,
+                            const double *__restrict__ diag, cd *__restrict__ AX) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const int c = blockIdx.y;
+    if (i >= N || c >= b) return;
+    const long j = idx[c];
+    const T *h = H + 2 * (i + (size_t)j * ld);
+    cd v = make_double2((double)h[0], (double)h[1]);
+    if (diag && i == j) v.x += diag[i];
+    AX[i + (size_t)c * N] = v;
+}
+
 // norm2[j] += sum_i |AX_ij - lam_j X_ij|^2   (lam may be null: plain column norms of AX)
 // Column norms^2 of AX - lam X (or of AX when lam is null), one workgroup per
 // column (a fixed reduction order, no atomics).
@@ -905,8 +922,14 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     WTB_LAUNCH_CHECK();
     const int mv_prec = precision;   // FP64 / split-FP16 (mixed) / FP32 products
     long mvcols = 0;
-    L.apply(mv_prec, b, L.S.p, L.AS.p);
-    mvcols += b;
+    // the image of the unit starting block is read off the store (its columns)
+    if (c.bse.precision == WTB_FP64)
+        hipLaunchKernelGGL(unit_images<double>, L.gN(b), dim3(256), 0, c.stream, N, c.bse.ld, b, L.idx.p,
+                           (const double *)c.bse.h64.p, (const double *)nullptr, L.AS.p);
+    else
+        hipLaunchKernelGGL(unit_images<float>, L.gN(b), dim3(256), 0, c.stream, N, c.bse.ld, b, L.idx.p,
+                           (const float *)c.bse.h32.p, (const double *)c.bse.diag.p, L.AS.p);
+    WTB_LAUNCH_CHECK();
     L.rayleigh_ritz(b, false);
     // FP32 products cannot resolve residuals much below ~1e-6 relative.  The
     // reduced-precision iteration stops at 5e-6: a Ritz value with relative
