@@ -53,6 +53,23 @@ __device__ __forceinline__ double block_reduce(double v, double *red) {
 // its scalar, d/e the real tridiagonal matrix T = Q^H A Q.
 // The whole matrix is staged in LDS (n <= kTridLds); NT threads (256 keeps
 // the nine barriers of a step cheap at these sizes).
+// Block sum with one barrier: alternate calls use the two halves of red, and
+// every thread has read a half before any thread passes the next call's
+// barrier, so the half is free again two calls later.
+template <int NT = kT>
+__device__ __forceinline__ double block_reduce_db(double v, double *red, int &half) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double *r = red + half * (NT / 64);
+    if (lane == 0) r[w] = v;
+    __syncthreads();
+    double s = 0.0;
+#pragma unroll
+    for (int i = 0; i < NT / 64; ++i) s += r[i];
+    half ^= 1;
+    return s;
+}
+
 template <int NT>
 __global__ void __launch_bounds__(NT)
 tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double *__restrict__ d,
@@ -62,8 +79,9 @@ tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double
     cd *y = v + n;                            // n
     cd *part = y + n;                         // NT partial sums
     cd *M = part + NT;                        // n x n
-    __shared__ double red[NT / 64];
+    __shared__ double red[2 * (NT / 64)];
     __shared__ cd s_tau;
+    int half = 0;
     const int tid = threadIdx.x;
     cd *A = M;
     const int lda = n;
@@ -77,7 +95,7 @@ tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double
             cd a = A[r + (size_t)i * lda];
             xn2 += a.x * a.x + a.y * a.y;
         }
-        xn2 = block_reduce<NT>(xn2, red);
+        xn2 = block_reduce_db<NT>(xn2, red, half);
         const cd alpha = A[(i + 1) + (size_t)i * lda];
         d[i] = A[i + (size_t)i * lda].x;
         if (xn2 == 0.0 && alpha.y == 0.0) {
@@ -141,7 +159,7 @@ tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double
         // mu = Re(v^H y), w = tau y - |tau|^2 mu / 2 v
         double mu = 0.0;
         for (int r = i + 1 + tid; r < n; r += NT) mu += v[r].x * y[r].x + v[r].y * y[r].y;
-        mu = block_reduce<NT>(mu, red);
+        mu = block_reduce_db<NT>(mu, red, half);
         const double t2 = t.x * t.x + t.y * t.y;
         for (int r = i + 1 + tid; r < n; r += NT) {
             cd yr = y[r], vr = v[r];
@@ -194,17 +212,18 @@ tridiag_blocked_kernel(int n, int nb, cd *__restrict__ A, int lda, cd *__restric
     cd *W = V + (size_t)nb * n;
     cd *y = W + (size_t)nb * n;
     cd *buf = y + n;
-    __shared__ double red[kT / 64];
+    __shared__ double red[2 * (kT / 64)];
     __shared__ cd s_c[2 * kMaxNb];      // W_j^H v, then V_j^H v
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     constexpr int kW = kT / 64;
-    int p0 = 0;
+    int p0 = 0, half = 0;
     for (int i = 0; i < n - 1; ++i) {
         const int ip = i - p0, t0 = i + 1;
         // column i of the current matrix (rows i..n-1) into y; |.|^2 below the subdiagonal
         double xn2 = 0.0;
         for (int r = i + tid; r < n; r += kT) {
             cd a = A[r + (size_t)i * lda];
+#pragma unroll 4
             for (int j = 0; j < ip; ++j) {
                 const cd vr = V[j * n + r], wr = W[j * n + r], vi = V[j * n + i], wi = W[j * n + i];
                 // a -= v_r conj(w_i) + w_r conj(v_i)
@@ -214,7 +233,7 @@ tridiag_blocked_kernel(int n, int nb, cd *__restrict__ A, int lda, cd *__restric
             y[r] = a;
             if (r >= i + 2) xn2 += a.x * a.x + a.y * a.y;
         }
-        xn2 = block_reduce(xn2, red);
+        xn2 = block_reduce_db(xn2, red, half);
         const cd alpha = y[t0];
         if (tid == 0) d[i] = y[i].x;
         cd *vp = V + (size_t)ip * n, *wp = W + (size_t)ip * n;
@@ -300,6 +319,7 @@ tridiag_blocked_kernel(int n, int nb, cd *__restrict__ A, int lda, cd *__restric
                     s.x += b.x;
                     s.y += b.y;
                 }
+#pragma unroll 4
                 for (int j = 0; j < ip; ++j) {
                     const cd vr = V[j * n + r], wr = W[j * n + r], cw = s_c[j], cv = s_c[kMaxNb + j];
                     s.x -= (vr.x * cw.x - vr.y * cw.y) + (wr.x * cv.x - wr.y * cv.y);
@@ -309,7 +329,7 @@ tridiag_blocked_kernel(int n, int nb, cd *__restrict__ A, int lda, cd *__restric
                 const cd vr = vp[r];
                 mu += vr.x * s.x + vr.y * s.y;
             }
-            mu = block_reduce(mu, red);
+            mu = block_reduce_db(mu, red, half);
             // w = tau y - |tau|^2 mu / 2 v (same thread per row as above)
             const double t2 = t.x * t.x + t.y * t.y;
             for (int r = t0 + tid; r < n; r += kT) {
@@ -694,6 +714,92 @@ backtransform_kernel(int n, int nvec, const cd *__restrict__ A, int lda, const c
     }
 }
 
+// The same product with the column in registers (lane l holds rows l + 64 s,
+// s < S, n <= 64 S) and the reflectors staged in LDS kBTChunk at a time,
+// shared by the four column wavefronts of the workgroup: the chain of n - 1
+// reflectors then waits on global memory once per chunk, not twice per
+// reflector.
+constexpr int kBTChunk = 16;
+template <int S>
+__global__ void __launch_bounds__(kBT)
+backtransform_reg_kernel(int n, int nvec, const cd *__restrict__ A, int lda, const cd *__restrict__ tau,
+                         const double *__restrict__ Y, cd *__restrict__ Z, int ldz) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cd *vs = reinterpret_cast<cd *>(smem);      // [kBTChunk][n]: v of reflector i, zero above row i + 1
+    __shared__ cd ts[kBTChunk];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int col = blockIdx.x * (kBT / 64) + (tid >> 6);
+    const bool active = col < nvec;             // every thread stages; inactive ones compute nothing
+    cd z[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+        const int r = lane + 64 * s;
+        z[s] = make_double2((active && r < n) ? Y[(size_t)col * n + r] : 0.0, 0.0);
+    }
+    for (int i1 = n - 2; i1 >= 0; i1 -= kBTChunk) {
+        const int cnt = min(kBTChunk, i1 + 1);     // reflectors i1, i1 - 1, ..., i1 - cnt + 1
+        __syncthreads();                           // the previous chunk is consumed
+        // reflector q of the chunk (i = i1 - q), rows r = tid + kBT u (S rows per
+        // 64 lanes: n <= 64 S, so u < S / 4 + 1)
+        for (int q = 0; q < cnt; ++q) {
+            const int i = i1 - q;
+            const cd *ai = A + (size_t)i * lda;
+            cd v[(S + 3) / 4];
+#pragma unroll
+            for (int u = 0; u < (S + 3) / 4; ++u) {
+                const int r = tid + kBT * u;
+                v[u] = make_double2(0.0, 0.0);
+                if (r == i + 1) v[u] = make_double2(1.0, 0.0);
+                else if (r > i + 1 && r < n) v[u] = ai[r];
+            }
+#pragma unroll
+            for (int u = 0; u < (S + 3) / 4; ++u) {
+                const int r = tid + kBT * u;
+                if (r < n) vs[(size_t)q * n + r] = v[u];
+            }
+        }
+        if (tid < cnt) ts[tid] = tau[i1 - tid];
+        __syncthreads();
+        if (!active) continue;
+        for (int q = 0; q < cnt; ++q) {
+            const cd t = ts[q];
+            if (t.x == 0.0 && t.y == 0.0) continue;
+            const cd *v = vs + (size_t)q * n;
+            // w = v^H z
+            double wr = 0.0, wi = 0.0;
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int r = lane + 64 * s;
+                if (r < n) {
+                    const cd vr = v[r];
+                    wr += vr.x * z[s].x + vr.y * z[s].y;
+                    wi += vr.x * z[s].y - vr.y * z[s].x;
+                }
+            }
+            wr = wave_sum_dpp(wr);
+            wi = wave_sum_dpp(wi);
+            // z -= tau v w
+            const cd tw = make_double2(t.x * wr - t.y * wi, t.x * wi + t.y * wr);
+#pragma unroll
+            for (int s = 0; s < S; ++s) {
+                const int r = lane + 64 * s;
+                if (r < n) {
+                    const cd vr = v[r];
+                    z[s].x -= vr.x * tw.x - vr.y * tw.y;
+                    z[s].y -= vr.x * tw.y + vr.y * tw.x;
+                }
+            }
+        }
+    }
+    if (active) {
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int r = lane + 64 * s;
+            if (r < n) Z[(size_t)col * ldz + r] = z[s];
+        }
+    }
+}
+
 // Upper Cholesky factor of a Hermitian positive definite G (n x n, ldg), in
 // place in the upper triangle: G = R^H R.  A pivot below rel * (original
 // diagonal) stops the factorisation and sets *flag = column + 1.
@@ -863,8 +969,20 @@ void heev_lowest(Ctx &c, int n, int nvec, cd *A, int lda, double *W, cd *Z, int 
     WTB_LAUNCH_CHECK();
     hipLaunchKernelGGL(cluster_gs_kernel, dim3(1), dim3(kT), 0, c.stream, n, nvec, d, e, W, Y);
     WTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(backtransform_kernel, dim3((nvec + 3) / 4), dim3(kBT), 0, c.stream, n, nvec, A, lda, tau,
-                       Y, Z, ldz);
+    const dim3 gbt((nvec + kBT / 64 - 1) / (kBT / 64));
+    const size_t shbt = (size_t)kBTChunk * n * sizeof(cd);
+    switch ((n + 63) / 64) {
+#define WTB_BT_CASE(K)                                                                                          \
+    case K:                                                                                                     \
+        hipLaunchKernelGGL(backtransform_reg_kernel<K>, gbt, dim3(kBT), shbt, c.stream, n, nvec, A, lda, tau, Y, \
+                           Z, ldz);                                                                             \
+        break;
+        WTB_BT_CASE(1) WTB_BT_CASE(2) WTB_BT_CASE(3) WTB_BT_CASE(4)
+        WTB_BT_CASE(5) WTB_BT_CASE(6) WTB_BT_CASE(7) WTB_BT_CASE(8)
+#undef WTB_BT_CASE
+    default:
+        hipLaunchKernelGGL(backtransform_kernel, gbt, dim3(kBT), 0, c.stream, n, nvec, A, lda, tau, Y, Z, ldz);
+    }
     WTB_LAUNCH_CHECK();
 }
 

@@ -930,7 +930,14 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
         hipLaunchKernelGGL(unit_images<float>, L.gN(b), dim3(256), 0, c.stream, N, c.bse.ld, b, L.idx.p,
                            (const float *)c.bse.h32.p, (const double *)c.bse.diag.p, L.AS.p);
     WTB_LAUNCH_CHECK();
-    L.rayleigh_ritz(b, false);
+    // The unit block starts as it is: its Rayleigh quotients are the diagonal
+    // entries (ascending), and the first Rayleigh-Ritz step on [X W] mixes it.
+    {
+        std::vector<double> lam0(b);
+        for (int j = 0; j < b; ++j) lam0[j] = d[order[j]];
+        std::copy(lam0.begin(), lam0.end(), c.lws.hd.p);
+        WTB_HIP_CHECK(hipMemcpyAsync(L.lam.p, c.lws.hd.p, sizeof(double) * b, hipMemcpyHostToDevice, c.stream));
+    }
     // FP32 products cannot resolve residuals much below ~1e-6 relative.  The
     // reduced-precision iteration stops at 5e-6: a Ritz value with relative
     // residual r is within (r |l|)^2 / gap of an eigenvalue (about 2e-10 eV

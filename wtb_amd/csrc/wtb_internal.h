@@ -246,6 +246,36 @@ struct Ctx {
 // float32 rounding midpoint, so the sign of the double rounding error decides
 // the float32 phase of the compat mode: every evaluation must use the same
 // plain IEEE operations.
+// value of lane i (uniform i) of a per-lane double
+__device__ __forceinline__ double wave_lane(double v, int i) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)b, i);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(b >> 32), i);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// v moved across lanes by the DPP pattern CTRL (both 32-bit halves)
+template <int CTRL>
+__device__ __forceinline__ double dpp_double(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over the 64 lanes, returned in every lane, without LDS round trips:
+// DPP quad_perm [1,0,3,2] and [2,3,0,1] give each lane its quad's sum,
+// row_half_mirror (lane i <- 7 - i) the sum over 8 and row_mirror (i <- 15 - i)
+// over 16; the four row sums are then read with readlane and added in a
+// fixed order, so every lane holds the same value.
+__device__ __forceinline__ double wave_sum_dpp(double v) {
+    v += dpp_double<0xB1>(v);
+    v += dpp_double<0x4E>(v);
+    v += dpp_double<0x141>(v);
+    v += dpp_double<0x140>(v);
+    return (wave_lane(v, 0) + wave_lane(v, 16)) + (wave_lane(v, 32) + wave_lane(v, 48));
+}
+
 __device__ __forceinline__ double kdotr(double kx, double ky, double kz, const double *R) {
 #pragma clang fp contract(off)
     const double a = kx * R[0];
