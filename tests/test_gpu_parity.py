@@ -58,6 +58,27 @@ def test_hk_eig(case):
     ctx.close()
 
 
+@pytest.mark.parametrize("cap", [None, "1", "200"])
+def test_hk_eig_rotation_log(cap, monkeypatch):
+    """The QL step runs on one lane per k and logs its rotations; a log that
+    would overflow sends the whole batch through the in-place path.  cap = 1
+    forces that path at every k, cap = 200 at some k only (hex22: about n^2 =
+    484 rotations per k); both must give the bands of the default path, to
+    rounding, and match the oracle."""
+    say("case hk rotation log", cap)
+    m, mesh = hex_model((6, 6, 1)), (6, 6, 1)
+    if cap is not None:
+        monkeypatch.setenv("WTB_QL_LOG_CAP", cap)
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, 2, 2)
+    e_all, v_all = ctx.get_bands()
+    assert np.abs(e_all - ref["eall"]).max() < 1e-10
+    zg = np.transpose(v_all, (0, 2, 1))
+    zr = np.transpose(ref["zall"], (0, 2, 1))
+    assert same_up_to_sign(zg, zr, 1e-8)
+    assert np.array_equal(nocp, ref["nocp"])
+    ctx.close()
+
+
 def test_hk_eig_errors():
     say("case hk errors")
     m = hex_model((4, 4, 1))

@@ -19,6 +19,9 @@
 //   4. nocp (first band above 0), scissor, band selection.
 #include "wtb_internal.h"
 
+#include <algorithm>
+#include <cstdlib>
+
 namespace wtb {
 namespace {
 
@@ -259,6 +262,136 @@ hk_ql_kernel(int n, double *__restrict__ tri, double *__restrict__ zout, int *__
     for (int idx = tid; idx < nn; idx += kLanes) zout[(size_t)k * nn + idx] = Z[idx];
 }
 
+// 3 (one lane per k): the same tql2 recurrence on T(k) with d and e in LDS
+// columns ([j][lane]), so a wavefront advances 64 k-points at once instead of
+// issuing one k's scalar chain on all its lanes.  Each rotation (i, c, s) is
+// appended to the k's log (rot, rix: cap entries per k, k-major), which
+// hk_rot_kernel replays on Z row-parallel; the eigenvalues (unsorted) go to
+// tri[k] at 2 kMaxOrb.  A k whose log would overflow sets status bit 8 and
+// the batch is redone by hk_ql_kernel.
+constexpr int kQLLanes = 64;
+__global__ void __launch_bounds__(kQLLanes)
+hk_ql_lane_kernel(int nk, int n, long cap, double *__restrict__ tri, double2 *__restrict__ rot,
+                  unsigned char *__restrict__ rix, int *__restrict__ nrot, int *__restrict__ status) {
+    __shared__ double D[kMaxOrb][kQLLanes], E[kMaxOrb][kQLLanes];
+    const int t = threadIdx.x;
+    const int k = blockIdx.x * kQLLanes + t;
+    if (k >= nk) return;                 // lanes are independent: no barrier below
+    double *dg = tri + (size_t)k * 3 * kMaxOrb;
+    for (int j = 0; j < n; ++j) {
+        D[j][t] = dg[j];
+        E[j][t] = dg[kMaxOrb + j];       // e[n-1] = 0
+    }
+    double2 *lg = rot + (size_t)k * cap;
+    unsigned char *li = rix + (size_t)k * cap;
+    long cnt = 0;
+    int bad = 0;
+    const double eps = 0x1p-52;
+    for (int l = 0; l < n && !bad; ++l) {
+        for (int iter = 0;; ++iter) {
+            // first m >= l with a negligible e_m (m = n - 1 at the end)
+            int m = l;
+            while (m < n - 1 && !(fabs(E[m][t]) <= eps * (fabs(D[m][t]) + fabs(D[m + 1][t])))) ++m;
+            if (m == l) break;
+            if (iter == 60) {
+                bad = 4;
+                break;
+            }
+            if (cnt + (m - l) > cap) {
+                bad = 8;
+                break;
+            }
+            const double dl = D[l][t], el = E[l][t];
+            double g = (D[l + 1][t] - dl) / (2.0 * el);
+            double r = sqrt(g * g + 1.0);
+            g = D[m][t] - dl + el / (g + (g >= 0.0 ? r : -r));
+            double sn = 1.0, cs = 1.0, p = 0.0;
+            bool deflated = false;
+            for (int i = m - 1; i >= l; --i) {
+                const double ei = E[i][t];
+                const double f = sn * ei, b = cs * ei;
+                r = sqrt(f * f + g * g);
+                E[i + 1][t] = r;
+                if (r == 0.0) {
+                    D[i + 1][t] -= p;
+                    E[m][t] = 0.0;
+                    deflated = true;
+                    break;
+                }
+                sn = f / r;
+                cs = g / r;
+                g = D[i + 1][t] - p;
+                r = (D[i][t] - g) * sn + 2.0 * cs * b;
+                p = sn * r;
+                D[i + 1][t] = g + p;
+                g = cs * r - b;
+                lg[cnt] = make_double2(cs, sn);
+                li[cnt] = (unsigned char)i;
+                ++cnt;
+            }
+            if (deflated) continue;
+            D[l][t] = dl - p;
+            E[l][t] = g;
+            E[m][t] = 0.0;
+        }
+    }
+    nrot[k] = (int)cnt;
+    for (int j = 0; j < n; ++j) dg[2 * kMaxOrb + j] = D[j][t];
+    if (bad) atomicOr(status, bad);
+}
+
+// 3b: Z of every k from its rotation log (one wavefront per k, lane o rotates
+// row o in LDS, the log read 64 entries at a time and each entry broadcast
+// with readlane), the largest |Z| entry of each column made positive, then
+// Z -> zout[k].  Few registers, so many wavefronts per SIMD share the latency
+// of the sequential rotations.
+__global__ void __launch_bounds__(kLanes)
+hk_rot_kernel(int n, long cap, const double2 *__restrict__ rot, const unsigned char *__restrict__ rix,
+              const int *__restrict__ nrot, double *__restrict__ zout) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int k = blockIdx.x;
+    const int tid = threadIdx.x;
+    const int nn = n * n;
+    double *Z = reinterpret_cast<double *>(smem);     // [o*n + j]
+    if (tid < n)
+        for (int j = 0; j < n; ++j) Z[tid * n + j] = j == tid ? 1.0 : 0.0;
+    const int cnt = nrot[k];
+    const double2 *lg = rot + (size_t)k * cap;
+    const unsigned char *li = rix + (size_t)k * cap;
+    for (int j0 = 0; j0 < cnt; j0 += kLanes) {
+        const int jl = j0 + tid;
+        const double2 e = jl < cnt ? lg[jl] : make_double2(1.0, 0.0);
+        const int il = jl < cnt ? (int)li[jl] : 0;
+        const int jn = min(kLanes, cnt - j0);
+        for (int jj = 0; jj < jn; ++jj) {
+            const double cs = lane_get(e.x, jj), sn = lane_get(e.y, jj);
+            const int i = __builtin_amdgcn_readlane(il, jj);
+            if (tid < n) {
+                const double zi = Z[tid * n + i], zf = Z[tid * n + i + 1];
+                Z[tid * n + i + 1] = sn * zi + cs * zf;
+                Z[tid * n + i] = cs * zi - sn * zf;
+            }
+        }
+    }
+    __syncthreads();
+    // sign: the largest |Z| entry of each column (Z = Q^H z) positive, the
+    // first one on ties
+    if (tid < n) {
+        double best = -1.0, sg = 1.0;
+        for (int r = 0; r < n; ++r) {
+            const double z = Z[r * n + tid];
+            if (fabs(z) > best) {
+                best = fabs(z);
+                sg = z < 0.0 ? -1.0 : 1.0;
+            }
+        }
+        if (sg < 0.0)
+            for (int r = 0; r < n; ++r) Z[r * n + tid] = -Z[r * n + tid];
+    }
+    __syncthreads();
+    for (int idx = tid; idx < nn; idx += kLanes) zout[(size_t)k * nn + idx] = Z[idx];
+}
+
 // 4: eigenvectors V = Q Z (Q read back from uall[k], where hk_house_kernel
 // left it), ascending order, nocp, scissor and the outputs
 __global__ void __launch_bounds__(kLanes)
@@ -274,11 +407,9 @@ hk_vec_kernel(int nk, int n, int nv, int nc, double scissor, const double *__res
     double *Z = reinterpret_cast<double *>(V + nn);           // [o*n + j]
     double *wv = Z + nn;                                      // 32: eigenvalues
     int *perm = reinterpret_cast<int *>(wv + kMaxOrb);        // 32
-    for (int idx = tid; idx < nn; idx += kLanes) {
-        V[idx] = uall[(size_t)k * nn + idx];
-        Z[idx] = zin[(size_t)k * nn + idx];
-    }
+    for (int idx = tid; idx < nn; idx += kLanes) V[idx] = uall[(size_t)k * nn + idx];
     if (tid < n) wv[tid] = tri[(size_t)k * 3 * kMaxOrb + 2 * kMaxOrb + tid];
+    for (int idx = tid; idx < nn; idx += kLanes) Z[idx] = zin[(size_t)k * nn + idx];
     __syncthreads();
     // eigenvectors V = Q Z, lane o forms row o
     if (tid < n) {
@@ -362,26 +493,53 @@ void launch_hk_eig(Ctx &c, int nv, int nc) {
     b.usel32.alloc((size_t)nk * nb * n);
     b.status.alloc(1);
     WTB_HIP_CHECK(hipMemsetAsync(b.status.p, 0, sizeof(int), c.stream));
-    b.zrot.alloc((size_t)nk * n * n);
     b.tri.alloc((size_t)nk * 3 * kMaxOrb);
+    // rotation log: 4 n^2 entries per k (tql2 typically takes about n^2);
+    // WTB_QL_LOG_CAP lowers it (tests force the overflow path with it)
+    long cap = 4L * n * n;
+    if (const char *ce = std::getenv("WTB_QL_LOG_CAP")) cap = std::max(1L, std::min(cap, std::atol(ce)));
+    b.rot.alloc((size_t)nk * cap);
+    b.rix.alloc((size_t)nk * cap);
+    b.nrot.alloc(nk);
+    b.zrot.alloc((size_t)nk * n * n);
     const size_t sh_house = 2 * (size_t)n * n * sizeof(cd) + (kLanes + 2 * kMaxOrb) * sizeof(cd);
-    hipLaunchKernelGGL(hk_house_kernel, dim3(nk), dim3(kLanes), sh_house, c.stream, n, m.nvec, m.flags, m.efermi,
-                       c.kpt.p, m.hop.p, m.rcart.p, m.invdeg.p, b.uall.p, b.tri.p);
-    WTB_LAUNCH_CHECK();
-    hipLaunchKernelGGL(hk_ql_kernel, dim3(nk), dim3(kLanes), (size_t)n * n * sizeof(double), c.stream, n, b.tri.p,
-                       b.zrot.p, b.status.p);
-    WTB_LAUNCH_CHECK();
     const size_t sh_vec = (size_t)n * n * (sizeof(cd) + sizeof(double)) + kMaxOrb * (sizeof(double) + sizeof(int));
-    hipLaunchKernelGGL(hk_vec_kernel, dim3(nk), dim3(kLanes), sh_vec, c.stream, nk, n, nv, nc, m.scissor, b.tri.p,
-                       b.zrot.p, b.eall.p, b.uall.p, b.nocp.p, b.esel.p, b.usel.p, b.status.p);
-    WTB_LAUNCH_CHECK();
-    size_t tot = (size_t)nk * nb * n;
-    hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream,
-                       b.usel.p, b.usel32.p, tot);
-    WTB_LAUNCH_CHECK();
-    int st = 0;
-    WTB_HIP_CHECK(hipMemcpyAsync(&st, b.status.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+    // lane_ql: the QL on one lane per k with a rotation log replayed by
+    // hk_rot_kernel; otherwise one wavefront per k rotates Z in place
+    auto run = [&](bool lane_ql) {
+        hipLaunchKernelGGL(hk_house_kernel, dim3(nk), dim3(kLanes), sh_house, c.stream, n, m.nvec, m.flags,
+                           m.efermi, c.kpt.p, m.hop.p, m.rcart.p, m.invdeg.p, b.uall.p, b.tri.p);
+        WTB_LAUNCH_CHECK();
+        if (lane_ql) {
+            hipLaunchKernelGGL(hk_ql_lane_kernel, dim3((unsigned)((nk + kQLLanes - 1) / kQLLanes)), dim3(kQLLanes), 0,
+                               c.stream, nk, n, cap, b.tri.p, b.rot.p, b.rix.p, b.nrot.p, b.status.p);
+            WTB_LAUNCH_CHECK();
+            hipLaunchKernelGGL(hk_rot_kernel, dim3(nk), dim3(kLanes), (size_t)n * n * sizeof(double), c.stream, n, cap,
+                               b.rot.p, b.rix.p, b.nrot.p, b.zrot.p);
+        } else {
+            hipLaunchKernelGGL(hk_ql_kernel, dim3(nk), dim3(kLanes), (size_t)n * n * sizeof(double), c.stream, n,
+                               b.tri.p, b.zrot.p, b.status.p);
+        }
+        WTB_LAUNCH_CHECK();
+        hipLaunchKernelGGL(hk_vec_kernel, dim3(nk), dim3(kLanes), sh_vec, c.stream, nk, n, nv, nc, m.scissor, b.tri.p,
+                           b.zrot.p, b.eall.p, b.uall.p, b.nocp.p, b.esel.p, b.usel.p, b.status.p);
+        WTB_LAUNCH_CHECK();
+        const size_t tot = (size_t)nk * nb * n;
+        hipLaunchKernelGGL(to_f32_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, c.stream, b.usel.p,
+                           b.usel32.p, tot);
+        WTB_LAUNCH_CHECK();
+        int st = 0;
+        WTB_HIP_CHECK(hipMemcpyAsync(&st, b.status.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+        WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+        return st;
+    };
+    int st = run(true);
+    if (st & 8) {
+        // a rotation log would have overflowed (more than 4 n^2 QL rotations
+        // at some k): the batch again with the in-place rotations
+        WTB_HIP_CHECK(hipMemsetAsync(b.status.p, 0, sizeof(int), c.stream));
+        st = run(false);
+    }
     if (st & 1) throw Error{WTB_ERR_NOCP, "no eigenvalue above the Fermi level at some k (nocp undefined)"};
     if (st & 2) throw Error{WTB_ERR_NOCP, "Number of valence/conduction states higher than occupied/unoccupied states"};
     if (st & 4) throw Error{WTB_ERR_NOCONV, "H(k) eigensolver: QL iteration did not converge at some k"};

@@ -130,8 +130,11 @@ struct Bands {
     DBuf<double> esel;   // [k][j] selected bands
     DBuf<cd> usel;       // [(o*nb + j)*nk + k]  (k fastest, as the reference's vector(k,j,o))
     DBuf<cf> usel32;     // FP32 copy for the mixed-precision assembly
-    DBuf<double> zrot;   // [k][o][j] real QL rotations of the batched eigensolver
+    DBuf<double> zrot;   // [k][o][j] real QL rotations of the batched eigensolver (fallback path)
     DBuf<double> tri;    // [k][3][32] diagonal, off-diagonal and eigenvalues of T(k)
+    DBuf<double2> rot;   // [k][cap] QL rotation log (cos, sin) of the batched eigensolver
+    DBuf<unsigned char> rix;  // [k][cap] rotated column pair (i, i + 1) of each logged rotation
+    DBuf<int> nrot;      // [k] logged rotations
     DBuf<int> status;    // device error flag
     bool valid = false;
 };
