@@ -334,16 +334,40 @@ inline rocblas_double_complex *rb(cd *p) { return reinterpret_cast<rocblas_doubl
 // fixed order (no atomics, so the result does not depend on scheduling).
 typedef double d4 __attribute__((ext_vector_type(4)));
 
-template <int MT, int NT>
+// HERM (m == n, square blocks): only the upper triangle of C is wanted.
+// Blocks below the diagonal return at once and a diagonal block forms only
+// its tiles with ti <= tj (15 of 25 at 80 x 80), dealt to the wavefronts in
+// that order; reduce_chunks then mirrors the upper triangle.
+template <int MT, int NT, bool HERM>
 __global__ void __launch_bounds__(256)
 gram_block_kernel(int m, int n, long K, long kc, const cd *__restrict__ A, int lda, const cd *__restrict__ B,
                   int ldb, cd *__restrict__ part) {
     constexpr int KS = 16, TI = MT / 16, TJ = NT / 16, NTILE = TI * TJ, TPW = (NTILE + 3) / 4;
     constexpr int AE = KS * MT / 256, BE = KS * NT / 256;       // staged elements per thread
     static_assert(KS * MT % 256 == 0 && KS * NT % 256 == 0, "block must split evenly over the workgroup");
+    static_assert(!HERM || MT == NT, "Hermitian blocks are square");
     __shared__ cd sa[KS][MT + 1], sb[KS][NT + 1];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    if (HERM && blockIdx.x > blockIdx.y) return;
     const int i0 = blockIdx.x * MT, j0 = blockIdx.y * NT;
+    const bool tri = HERM && blockIdx.x == blockIdx.y;
+    const int ntile = tri ? TI * (TI + 1) / 2 : NTILE;
+    // tile t of the wavefront -> (ti, tj): column-major over all tiles, or
+    // over the upper ones (ti <= tj) of a diagonal block
+    int tis[TPW], tjs[TPW];
+#pragma unroll
+    for (int t = 0; t < TPW; ++t) {
+        const int tile = w + 4 * t;
+        if (tri) {
+            int tj = 0;
+            while ((tj + 1) * (tj + 2) / 2 <= tile) ++tj;
+            tjs[t] = tj;
+            tis[t] = tile - tj * (tj + 1) / 2;
+        } else {
+            tis[t] = tile % TI;
+            tjs[t] = tile / TI;
+        }
+    }
     const long k0 = (long)blockIdx.z * kc, k1 = min(K, k0 + kc);
     d4 cre[TPW], cim[TPW];
 #pragma unroll
@@ -388,8 +412,8 @@ gram_block_kernel(int m, int n, long K, long kc, const cd *__restrict__ A, int l
 #pragma unroll
             for (int t = 0; t < TPW; ++t) {
                 const int tile = w + 4 * t;
-                if (tile < NTILE) {
-                    const int ti = tile % TI, tj = tile / TI;
+                if (tile < ntile) {
+                    const int ti = tis[t], tj = tjs[t];
                     const cd a = sa[k][16 * ti + (lane & 15)], b = sb[k][16 * tj + (lane & 15)];
                     // conj(a) b
                     cre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a.x, b.x, cre[t], 0, 0, 0);
@@ -404,8 +428,8 @@ gram_block_kernel(int m, int n, long K, long kc, const cd *__restrict__ A, int l
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
         const int tile = w + 4 * t;
-        if (tile >= NTILE) continue;
-        const int ti = tile % TI, tj = tile / TI;
+        if (tile >= ntile) continue;
+        const int ti = tis[t], tj = tjs[t];
         const int jj = j0 + 16 * tj + (lane & 15);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -421,7 +445,7 @@ gram_block_kernel(int m, int n, long K, long kc, const cd *__restrict__ A, int l
 // enough loads in flight to stream the partials near the HBM rate.
 constexpr int kRedG = 16;
 __global__ void __launch_bounds__(64 * kRedG)
-reduce_chunks(int m, int n, int nch, const cd *__restrict__ part, cd *__restrict__ C, int ldc) {
+reduce_chunks(int m, int n, int nch, const cd *__restrict__ part, cd *__restrict__ C, int ldc, int herm) {
     __shared__ cd red[kRedG][64];
     const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
     const long mn = (long)m * n, e = (long)blockIdx.x * 64 + lane;
@@ -454,14 +478,27 @@ reduce_chunks(int m, int n, int nch, const cd *__restrict__ part, cd *__restrict
             r.x += red[q][lane].x;
             r.y += red[q][lane].y;
         }
-        C[e % m + (size_t)(e / m) * ldc] = r;
+        const long i = e % m, j = e / m;
+        if (!herm) {
+            C[i + (size_t)j * ldc] = r;
+        } else if (i <= j) {
+            // herm: the upper triangle (the only part formed) and its mirror
+            if (i == j) r.y = 0.0;
+            C[i + (size_t)j * ldc] = r;
+            C[j + (size_t)i * ldc] = make_double2(r.x, -r.y);
+        }
     }
 }
 
 // C (m x n) = A^H B for tall A (K x m) and B (K x n).  Output blocks of up to
 // 80 x 80; K is split so that about 256 workgroups run.
-void gram(Ctx &c, int m, int n, long K, const cd *A, int lda, const cd *B, int ldb, cd *C, int ldc) {
+// herm (m == n): the Hermitian part of A^H B from its upper triangle (exact
+// for B = A; for an approximately Hermitian A^H B it replaces the mean of C
+// and C^H by the upper triangle mirrored).
+void gram(Ctx &c, int m, int n, long K, const cd *A, int lda, const cd *B, int ldb, cd *C, int ldc,
+          bool herm = false) {
     if (m <= 0 || n <= 0) return;
+    if (herm && m != n) throw Error{WTB_ERR_STATE, "Hermitian Gram product of a non-square block"};
     const int bs = std::min(80, std::max((m + 15) / 16, (n + 15) / 16) * 16);
     const int bx = (m + bs - 1) / bs, by = (n + bs - 1) / bs;
     int nsp = std::max(1, 256 / (bx * by));
@@ -471,17 +508,21 @@ void gram(Ctx &c, int m, int n, long K, const cd *A, int lda, const cd *B, int l
     c.ws_gram.alloc((size_t)nsp * m * n);
     const dim3 g(bx, by, nsp);
     switch (bs) {
-#define WTB_GRAM_CASE(S)                                                                                     \
-    case S:                                                                                                  \
-        hipLaunchKernelGGL((gram_block_kernel<S, S>), g, dim3(256), 0, c.stream, m, n, K, kc, A, lda, B, ldb, \
-                           c.ws_gram.p);                                                                     \
+#define WTB_GRAM_CASE(S)                                                                                       \
+    case S:                                                                                                    \
+        if (herm)                                                                                              \
+            hipLaunchKernelGGL((gram_block_kernel<S, S, true>), g, dim3(256), 0, c.stream, m, n, K, kc, A, lda, B, \
+                               ldb, c.ws_gram.p);                                                              \
+        else                                                                                                   \
+            hipLaunchKernelGGL((gram_block_kernel<S, S, false>), g, dim3(256), 0, c.stream, m, n, K, kc, A, lda,   \
+                               B, ldb, c.ws_gram.p);                                                           \
         break;
         WTB_GRAM_CASE(16) WTB_GRAM_CASE(32) WTB_GRAM_CASE(48) WTB_GRAM_CASE(64) WTB_GRAM_CASE(80)
 #undef WTB_GRAM_CASE
     }
     WTB_LAUNCH_CHECK();
     hipLaunchKernelGGL(reduce_chunks, dim3((unsigned)(((long)m * n + 63) / 64)), dim3(64 * kRedG), 0, c.stream, m, n, nsp,
-                       c.ws_gram.p, C, ldc);
+                       c.ws_gram.p, C, ldc, herm ? 1 : 0);
     WTB_LAUNCH_CHECK();
 }
 
@@ -764,9 +805,7 @@ struct Lobpcg {
     bool cholqr(int nb, cd *B, cd *B2, double piv) {
         for (int pass = 0; pass < 2; ++pass) {
             trr.begin();
-            gram(c, nb, nb, N, B, (int)N, B, (int)N, Tmp.p, m3);
-            hipLaunchKernelGGL(hermitize, dim3((nb + 63) / 64, nb), dim3(64), 0, c.stream, nb, Tmp.p, m3);
-            WTB_LAUNCH_CHECK();
+            gram(c, nb, nb, N, B, (int)N, B, (int)N, Tmp.p, m3, true);
             // the second pass factors I + O(eps cond^2) and cannot fail once the
             // first has passed: only the first pass's pivot test is read back
             int fail = cholesky_upper(c, nb, Tmp.p, m3, piv, pass == 0);
@@ -791,15 +830,11 @@ struct Lobpcg {
         const int nb = nout > 0 ? nout : b;
         if (withP && nb != b) throw Error{WTB_ERR_STATE, "partial Rayleigh-Ritz with P"};
         trr.begin();
-        gram(c, m, m, N, S.p, (int)N, AS.p, (int)N, A.p, m3);
-        hipLaunchKernelGGL(hermitize, dim3((m + 63) / 64, m), dim3(64), 0, c.stream, m, A.p, m3);
-        WTB_LAUNCH_CHECK();
+        gram(c, m, m, N, S.p, (int)N, AS.p, (int)N, A.p, m3, true);
         if (orth) {
             heev_lowest(c, m, nb, A.p, m3, lamr.p, Cm.p, m3);
         } else {
-            gram(c, m, m, N, S.p, (int)N, S.p, (int)N, G.p, m3);
-            hipLaunchKernelGGL(hermitize, dim3((m + 63) / 64, m), dim3(64), 0, c.stream, m, G.p, m3);
-            WTB_LAUNCH_CHECK();
+            gram(c, m, m, N, S.p, (int)N, S.p, (int)N, G.p, m3, true);
             small_heev(c, m, G.p, m3, D.p, U.p, m3);
             std::vector<double> hd(m);
             WTB_HIP_CHECK(hipMemcpyAsync(hd.data(), D.p, sizeof(double) * m, hipMemcpyDeviceToHost, c.stream));
