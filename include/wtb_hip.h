@@ -137,6 +137,19 @@ int wtb_get_bands(wtb_ctx *ctx, double *energies, double *vectors);
  * convention: WTB_CONV_*.  The N x N matrix (N = nk*nc*nv) stays on the GPU. */
 int wtb_bse_build(wtb_ctx *ctx, const wtb_coulomb *coulomb, int precision, int convention);
 
+/* Temperature BSE (the BSET=T path: bsesolvertemp, module_bse.F90:2649-3263).
+ * temp_k >= 0 makes the following wtb_bse_build multiply every kernel
+ * element H(i,j), i <= j, by f(i) = f_v - f_c of state i (Fermi level 0;
+ * matrizelbsetemp :580-660 and fermidisteh, module_math.F90:410-421, which
+ * takes T = 0 as 1e-8 K) and H(j,i) = conj(H(i,j)); the diagonal becomes
+ * E_c - E_v + V(0) f(i).  wtb_oscillator then weights its second dipole sum
+ * by f (dielbseptemp :784-810).  temp_k < 0 (the default) switches it off.
+ * The gap correction of the TA= card (gapcortemp/gapcortemp2, module_math.F90:
+ * 383-404) is a scissor shift, passed through the model's scissor.  A store
+ * built at another temperature is dropped.  The gradient entry points return
+ * WTB_ERR_UNSUPPORTED on a temperature store. */
+int wtb_set_temperature(wtb_ctx *ctx, double temp_k);
+
 /* Lowest nev eigenpairs of the BSE matrix: replaces slepc_solver(HX,EX,HE,NX,NE)
  * (module_slepc_solver.F90:12-97) with nev explicit.  tol: relative residual
  * (SLEPc default 1e-8); max_it: outer iterations (200).

@@ -184,6 +184,42 @@ def bse_matrix(kpt, eig, vec, c, ngrid, rlat, nc, nv, textbook=False):
     return h
 
 
+def fermidisteh(ec, ev, temp):
+    """f_v - f_c with the Fermi level at 0 (fermidisteh, src/module_math.F90:410-421):
+    the Boltzmann constant is the single-precision literal 8.617330350E-5 and
+    T = 0 is replaced by the single-precision 1E-8."""
+    kb = float(np.float32(8.617330350e-5))
+    t = float(np.float32(1e-8)) if temp == 0.0 else float(temp)
+    beta = 1.0 / (kb * t)
+    with np.errstate(over="ignore"):
+        return 1.0 / (np.exp(np.asarray(ev) * beta) + 1.0) - 1.0 / (np.exp(np.asarray(ec) * beta) + 1.0)
+
+
+def state_factors(eig, nc, nv, temp):
+    """fdeh(i) of every BSE state i = k nc nv + c nv + v (bsesolvertemp,
+    src/module_bse.F90:2984-2988): f_v - f_c of its band pair."""
+    eig = np.asarray(eig, float)
+    nk = eig.shape[0]
+    ec = np.repeat(eig[:, nv:nv + nc], nv, axis=1).reshape(nk * nc * nv)
+    ev = np.tile(eig[:, :nv], (1, nc)).reshape(nk * nc * nv)
+    return fermidisteh(ec, ev, temp), ec - ev
+
+
+def bse_matrix_temp(kpt, eig, vec, c, ngrid, rlat, nc, nv, temp, textbook=False):
+    """Temperature BSE matrix (matrizelbsetemp, src/module_bse.F90:580-660, filled
+    for j >= i and read through its upper triangle by ZHEEV('U'),
+    :3057-3087): H(i,j) = f(i) K(i,j) for i < j with K the plain kernel of
+    bse_matrix, H(j,i) = conj(H(i,j)), and H(i,i) = (E_c - E_v) + V(0) f(i)."""
+    K = bse_matrix(kpt, eig, vec, c, ngrid, rlat, nc, nv, textbook=textbook)
+    f, de = state_factors(eig, nc, nv, temp)
+    n = K.shape[0]
+    idx = np.arange(n)
+    H = K * f[np.minimum.outer(idx, idx)]
+    v0 = np.diag(K).real - de
+    H[idx, idx] = de + v0 * f
+    return H
+
+
 def dipoles(model: Model, kpt, eig, vec, nc, nv, cshift):
     nk = kpt.shape[0]
     N = nk * nc * nv

@@ -244,6 +244,54 @@ def test_bse_energies_gauge_invariant_both_conventions(rng):
     assert np.abs(wr - wt).max() > 1e-6      # the two conventions are different physics
 
 
+# ------------------------- temperature BSE (BSET=T) -----------------------
+
+def test_fermidisteh_closed_forms():
+    kb = float(np.float32(8.617330350e-5))
+    e = np.array([0.05, 0.3, 0.9, 2.0])
+    for T in (300.0, 3000.0, 20000.0):
+        # symmetric pair (E_v = -E_c): f_v - f_c = tanh(E_c / (2 kB T))
+        assert np.allclose(O.fermidisteh(e, -e, T), np.tanh(e / (2 * kb * T)), rtol=0, atol=1e-15)
+    # T = 0 is taken as 1e-8 K: fully occupied valence, empty conduction
+    assert np.array_equal(O.fermidisteh(e, -e, 0.0), np.ones(4))
+    # both bands on the same side of the Fermi level: no transition weight at T -> 0
+    assert np.array_equal(O.fermidisteh(np.array([0.5]), np.array([0.2]), 0.0), [0.0])
+    # very high T: occupations tend to 1/2 and the factor to 0
+    assert np.abs(O.fermidisteh(e, -e, 1e9)).max() < 1e-4
+
+
+def test_bse_matrix_temp_fill_rule():
+    """The vectorised f[min(i, j)] rule against a loop restatement of the
+    reference's fill (matrizelbsetemp for j >= i with the factor of row i, the
+    lower triangle read as the conjugate)."""
+    m, k, r = _small_problem()
+    c = O.coulomb(**COUL)
+    T = 8000.0
+    K = O.bse_matrix(k, r["eig"], r["vec"], c, (6, 6, 1), m.rlat, 2, 2)
+    H = O.bse_matrix_temp(k, r["eig"], r["vec"], c, (6, 6, 1), m.rlat, 2, 2, T)
+    n = K.shape[0]
+    nk = r["eig"].shape[0]
+    L = np.zeros_like(K)
+    for i in range(n):
+        kk, cc, vv = i // 4, (i // 2) % 2, i % 2
+        ec, ev = r["eig"][kk, 2 + cc], r["eig"][kk, vv]
+        fi = float(O.fermidisteh(ec, ev, T))
+        for j in range(i, n):
+            if i == j:
+                L[i, i] = (ec - ev) + (K[i, i].real - (ec - ev)) * fi
+            else:
+                L[i, j] = K[i, j] * fi
+                L[j, i] = np.conj(L[i, j])
+    assert nk * 4 == n
+    assert np.abs(H - L).max() < 1e-15 * np.abs(K).max()
+    assert np.array_equal(H, H.conj().T)
+    f, _ = O.state_factors(r["eig"], 2, 2, T)
+    assert 0.3 < f.min() < 0.99           # a visible effect at this temperature
+    # T -> 0: every factor is 1 and the plain kernel comes back
+    H0 = O.bse_matrix_temp(k, r["eig"], r["vec"], c, (6, 6, 1), m.rlat, 2, 2, 0.0)
+    assert np.abs(H0 - K).max() < 1e-14 * np.abs(K).max()
+
+
 # ------------------------------ optics ------------------------------------
 
 def test_dhk_matches_finite_difference():

@@ -170,12 +170,17 @@ constexpr int pair_slots() { return TK * (TK + 1) + 1; }
 // bands; TK: k-points per side of the square tile (TK nc nv rows and columns).
 // Workgroup t covers the tile (bk, bkp) of the lower block triangle,
 // t = bk (bk + 1) / 2 + bkp with bkp <= bk.
+// fdeh (temperature BSE, else null): every element H(I, J) off the diagonal
+// is multiplied by fdeh[min(I, J)] and the diagonal V(0) term by fdeh[I]
+// (matrizelbsetemp fills j >= i with the factor of row i,
+// src/module_bse.F90:580-660,3062-3070); the mirror of an element carries the
+// same factor, so the store stays exactly Hermitian.
 template <class R, int NC, int NV, int TK>
 __global__ void __launch_bounds__(kThreads)
 bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
                  const double *__restrict__ kpt, const double *__restrict__ esel,
                  const typename CT<R>::C *__restrict__ usel, typename CT<R>::C *__restrict__ H,
-                 unsigned *__restrict__ hmax) {
+                 unsigned *__restrict__ hmax, const double *__restrict__ fdeh) {
     using C = typename CT<R>::C;
     constexpr int NB = NC + NV, NCV = NC * NV, NC2 = NC * NC, NV2 = NV * NV;
     constexpr int S = TK * TK / kThreads;              // pairs per thread
@@ -292,10 +297,15 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
                     } else {
                         const int k = k0 + kl;
                         double de = esel[(size_t)k * NB + NV + ci] - esel[(size_t)k * NB + vi];
-                        val = mk<C>(de + (double)vq, 0.0);
+                        val = mk<C>(fdeh ? de + (double)vq * fdeh[I] : de + (double)vq, 0.0);
                     }
                 } else {
                     val = pair_value(vq, a[cj], b[vj]);
+                    if (fdeh) {
+                        const R f = (R)fdeh[I < J ? I : J];
+                        val.x *= f;
+                        val.y *= f;
+                    }
                 }
                 H[I + J * ld] = val;
                 vmax = fmaxf(vmax, fmaxf(fabsf((float)val.x), fabsf((float)val.y)));
@@ -326,7 +336,12 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
             for (int cj = 0; cj < NC; ++cj)
 #pragma unroll
                 for (int vj = 0; vj < NV; ++vj) {
-                    const C val = pair_value(vq, a[cj], b[vj]);
+                    C val = pair_value(vq, a[cj], b[vj]);
+                    if (fdeh) {     // J < I off the diagonal tiles: the factor of J
+                        const R f = (R)fdeh[Jb + cj * NV + vj];
+                        val.x *= f;
+                        val.y *= f;
+                    }
                     col[cj * NV + vj] = C{val.x, -val.y};
                 }
         }
@@ -363,8 +378,9 @@ void launch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const typename CT<
     const long nblk = (nk + TK - 1) / TK;
     const long ntile = nblk * (nblk + 1) / 2;
     unsigned *hm = sizeof(R) == sizeof(float) ? c.bse.hmax.p : nullptr;
+    const double *fd = c.bse.temperature >= 0.0 ? c.bse.fdeh.p : nullptr;
     hipLaunchKernelGGL((bse_build_kernel<R, NC, NV, TK>), dim3((unsigned)ntile), dim3(kThreads), shmem, c.stream,
-                       nk, norb, ld, textbook, cd_, c.kpt.p, c.bands.esel.p, usel, H, hm);
+                       nk, norb, ld, textbook, cd_, c.kpt.p, c.bands.esel.p, usel, H, hm, fd);
     WTB_LAUNCH_CHECK();
     if (ld > N) {
         const long tot = (ld - N) * N;
@@ -389,13 +405,32 @@ void dispatch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const typename C
 }
 
 __global__ void bse_diag_kernel(int nk, int nc, int nv, CoulombDev coul, const double *__restrict__ esel,
-                                double *__restrict__ diag) {
+                                const double *__restrict__ fdeh, double *__restrict__ diag) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     long N = (long)nk * nc * nv;
     if (i >= N) return;
     int k = (int)(i / (nc * nv)), c = (int)((i / nv) % nc), v = (int)(i % nv);
     int nb = nc + nv;
-    diag[i] = esel[(size_t)k * nb + nv + c] - esel[(size_t)k * nb + v] + coulomb_v(coul, 0.0, 0.0, 0.0);
+    const double v0 = coulomb_v(coul, 0.0, 0.0, 0.0);
+    diag[i] = esel[(size_t)k * nb + nv + c] - esel[(size_t)k * nb + v] + (fdeh ? v0 * fdeh[i] : v0);
+}
+
+// f_v - f_c of every state at temperature temp (fermidisteh,
+// src/module_math.F90:410-421): Fermi level 0, the reference's single-precision
+// Boltzmann constant, and T = 0 taken as 1e-8 K (also a single-precision literal).
+__global__ void fdeh_kernel(int nk, int nc, int nv, double temp, const double *__restrict__ esel,
+                            double *__restrict__ fdeh) {
+    long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    long N = (long)nk * nc * nv;
+    if (i >= N) return;
+    int k = (int)(i / (nc * nv)), c = (int)((i / nv) % nc), v = (int)(i % nv);
+    int nb = nc + nv;
+    const double ec = esel[(size_t)k * nb + nv + c], ev = esel[(size_t)k * nb + v];
+    const double kb = (double)8.617330350e-5f;
+    const double t = temp == 0.0 ? (double)1e-8f : temp;
+    const double beta = 1.0 / (kb * t);
+    const double fv = 1.0 / (exp(ev * beta) + 1.0), fc = 1.0 / (exp(ec * beta) + 1.0);
+    fdeh[i] = fv - fc;
 }
 
 }  // namespace
@@ -423,10 +458,17 @@ void launch_bse_build(Ctx &c, const wtb_coulomb &coul, int precision, int conven
     s.diag.alloc(N);
     s.hmax.alloc(1);
     WTB_HIP_CHECK(hipMemsetAsync(s.hmax.p, 0, sizeof(unsigned), c.stream));
+    s.temperature = c.temperature;
+    if (s.temperature >= 0.0) {
+        s.fdeh.alloc(N);
+        hipLaunchKernelGGL(fdeh_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c.stream, nk, nc, nv,
+                           s.temperature, b.esel.p, s.fdeh.p);
+        WTB_LAUNCH_CHECK();
+    }
     if (f64) dispatch_tile<double>(c, cd_, convention == WTB_CONV_TEXTBOOK, b.usel.p, s.h64.p);
     else dispatch_tile<float>(c, cd_, convention == WTB_CONV_TEXTBOOK, b.usel32.p, s.h32.p);
     hipLaunchKernelGGL(bse_diag_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c.stream, nk, nc, nv,
-                       cd_, b.esel.p, s.diag.p);
+                       cd_, b.esel.p, s.temperature >= 0.0 ? (const double *)s.fdeh.p : nullptr, s.diag.p);
     WTB_LAUNCH_CHECK();
     if (!f64) {
         // power-of-two scale for the FP16 split: max |H| / hscale in [2^12, 2^13)

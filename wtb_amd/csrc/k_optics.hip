@@ -7,7 +7,9 @@
 // once for a = x, y, z in LDS and contracts it with all nc*nv band pairs:
 //   d_a(k,c,v) = <c|dH/dk_a|v> * Re(1/(E_c - E_v + i eta)).
 // Oscillator strengths (dielbsev/dielbsep, src/module_bse.F90:171-245):
-//   act(n) = Re[(sum_j X_jn d1_j) conj(sum_j X_jn d2_j)]   (no conjugate on X).
+//   act(n) = Re[(sum_j X_jn d1_j) conj(sum_j X_jn d2_j)]   (no conjugate on X);
+// a store built with a temperature weights the second sum by f_v - f_c of
+// state j (dielbseptemp, src/module_bse.F90:784-810).
 #include "wtb_internal.h"
 
 namespace wtb {
@@ -86,13 +88,17 @@ dipole_kernel(int nk, int n, int nvec, int nv, int nc, int compat, double eta, d
 
 __global__ void __launch_bounds__(256)
 oscillator_kernel(long N, const cd *__restrict__ X, const cd *__restrict__ d1, const cd *__restrict__ d2,
-                  double *__restrict__ act) {
+                  const double *__restrict__ fdeh, double *__restrict__ act) {
     __shared__ double red[4][4];
     const int n = blockIdx.x;
     const cd *x = X + (size_t)n * N;
     double s1r = 0, s1i = 0, s2r = 0, s2i = 0;
     for (long j = threadIdx.x; j < N; j += 256) {
         cd xv = x[j], a = d1[j], b = d2[j];
+        if (fdeh) {     // temperature BSE: the second sum carries f_v - f_c (dielbseptemp)
+            b.x *= fdeh[j];
+            b.y *= fdeh[j];
+        }
         s1r += xv.x * a.x - xv.y * a.y;
         s1i += xv.x * a.y + xv.y * a.x;
         s2r += xv.x * b.x - xv.y * b.y;
@@ -136,8 +142,9 @@ void launch_optics(Ctx &c, double cshift) {
 
 void launch_oscillator(Ctx &c, int nev, int a, int b, double *act_dev) {
     const long N = c.bse.N;
+    const double *f = c.bse.temperature >= 0.0 ? c.bse.fdeh.p : nullptr;
     hipLaunchKernelGGL(oscillator_kernel, dim3(nev), dim3(256), 0, c.stream, N, c.sol.evecs.p,
-                       c.dip.p + (size_t)a * N, c.dip.p + (size_t)b * N, act_dev);
+                       c.dip.p + (size_t)a * N, c.dip.p + (size_t)b * N, f, act_dev);
     WTB_LAUNCH_CHECK();
 }
 

@@ -242,6 +242,18 @@ int wtb_bse_build(wtb_ctx *ctx, const wtb_coulomb *coulomb, int precision, int c
     });
 }
 
+int wtb_set_temperature(wtb_ctx *ctx, double temp_k) {
+    return guarded(ctx, [&](Ctx &c) {
+        if (!(temp_k == temp_k)) throw Error{WTB_ERR_ARG, "temperature is NaN"};
+        c.temperature = temp_k < 0.0 ? -1.0 : temp_k;
+        // a store built at another temperature is stale
+        if (c.bse.valid && c.bse.temperature != c.temperature) {
+            c.bse.valid = false;
+            c.sol.valid = false;
+        }
+    });
+}
+
 int wtb_bse_solve(wtb_ctx *ctx, int nev, double tol, int max_it, double *E, double *X, wtb_report *rep) {
     return guarded(ctx, [&](Ctx &c) {
         if (!c.bse.valid) throw Error{WTB_ERR_STATE, "wtb_bse_build must run before wtb_bse_solve"};
@@ -327,13 +339,19 @@ int wtb_bse_diag(wtb_ctx *ctx, double *diag) {
 }
 
 int wtb_grad_energies(wtb_ctx *ctx, const double *w, wtb_grad *g, double *hop_re_grad, double *hop_im_grad) {
-    return guarded(ctx, [&](Ctx &c) { grad_energies(c, w, g, hop_re_grad, hop_im_grad); });
+    return guarded(ctx, [&](Ctx &c) {
+        if (c.bse.temperature >= 0.0)
+            throw Error{WTB_ERR_UNSUPPORTED, "gradients of the temperature BSE are not available"};
+        grad_energies(c, w, g, hop_re_grad, hop_im_grad);
+    });
 }
 
 int wtb_grad_spectrum(wtb_ctx *ctx, int comp, double cshift, double gamma, int nw, const double *omega,
                       const double *target, double *loss, double *eps2, wtb_grad *g, double *hop_re_grad,
                       double *hop_im_grad) {
     return guarded(ctx, [&](Ctx &c) {
+        if (c.bse.temperature >= 0.0)
+            throw Error{WTB_ERR_UNSUPPORTED, "gradients of the temperature BSE are not available"};
         grad_spectrum(c, comp, cshift, gamma, nw, omega, target, loss, eps2, g, hop_re_grad, hop_im_grad);
     });
 }

@@ -155,6 +155,8 @@ struct Bse {
     DBuf<double> diag;   // E_c - E_v
     DBuf<unsigned> hmax; // max |Re|, |Im| of the FP32 store (float bits)
     float hscale = 1.f;  // power of two: max |H| / hscale ~ 2^13 for the FP16 split
+    double temperature = -1.0;  // of the build (< 0: none)
+    DBuf<double> fdeh;   // f_v - f_c of every state when built with a temperature
     bool valid = false;
 };
 
@@ -242,6 +244,7 @@ struct Ctx {
     DBuf<cd> ws_ypart;         // k-chunk partial products of the int8-digit matvec
     SolverWs lws;
     KernelClock kclock;
+    double temperature = -1.0;  // wtb_set_temperature: < 0 off (kelvin otherwise)
 };
 
 // k.R = (kx Rx + ky Ry) + kz Rz, evaluated without FMA contraction in the

@@ -109,6 +109,14 @@ module wtb_hip
             integer(c_int), value :: precision, convention
         end function wtb_bse_build
 
+        ! BSET=T (bsesolvertemp, module_bse.F90:2649-3263): temperature factors
+        ! f_v - f_c for the following wtb_bse_build and wtb_oscillator; temp_k < 0: off
+        integer(c_int) function wtb_set_temperature(ctx, temp_k) bind(C, name="wtb_set_temperature")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: temp_k
+        end function wtb_set_temperature
+
         ! slepc_solver(HX,EX,HE,NX,NE) (module_slepc_solver.F90:12-97) with nev explicit;
         ! X: c_loc(HE(N,nev)) or c_null_ptr; rep: c_loc of a wtb_report or c_null_ptr
         integer(c_int) function wtb_bse_solve(ctx, nev, tol, max_it, e, x, rep) bind(C, name="wtb_bse_solve")

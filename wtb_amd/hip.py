@@ -30,7 +30,7 @@ COMPONENTS = {"x": 0, "y": 1, "z": 2, "sp": 3, "sm": 4}
 EXPORTS = ["wtb_create", "wtb_destroy", "wtb_last_error", "wtb_set_tb_model", "wtb_set_kpoints",
            "wtb_hk_eig", "wtb_get_bands", "wtb_bse_build", "wtb_bse_solve", "wtb_optics",
            "wtb_oscillator", "wtb_bse_get_block", "wtb_bse_matvec", "wtb_bse_diag",
-           "wtb_slepc_solver_compat", "wtb_grad_energies", "wtb_grad_spectrum"]
+           "wtb_slepc_solver_compat", "wtb_grad_energies", "wtb_grad_spectrum", "wtb_set_temperature"]
 
 
 class WTBError(RuntimeError):
@@ -94,6 +94,7 @@ def load():
         "wtb_slepc_solver_compat": [P, P, P, I, I],
         "wtb_grad_energies": [P, P, C.POINTER(GradScalars), P, P],
         "wtb_grad_spectrum": [P, I, D, D, I, P, P, P, P, C.POINTER(GradScalars), P, P],
+        "wtb_set_temperature": [P, D],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -193,6 +194,12 @@ class Context:
         if isinstance(coulomb, dict):
             coulomb = coulomb_params(**coulomb)
         self._check(self.lib.wtb_bse_build(self.h, C.byref(coulomb), int(precision), int(convention)))
+
+    def set_temperature(self, temp_k):
+        """Temperature BSE (BSET=T) for the following bse_build: kernel
+        elements H(i,j), i <= j, times f_v - f_c of state i, and the oscillator
+        strengths weighted accordingly; temp_k < 0 (None) switches it off."""
+        self._check(self.lib.wtb_set_temperature(self.h, -1.0 if temp_k is None else float(temp_k)))
 
     def bse_solve(self, nev, tol=1e-8, max_it=200, vectors=True, check=True):
         """Lowest nev pairs.  With check=False a WTB_ERR_NOCONV status is
