@@ -164,3 +164,54 @@ def test_optical_constants_and_lifetimes(honeycomb_run):
     sel = f > 1e-3
     ref = (144 * a2 * hbar) / ((8.0 * PI32 * cfe * out[:, 0]) * f)
     assert np.allclose(lt[sel, 2], ref[sel], rtol=1e-3)
+
+
+def test_temperature_driver_run():
+    """BSET=T with the Varshni gap correction (TA= VE): the log names the
+    temperature, exciton energies are the eigenvalues of the temperature
+    matrix built on bands shifted by the correction, the strengths and
+    sp_optics columns carry f_v - f_c."""
+    say("case driver temperature")
+    if not os.path.exists(DRIVER):
+        pytest.fail("wtb_amd/wtb_bse is not built (make -C wtb_amd/fortran)")
+    temp, st, ph = 12000.0, 5e-4, 0.02
+    cards = ["BSET= T", f"TEMP= {temp}", 'TA= "VE"', f"ST= {st}", f"PHAVG= {ph}", "SPEC= T",
+             "DTDIAG= T", "SYSDIM= \"2D\"", "NGX= 12", "NGY= 12", "NGZ= 1", "NBANDSC= 1", "NBANDSV= 1",
+             "COULOMB_POT= V2DRK", "R_0= 10.0", "EDIEL= 1.0", "EDIEL_T= 1.0", "EDIEL_B= 1.0", "EDIEL_Z= 1.0",
+             "LC= 1.0", "KTOL= 0.001", "CSHIFT= 0.05", "ENSPECI= 3.0", "ENSPECF= 7.0", "NESPEC= 400",
+             "NEV= 144", "PRECISION= FP64"]
+    d, m = run_driver(cards)
+    with open(os.path.join(d, "log_bse-diel.dat")) as fh:
+        first = fh.readline().split()
+    assert first[0] == "temperature" and float(first[1]) == temp
+    tcor = T.temperature_gap_shift("VE", st, ph, temp)
+    say("gap correction", tcor)
+    assert -2e-3 < tcor < -5e-4
+    m2 = m.copy()
+    m2.scissor = m.scissor + tcor
+    mesh = (12, 12, 1)
+    k = T.mp_mesh(*mesh, m.rlat)
+    om = O.Model(m2)
+    ref = O.hk_eig(om, k, 1, 1)
+    c = O.coulomb(kind="V2DRK", r0=10.0, ediel=(1.0, 1.0, 1.0), ez=1.0, w=0.0, lc=1.0, ktol=0.001)
+    H = O.bse_matrix_temp(k, ref["eig"], ref["vec"], c, mesh, m.rlat, 1, 1, temp)
+    w, X = np.linalg.eigh(H)
+    f, _ = O.state_factors(ref["eig"], 1, 1, temp)
+    assert f.min() < 0.95
+    dip = O.dipoles(om, k, ref["eig"], ref["vec"], 1, 1, 0.05)
+    out = load(os.path.join(d, "bse_opt_diel.dat"))
+    assert out.shape == (144, 7)
+    assert np.abs(out[:, 0] - w).max() <= 5.1e-5
+    levels = np.split(np.arange(144), np.where(np.diff(w) > 1e-6)[0] + 1)
+    for comp, col in ((0, 1), (1, 2), (2, 3)):
+        fr = np.real((X.T @ dip[comp]) * np.conj(X.T @ (dip[comp] * f)))
+        for lv in levels:
+            assert abs(out[lv, col].sum() - fr[lv].sum()) <= 5.1e-5 * len(lv) + 1e-6 * abs(fr[lv].sum())
+    sp = load(os.path.join(d, "sp_optics.dat"))
+    de = (ref["eig"][:, 1] - ref["eig"][:, 0]).astype(np.float32)
+    ax = (np.abs(dip[0]) ** 2 * f).astype(np.float32)
+    ay = (np.abs(dip[1]) ** 2 * f).astype(np.float32)
+    mine = np.round(np.stack([de, ax, ay], 1).astype(np.float64), 4)
+    mine = mine[np.lexsort(mine.T[::-1])]
+    theirs = sp[:, :3][np.lexsort(sp[:, :3].T[::-1])]
+    assert np.abs(mine - theirs).max() <= 1.01e-4

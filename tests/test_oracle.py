@@ -260,6 +260,26 @@ def test_fermidisteh_closed_forms():
     assert np.abs(O.fermidisteh(e, -e, 1e9)).max() < 1e-4
 
 
+def test_temperature_gap_shift_closed_forms():
+    """TA= VE / BE / FA gap shifts (gapcortemp, gapcortemp2) at points where
+    coth and the Bose factor take known values."""
+    kb = float(np.float32(8.617330350e-5))
+    s, ph = 4e-4, 0.025
+    # coth(x) = 2 at x = ln(3) / 2: the VE shift is -S <hw>
+    t = ph / (2 * kb * (math.log(3.0) / 2))
+    assert abs(T.temperature_gap_shift("VE", s, ph, t) + s * ph) < 1e-15
+    # T -> 0: coth -> 1 and no shift; high T, x = <hw> / 2 kB T small:
+    # coth x = 1/x + x/3 + O(x^3), the shift -2 S kB T + S <hw> - S <hw> x / 3
+    assert T.temperature_gap_shift("VE", s, ph, 1.0) == 0.0
+    hi = 1e5
+    x = ph / (2 * kb * hi)
+    assert abs(T.temperature_gap_shift("VE", s, ph, hi) - (-2 * s * kb * hi + s * ph - s * ph * x / 3)) < 1e-13
+    # Bose factor 1 at exp(<hw> / T) = 2
+    assert abs(T.temperature_gap_shift("BE", s, ph, ph / math.log(2.0)) + 2 * s) < 1e-15
+    assert T.temperature_gap_shift("FA", s, ph, 300.0) == 0.0
+    assert T.temperature_gap_shift("XX", s, ph, 300.0) == 0.0
+
+
 def test_bse_matrix_temp_fill_rule():
     """The vectorised f[min(i, j)] rule against a loop restatement of the
     reference's fill (matrizelbsetemp for j >= i with the factor of row i, the

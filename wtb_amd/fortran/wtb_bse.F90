@@ -9,7 +9,10 @@
 ! bse_opt_diel.dat, bse_opt_diel-pol.dat and, with SPEC=T, bse_diel_{xx,xy,
 ! xz,yy,yz,zz}.dat, bse_refractive_index.dat, bse_extinction_coef.dat,
 ! bse_reflectibility.dat, bse_absorption_coef.dat, bse_en_loss_func.dat,
-! exciton_lifetime.dat.
+! exciton_lifetime.dat.  BSET=T runs the temperature BSE (bsesolvertemp,
+! module_bse.F90:2649-3263; main.F90:102-120): TEMP= in K, TA= FA | VE | BE
+! with ST= and PHAVG= for the gap correction, a 'temperature' line in the log
+! and |d|^2 weighted by f_v - f_c.
 !
 ! Cards of this driver beyond the reference's: NEV= (eigenpairs, default 100
 ! as the reference's SLEPc call), PRECISION= (FP64 | MIXED | FP32), KERNEL=
@@ -29,6 +32,10 @@ module wtb_cards
     character(len=8) :: precision = "FP64", kernel = "REF"
     logical :: bse = .false., spec = .false., dtfull = .false., cpol = .false., renorm = .true.
     logical :: compat = .true.
+    ! BSET=T (bsesolvertemp): TEMP= in K, TA= gap correction FA | VE | BE with ST=, PHAVG=
+    logical :: bset = .false.
+    double precision :: temp = 0.0d0, st = 0.0d0, phavg = 0.0d0
+    character(len=2) :: ta = "FA"
 contains
     ! "CARD= value" lines from standard input, defaults of module_input.F90:53-120
     subroutine read_cards()
@@ -68,6 +75,11 @@ contains
             case ("CPOL=");         read(b, *) cpol
             case ("RNMD=");         read(b, *) renorm
             case ("BSE=");          read(b, *) bse
+            case ("BSET=");         read(b, *) bset
+            case ("TEMP=");         read(b, *) temp
+            case ("TA=");           read(b, *) ta
+            case ("ST=");           read(b, *) st
+            case ("PHAVG=");        read(b, *) phavg
             case ("SPEC=");         read(b, *) spec
             case ("NEV=");          read(b, *) nev
             case ("PRECISION=");    precision = trim(adjustl(b))
@@ -412,10 +424,21 @@ program wtb_bse
     double precision, allocatable :: act(:, :)
     real, allocatable :: vecres(:, :)
     integer, allocatable :: order(:)
-    double precision :: egap, spinf
+    double precision :: egap, spinf, tcor
+    double precision, allocatable :: fdeh(:)
 
     call read_cards()
-    if (.not. bse) stop "wtb_bse: only the BSE=T path is provided"
+    if (bse .and. bset) stop "ERROR: Choose BSE=T or BSET=T not both"
+    if (.not. (bse .or. bset)) stop "wtb_bse: only the BSE=T and BSET=T paths are provided"
+    ! gap correction of the temperature path (bsesolvertemp, module_bse.F90:2857-2875),
+    ! applied with the scissor to the conduction bands
+    tcor = 0.0d0
+    if (bset) then
+        select case (trim(ta))
+        case ("VE"); tcor = gap_shift_varshni(st, phavg, temp)
+        case ("BE"); tcor = gap_shift_bose(st, phavg, temp)
+        end select
+    end if
     call read_tb(params)
     select case (trim(meshtype))
     case ("MKH")
@@ -460,11 +483,12 @@ program wtb_bse
 
     ! ---- GPU path: eigsys loop, optsp, matrizelbse, slepc_solver, dielbse*
     call wtb_check(ctx, wtb_create(ctx, int(device, c_int)), "wtb_create")
-    call wtb_check(ctx, wtb_set_tb_model(ctx, norb, nvec, rvec, ffactor, hop, ihop, rlat, efermi, scs, flags), &
-                   "wtb_set_tb_model")
+    call wtb_check(ctx, wtb_set_tb_model(ctx, norb, nvec, rvec, ffactor, hop, ihop, rlat, efermi, scs + tcor, &
+                   flags), "wtb_set_tb_model")
     call wtb_check(ctx, wtb_set_kpoints(ctx, nk, kpt, ngrid), "wtb_set_kpoints")
     call wtb_check(ctx, wtb_hk_eig(ctx, nv, nc, c_loc(eigv), c_null_ptr, c_loc(nocpk)), "wtb_hk_eig")
     call wtb_check(ctx, wtb_optics(ctx, cshift, c_loc(dip)), "wtb_optics")
+    if (bset) call wtb_check(ctx, wtb_set_temperature(ctx, temp), "wtb_set_temperature")
     call wtb_check(ctx, wtb_bse_build(ctx, coul, prec, conv), "wtb_bse_build")
     ws = 0.0d0
     call wtb_check(ctx, wtb_bse_solve(ctx, nevs, 1.0d-8, 200, ws, c_null_ptr, c_loc(rep)), "wtb_bse_solve")
@@ -490,6 +514,7 @@ program wtb_bse
     end do
     open(unit=300, file=trim(outputfolder)//"/log_bse-diel.dat", status='unknown', iostat=erro)
     if (erro /= 0) stop "Error opening log_bse-diel output file"
+    if (bset) write(300, *) 'temperature', temp
     write(300, *) 'grid:', ngrid(1), ngrid(2), ngrid(3)
     write(300, "(A14,1E15.4)") 'ktol-coulomb:', ktol
     write(300, "(A13,3F15.4)") 'kmesh shift:', mshift(1), mshift(2), mshift(3)
@@ -503,7 +528,21 @@ program wtb_bse
 
     ! ---- sp_optics.dat / sp_optics-pol.dat: transitions sorted by energy
     ! (module_bse.F90:2300-2375); float32 columns as the reference's vecres
-    allocate(vecres(dimbse, 15), order(dimbse))
+    ! the temperature path weights |d|^2 by f_v - f_c of the transition
+    ! (bsesolvertemp, module_bse.F90:2984-3013)
+    allocate(vecres(dimbse, 15), order(dimbse), fdeh(dimbse))
+    fdeh = 1.0d0
+    if (bset) then
+        ib = 0
+        do k = 1, nk
+            do c = 1, nc
+                do v = 1, nv
+                    ib = ib + 1
+                    fdeh(ib) = fermi_factor(eigv(k, nv + c), eigv(k, v), temp)
+                end do
+            end do
+        end do
+    end if
     ib = 0
     do k = 1, nk
         do c = 1, nc
@@ -514,14 +553,14 @@ program wtb_bse
                 vecres(ib, 5) = real(nocpk(k) - nv + (nv + c))
                 vecres(ib, 6) = real(nocpk(k) - nv + v)
                 vecres(ib, 7) = real(eigv(k, nv + c) - eigv(k, v))
-                vecres(ib, 8) = real(dip(ib, 1) * conjg(dip(ib, 1)))
-                vecres(ib, 9) = real(dip(ib, 2) * conjg(dip(ib, 2)))
-                vecres(ib, 10) = real(dip(ib, 3) * conjg(dip(ib, 3)))
-                vecres(ib, 11) = real(dip(ib, 1) * conjg(dip(ib, 2)))
-                vecres(ib, 12) = real(dip(ib, 1) * conjg(dip(ib, 3)))
-                vecres(ib, 13) = real(dip(ib, 2) * conjg(dip(ib, 3)))
-                vecres(ib, 14) = real(dip(ib, 4) * conjg(dip(ib, 4)))
-                vecres(ib, 15) = real(dip(ib, 5) * conjg(dip(ib, 5)))
+                vecres(ib, 8) = real(dip(ib, 1) * conjg(dip(ib, 1))) * fdeh(ib)
+                vecres(ib, 9) = real(dip(ib, 2) * conjg(dip(ib, 2))) * fdeh(ib)
+                vecres(ib, 10) = real(dip(ib, 3) * conjg(dip(ib, 3))) * fdeh(ib)
+                vecres(ib, 11) = real(dip(ib, 1) * conjg(dip(ib, 2))) * fdeh(ib)
+                vecres(ib, 12) = real(dip(ib, 1) * conjg(dip(ib, 3))) * fdeh(ib)
+                vecres(ib, 13) = real(dip(ib, 2) * conjg(dip(ib, 3))) * fdeh(ib)
+                vecres(ib, 14) = real(dip(ib, 4) * conjg(dip(ib, 4))) * fdeh(ib)
+                vecres(ib, 15) = real(dip(ib, 5) * conjg(dip(ib, 5))) * fdeh(ib)
             end do
         end do
     end do
@@ -567,6 +606,31 @@ program wtb_bse
         ' eV, max rel residual ', rep%max_rel_residual
 
 contains
+
+    ! f_v - f_c at T kelvin, Fermi level 0 (fermidisteh, module_math.F90:410-421):
+    ! the Boltzmann constant is a default-real literal there, and T = 0 is 1E-8
+    double precision function fermi_factor(ec, ev, t)
+        double precision, intent(in) :: ec, ev, t
+        double precision, parameter :: kb = 8.617330350E-5
+        double precision :: tt, beta
+        tt = t
+        if (tt == 0d0) tt = 1E-8
+        beta = 1.0 / (kb * tt)
+        fermi_factor = 1.0 / (exp(ev * beta) + 1.0) - 1.0 / (exp(ec * beta) + 1.0)
+    end function fermi_factor
+
+    ! TA= VE gap shift (gapcortemp, module_math.F90:383-391)
+    double precision function gap_shift_varshni(s, ph, t)
+        double precision, intent(in) :: s, ph, t
+        double precision, parameter :: kb = 8.617330350E-5
+        gap_shift_varshni = -s * ph * ((1.0d0 / tanh(ph / (2.0 * kb * t))) - 1d0)
+    end function gap_shift_varshni
+
+    ! TA= BE gap shift (gapcortemp2, module_math.F90:395-404)
+    double precision function gap_shift_bose(s, ph, t)
+        double precision, intent(in) :: s, ph, t
+        gap_shift_bose = -(2.0 * s / (exp(ph / t) - 1.0))
+    end function gap_shift_bose
 
     ! bsedielraw (module_bse.F90:835-1036): re-reads bse_opt_diel.dat
     subroutine diel_files(spinf)

@@ -173,6 +173,24 @@ def mp_mesh(n1: int, n2: int, n3: int, rlat: np.ndarray,
     return k + kshift[None, :]
 
 
+KB_F32 = float(np.float32(8.617330350e-5))   # eV/K, a default-real literal in the reference
+
+
+def temperature_gap_shift(ta: str, st: float, phavg: float, temp: float) -> float:
+    """Gap shift that the temperature BSE adds to the scissor (bsesolvertemp,
+    src/module_bse.F90:2857-2875), in eV.
+
+    TA "VE": -S <hw> (coth(<hw> / 2 kB T) - 1) (gapcortemp,
+    src/module_math.F90:383-391); TA "BE": -2 S / (exp(<hw> / T) - 1)
+    (gapcortemp2, :395-404, with T in the exponent as written there); any
+    other TA ("FA"): 0."""
+    if ta == "VE":
+        return -st * phavg * (1.0 / math.tanh(phavg / (2.0 * KB_F32 * temp)) - 1.0)
+    if ta == "BE":
+        return -(2.0 * st / (math.exp(phavg / temp) - 1.0))
+    return 0.0
+
+
 # --------------------------------------------------------------------------
 # synthetic generators
 # --------------------------------------------------------------------------
