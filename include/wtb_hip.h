@@ -128,6 +128,21 @@ int wtb_set_kpoints(wtb_ctx *ctx, int nk, const double *kpt, const int *ngrid);
  *   nocp : int nocpk(nk) */
 int wtb_hk_eig(wtb_ctx *ctx, int nv, int nc, double *eig, double *vec, int *nocp);
 
+/* Finite-Q exciton (the BSE_BND=T path: bsebnds, module_bse.F90:1206-1566).
+ * Replaces the eigsys loop over the k+Q mesh (:1398-1417): qpt(nk,3)
+ * column-major is the monhkhorst_packq mesh (module_kmesh.F90:281-334) of the
+ * nk points set by wtb_set_kpoints, q[3] the Cartesian Q (the reference's
+ * q(2:4)).  The following wtb_bse_build then forms matrizelbsekq
+ * (module_bse.F90:452-569): conduction bands from k+Q, valence bands from k,
+ * V(k_i - k_j) of the k mesh and, when |Q| != 0, the exchange term
+ * -V(Q) <c_i|v_i><v_j|c_j> (with a temperature: matrizelbsekqtemp :664-780).
+ * nv, nc must equal those of the last wtb_hk_eig.  eig/vec/nocp (optional)
+ * as for wtb_hk_eig, of the k+Q mesh.  q == NULL switches finite Q off.
+ * wtb_set_tb_model and wtb_set_kpoints switch it off too.  On a finite-Q
+ * store wtb_oscillator and the gradients return WTB_ERR_UNSUPPORTED. */
+int wtb_hk_eig_q(wtb_ctx *ctx, const double *q, const double *qpt, int nv, int nc, double *eig, double *vec,
+                 int *nocp);
+
 /* Full spectrum of every H(k): energies(nk,norb) (scissor applied) and
  * eigenvectors(nk,norb,norb) with ev(k,o,n) = component o of band n. */
 int wtb_get_bands(wtb_ctx *ctx, double *energies, double *vectors);

@@ -8,6 +8,7 @@ complex128 interleaved).
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -217,6 +218,52 @@ def bse_matrix_temp(kpt, eig, vec, c, ngrid, rlat, nc, nv, temp, textbook=False)
     H = K * f[np.minimum.outer(idx, idx)]
     v0 = np.diag(K).real - de
     H[idx, idx] = de + v0 * f
+    return H
+
+
+def splice_q(eig, vec, eigq, vecq, nv):
+    """Bands of the finite-Q basis (bsebnds, src/module_bse.F90:1436-1442):
+    valence slots j < nv from the k mesh, conduction slots from the k+Q mesh."""
+    e = np.array(eig, float)
+    u = np.array(vec, np.complex128)
+    e[:, nv:] = np.asarray(eigq)[:, nv:]
+    u[:, nv:, :] = np.asarray(vecq)[:, nv:, :]
+    return e, u
+
+
+def exchange_amplitudes(vec, nc, nv):
+    """a_i = <c(k+Q)|v(k)> = sum_o conj(c_o) v_o of every state
+    i = k nc nv + c nv + v of the spliced bands (vcv of matrizelbsekq,
+    src/module_bse.F90:551-555)."""
+    u = np.asarray(vec, np.complex128)
+    nk = u.shape[0]
+    cb = u[:, nv:nv + nc, :]                   # (nk, nc, norb)
+    vb = u[:, :nv, :]                          # (nk, nv, norb)
+    return np.einsum("kco,kvo->kcv", cb.conj(), vb).reshape(nk * nc * nv)
+
+
+def bse_matrix_q(kpt, eig, vec, eigq, vecq, c, ngrid, rlat, nc, nv, q, temp=None, exchange=True,
+                 textbook=False):
+    """Finite-Q BSE matrix (matrizelbsekq, src/module_bse.F90:452-569, filled
+    in bsebnds :1444-1454).  Conduction bands at k+Q, valence at k, the direct
+    term V(k_i - k_j) <c_i|c_j><v_i|v_j> of bse_matrix on those bands and, for
+    |Q| != 0, the exchange -V(Q) <c_i|v_i><v_j|c_j> on every element.  With
+    temp: matrizelbsekqtemp (:664-780), the whole kernel part times f of the
+    row state, read through the upper triangle as bse_matrix_temp."""
+    e, u = splice_q(eig, vec, eigq, vecq, nv)
+    K = bse_matrix(kpt, e, u, c, ngrid, rlat, nc, nv, textbook=textbook)
+    qv = np.asarray(q, float)
+    if exchange and math.sqrt(qv[0] ** 2 + qv[1] ** 2 + qv[2] ** 2) != 0.0:
+        a = exchange_amplitudes(u, nc, nv)
+        vq = coulomb_v(c, qv, np.zeros(3), np.asarray(ngrid, np.int32), rlat)
+        K = K - vq * np.outer(a, a.conj())
+    if temp is None:
+        return K
+    f, de = state_factors(e, nc, nv, temp)
+    n = K.shape[0]
+    idx = np.arange(n)
+    H = K * f[np.minimum.outer(idx, idx)]
+    H[idx, idx] = de + (np.diag(K).real - de) * f
     return H
 
 

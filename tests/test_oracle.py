@@ -312,6 +312,101 @@ def test_bse_matrix_temp_fill_rule():
     assert np.abs(H0 - K).max() < 1e-14 * np.abs(K).max()
 
 
+# ----------------------------- finite Q -----------------------------------
+
+def _q_problem(qfrac, mesh=(3, 3, 1), nv=2, nc=2, flags=O.COMPAT_F32 | O.GAUGE_LAPACK, pi=T.PI_F32):
+    m = hex_model(mesh)
+    k = T.mp_mesh(*mesh, m.rlat, pi=pi)
+    q = np.asarray(qfrac, float) @ T.recip_vectors(m.rlat, pi)
+    kq = T.mp_mesh_q(*mesh, m.rlat, q, pi=pi)
+    om = O.Model(m, flags=flags)
+    r, rq = O.hk_eig(om, k, nv, nc), O.hk_eig(om, kq, nv, nc)
+    assert r["status"] == 0 and rq["status"] == 0
+    return m, k, q, r, rq
+
+
+def test_mp_mesh_q_is_the_shifted_mesh():
+    m = hex_model((4, 4, 1))
+    q = np.array([0.031, -0.017, 0.0])
+    kq = T.mp_mesh_q(4, 4, 1, m.rlat, q, shift=(0.1, 0.0, 0.0))
+    k = T.mp_mesh(4, 4, 1, m.rlat, shift=(0.1, 0.0, 0.0))
+    assert np.abs(kq - (k + q)).max() < 1e-15
+    assert np.array_equal(T.mp_mesh_q(4, 4, 1, m.rlat, np.zeros(3)), T.mp_mesh(4, 4, 1, m.rlat))
+
+
+@pytest.mark.parametrize("temp", [None, 9000.0])
+def test_bse_matrix_q_against_the_element_formula(temp):
+    """The finite-Q matrix (splice + direct kernel + rank-one exchange) against
+    a loop over matrizelbsekq's (and matrizelbsekqtemp's) element formula for
+    j >= i, with the lower triangle its conjugate."""
+    mesh, nv, nc = (3, 3, 1), 2, 2
+    m, k, q, r, rq = _q_problem((0.11, -0.07, 0.0), mesh, nv, nc)
+    c = O.coulomb(**COUL)
+    H = O.bse_matrix_q(k, r["eig"], r["vec"], rq["eig"], rq["vec"], c, mesh, m.rlat, nc, nv, q, temp=temp)
+    g = np.array(mesh, np.int32)
+    vq = O.coulomb_v(c, q, np.zeros(3), g, m.rlat)
+    assert vq != 0.0
+    N = k.shape[0] * nc * nv
+    L = np.zeros((N, N), np.complex128)
+    for i in range(N):
+        ki, ci, vi = i // (nc * nv), (i // nv) % nc, i % nv
+        c1, v1 = rq["vec"][ki, nv + ci], r["vec"][ki, vi]
+        ec1, ev1 = rq["eig"][ki, nv + ci], r["eig"][ki, vi]
+        f1 = 1.0 if temp is None else O.fermidisteh(ec1, ev1, temp)
+        for j in range(i, N):
+            kj, cj, vj = j // (nc * nv), (j // nv) % nc, j % nv
+            c2, v2 = rq["vec"][kj, nv + cj], r["vec"][kj, vj]
+            vk = O.coulomb_v(c, k[ki], k[kj], g, m.rlat)
+            vcv = np.sum(c1.conj() * v1)
+            vvc = np.sum(v2.conj() * c2)
+            if i == j:
+                L[i, i] = (ec1 - ev1) + (vk - vq * vcv * vvc) * f1
+            else:
+                vc = np.sum(c1.conj() * c2)
+                vv = np.sum(v1.conj() * v2)
+                L[i, j] = (vk * vc * vv - vq * vcv * vvc) * f1
+                L[j, i] = np.conj(L[i, j])
+    assert np.abs(H - L).max() < 1e-13 * np.abs(L).max()
+    # the exchange is visible at this Q
+    H0 = O.bse_matrix_q(k, r["eig"], r["vec"], rq["eig"], rq["vec"], c, mesh, m.rlat, nc, nv, q, temp=temp,
+                        exchange=False)
+    assert np.abs(H - H0).max() > 1e-6 * np.abs(H).max()
+
+
+def test_exchange_is_a_positive_rank_one_update():
+    """-V(Q) a a^H with V(Q) < 0: one positive singular direction, and the
+    spectra interlace, w0[i] <= w1[i] <= w0[i + 1]."""
+    mesh = (3, 3, 1)
+    m, k, q, r, rq = _q_problem((0.08, 0.05, 0.0), mesh)
+    c = O.coulomb(**COUL)
+    args = (k, r["eig"], r["vec"], rq["eig"], rq["vec"], c, mesh, m.rlat, 2, 2, q)
+    H1, H0 = O.bse_matrix_q(*args), O.bse_matrix_q(*args, exchange=False)
+    X = H1 - H0
+    s = np.linalg.svd(X, compute_uv=False)
+    assert s[1] < 1e-12 * s[0]
+    assert np.linalg.eigvalsh(X)[-1] > 0 and np.linalg.eigvalsh(X)[0] > -1e-12 * s[0]
+    w0, w1 = np.linalg.eigvalsh(H0), np.linalg.eigvalsh(H1)
+    eps = 1e-10 * np.abs(w0).max()
+    assert np.all(w0 <= w1 + eps)
+    assert np.all(w1[:-1] <= w0[1:] + eps)
+
+
+def test_finite_q_at_zero_and_at_a_reciprocal_vector():
+    """Q = 0 is the plain kernel; Q = G (H(k + G) = H(k) with FP64 phases and
+    an exact 2 pi) gives the plain spectrum, the exchange amplitudes
+    <c(k)|v(k)> being zero."""
+    mesh = (3, 3, 1)
+    c = O.coulomb(**COUL)
+    m, k, q, r, rq = _q_problem((0.0, 0.0, 0.0), mesh)
+    K = O.bse_matrix(k, r["eig"], r["vec"], c, mesh, m.rlat, 2, 2)
+    assert np.array_equal(O.bse_matrix_q(k, r["eig"], r["vec"], rq["eig"], rq["vec"], c, mesh, m.rlat, 2, 2, q), K)
+    m, k, q, r, rq = _q_problem((1.0, 0.0, 0.0), mesh, flags=O.GAUGE_LAPACK, pi=math.pi)
+    HG = O.bse_matrix_q(k, r["eig"], r["vec"], rq["eig"], rq["vec"], c, mesh, m.rlat, 2, 2, q)
+    K = O.bse_matrix(k, r["eig"], r["vec"], c, mesh, m.rlat, 2, 2)
+    assert np.abs(np.linalg.eigvalsh(HG) - np.linalg.eigvalsh(K)).max() < 1e-10
+    assert np.abs(O.exchange_amplitudes(O.splice_q(r["eig"], r["vec"], rq["eig"], rq["vec"], 2)[1], 2, 2)).max() < 1e-12
+
+
 # ------------------------------ optics ------------------------------------
 
 def test_dhk_matches_finite_difference():

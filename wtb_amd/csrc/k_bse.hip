@@ -160,6 +160,19 @@ __device__ __forceinline__ void pair_products(int k, int nk, int norb, const C *
     }
 }
 
+// Exchange term of the finite-Q kernel (matrizelbsekq, src/module_bse.F90:536-567):
+// -V(Q) <c_I|v_I> <v_J|c_J> = -V(Q) a_I conj(a_J) with a_I = <c(k_I + Q)|v(k_I)>,
+// the same expression for either triangle (its conjugate is the swapped term).
+// Uncontracted, so that (J, I) of a diagonal tile is the exact conjugate of (I, J).
+template <class C>
+__device__ __forceinline__ void sub_exchange(C &val, const cd *__restrict__ xa, double vx, long I, long J) {
+#pragma clang fp contract(off)
+    const cd ai = xa[I], aj = xa[J];
+    const double re = ai.x * aj.x + ai.y * aj.y, im = ai.y * aj.x - ai.x * aj.y;
+    val.x -= (decltype(val.x))(vx * re);
+    val.y -= (decltype(val.y))(vx * im);
+}
+
 // Pair slots of one plane of pair data in LDS: pair (kl, kpl) sits at
 // kpl (TK + 1) + kl, so that lanes running over kl (row writes) and lanes
 // running over kpl (mirror writes) both hit distinct banks.
@@ -175,14 +188,18 @@ constexpr int pair_slots() { return TK * (TK + 1) + 1; }
 // (matrizelbsetemp fills j >= i with the factor of row i,
 // src/module_bse.F90:580-660,3062-3070); the mirror of an element carries the
 // same factor, so the store stays exactly Hermitian.
+// xa (finite Q != 0, else null): the exchange term above, vqx[0] = V(Q), added
+// before the factor (matrizelbsekqtemp, src/module_bse.F90:664-780).
 template <class R, int NC, int NV, int TK>
 __global__ void __launch_bounds__(kThreads)
 bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
                  const double *__restrict__ kpt, const double *__restrict__ esel,
                  const typename CT<R>::C *__restrict__ usel, typename CT<R>::C *__restrict__ H,
-                 unsigned *__restrict__ hmax, const double *__restrict__ fdeh) {
+                 unsigned *__restrict__ hmax, const double *__restrict__ fdeh, const cd *__restrict__ xa,
+                 const double *__restrict__ vqx) {
     using C = typename CT<R>::C;
     constexpr int NB = NC + NV, NCV = NC * NV, NC2 = NC * NC, NV2 = NV * NV;
+    const double vx = xa ? vqx[0] : 0.0;
     constexpr int S = TK * TK / kThreads;              // pairs per thread
     constexpr int NPP = pair_slots<TK>();
     static_assert(kThreads % TK == 0 && S >= 1, "tile must split evenly over the workgroup");
@@ -297,10 +314,13 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
                     } else {
                         const int k = k0 + kl;
                         double de = esel[(size_t)k * NB + NV + ci] - esel[(size_t)k * NB + vi];
-                        val = mk<C>(fdeh ? de + (double)vq * fdeh[I] : de + (double)vq, 0.0);
+                        double w = (double)vq;
+                        if (xa) w -= vx * (xa[I].x * xa[I].x + xa[I].y * xa[I].y);
+                        val = mk<C>(fdeh ? de + w * fdeh[I] : de + w, 0.0);
                     }
                 } else {
                     val = pair_value(vq, a[cj], b[vj]);
+                    if (xa) sub_exchange(val, xa, vx, I, J);
                     if (fdeh) {
                         const R f = (R)fdeh[I < J ? I : J];
                         val.x *= f;
@@ -337,6 +357,7 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
 #pragma unroll
                 for (int vj = 0; vj < NV; ++vj) {
                     C val = pair_value(vq, a[cj], b[vj]);
+                    if (xa) sub_exchange(val, xa, vx, I, Jb + cj * NV + vj);
                     if (fdeh) {     // J < I off the diagonal tiles: the factor of J
                         const R f = (R)fdeh[Jb + cj * NV + vj];
                         val.x *= f;
@@ -363,7 +384,7 @@ __global__ void zero_pad_rows_kernel(long N, long ld, C *__restrict__ H) {
 }
 
 template <class R, int NC, int NV>
-void launch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const typename CT<R>::C *usel,
+void launch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const double *esel, const typename CT<R>::C *usel,
                  typename CT<R>::C *H) {
     using C = typename CT<R>::C;
     constexpr int per = NC * NC + NV * NV;
@@ -379,8 +400,9 @@ void launch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const typename CT<
     const long ntile = nblk * (nblk + 1) / 2;
     unsigned *hm = sizeof(R) == sizeof(float) ? c.bse.hmax.p : nullptr;
     const double *fd = c.bse.temperature >= 0.0 ? c.bse.fdeh.p : nullptr;
+    const cd *xa = c.bse.xa.n ? c.bse.xa.p : nullptr;
     hipLaunchKernelGGL((bse_build_kernel<R, NC, NV, TK>), dim3((unsigned)ntile), dim3(kThreads), shmem, c.stream,
-                       nk, norb, ld, textbook, cd_, c.kpt.p, c.bands.esel.p, usel, H, hm, fd);
+                       nk, norb, ld, textbook, cd_, c.kpt.p, esel, usel, H, hm, fd, xa, c.bse.vqx.p);
     WTB_LAUNCH_CHECK();
     if (ld > N) {
         const long tot = (ld - N) * N;
@@ -391,11 +413,11 @@ void launch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const typename CT<
 }
 
 template <class R>
-void dispatch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const typename CT<R>::C *usel,
+void dispatch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const double *esel, const typename CT<R>::C *usel,
                    typename CT<R>::C *H) {
     const int nc = c.bands.nc, nv = c.bands.nv;
 #define WTB_TILE_CASE(A, B) \
-    if (nc == A && nv == B) return launch_tile<R, A, B>(c, cd_, textbook, usel, H);
+    if (nc == A && nv == B) return launch_tile<R, A, B>(c, cd_, textbook, esel, usel, H);
     WTB_TILE_CASE(1, 1) WTB_TILE_CASE(1, 2) WTB_TILE_CASE(2, 1) WTB_TILE_CASE(2, 2)
     WTB_TILE_CASE(1, 3) WTB_TILE_CASE(3, 1) WTB_TILE_CASE(2, 3) WTB_TILE_CASE(3, 2) WTB_TILE_CASE(3, 3)
     WTB_TILE_CASE(1, 4) WTB_TILE_CASE(4, 1) WTB_TILE_CASE(2, 4) WTB_TILE_CASE(4, 2) WTB_TILE_CASE(3, 4)
@@ -405,14 +427,55 @@ void dispatch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const typename C
 }
 
 __global__ void bse_diag_kernel(int nk, int nc, int nv, CoulombDev coul, const double *__restrict__ esel,
-                                const double *__restrict__ fdeh, double *__restrict__ diag) {
+                                const double *__restrict__ fdeh, const cd *__restrict__ xa,
+                                const double *__restrict__ vqx, double *__restrict__ diag) {
     long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
     long N = (long)nk * nc * nv;
     if (i >= N) return;
     int k = (int)(i / (nc * nv)), c = (int)((i / nv) % nc), v = (int)(i % nv);
     int nb = nc + nv;
-    const double v0 = coulomb_v(coul, 0.0, 0.0, 0.0);
-    diag[i] = esel[(size_t)k * nb + nv + c] - esel[(size_t)k * nb + v] + (fdeh ? v0 * fdeh[i] : v0);
+    double w = coulomb_v(coul, 0.0, 0.0, 0.0);
+    if (xa) w -= vqx[0] * (xa[i].x * xa[i].x + xa[i].y * xa[i].y);
+    diag[i] = esel[(size_t)k * nb + nv + c] - esel[(size_t)k * nb + v] + (fdeh ? w * fdeh[i] : w);
+}
+
+// Finite-Q band set: slot j < nv of every k from the k mesh, slots nv.. from
+// the k+Q mesh (bsebnds pairs vector(k, v) with vectorq(k, c),
+// src/module_bse.F90:1436-1442); energies [k][j], vectors [(o nb + j) nk + k].
+__global__ void fq_splice_kernel(int nk, int nb, int nv, int norb, const double *__restrict__ ek,
+                                 const double *__restrict__ eq, const cd *__restrict__ uk,
+                                 const cd *__restrict__ uq, double *__restrict__ e, cd *__restrict__ u,
+                                 cf *__restrict__ u32) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long nu = (long)nk * nb * norb;
+    if (i < nu) {
+        const int j = (int)((i / nk) % nb);
+        const cd x = j < nv ? uk[i] : uq[i];
+        u[i] = x;
+        u32[i] = make_float2((float)x.x, (float)x.y);
+    }
+    if (i < (long)nk * nb) e[i] = (i % nb) < nv ? ek[i] : eq[i];
+}
+
+// a_I = <c(k+Q)|v(k)> = sum_o conj(c_o) v_o of every state I = k nc nv + c nv + v
+// (vcv of matrizelbsekq, src/module_bse.F90:551-555), and V(Q) (vcoulq: the
+// potential at q(2:4) - 0, :478-533)
+__global__ void fq_exchange_kernel(int nk, int nc, int nv, int norb, CoulombDev coul, double qx, double qy,
+                                   double qz, const cd *__restrict__ u, cd *__restrict__ xa,
+                                   double *__restrict__ vqx) {
+    const long i = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    const long N = (long)nk * nc * nv;
+    if (i == 0) vqx[0] = coulomb_v(coul, qx, qy, qz);
+    if (i >= N) return;
+    const int k = (int)(i / (nc * nv)), c = (int)((i / nv) % nc), v = (int)(i % nv);
+    const int nb = nc + nv;
+    double re = 0.0, im = 0.0;
+    for (int o = 0; o < norb; ++o) {
+        const cd cc = u[((size_t)o * nb + nv + c) * nk + k], vv = u[((size_t)o * nb + v) * nk + k];
+        re += cc.x * vv.x + cc.y * vv.y;
+        im += cc.x * vv.y - cc.y * vv.x;
+    }
+    xa[i] = make_double2(re, im);
 }
 
 // f_v - f_c of every state at temperature temp (fermidisteh,
@@ -458,17 +521,50 @@ void launch_bse_build(Ctx &c, const wtb_coulomb &coul, int precision, int conven
     s.diag.alloc(N);
     s.hmax.alloc(1);
     WTB_HIP_CHECK(hipMemsetAsync(s.hmax.p, 0, sizeof(unsigned), c.stream));
+    // finite Q: conduction bands from the k+Q mesh, exchange when |Q| != 0
+    // (bsebnds, src/module_bse.F90:1398-1447; modq test of matrizelbsekq :535)
+    const double *esel = b.esel.p;
+    const cd *u64 = b.usel.p;
+    const cf *u32 = b.usel32.p;
+    s.finite_q = c.fq.on;
+    s.xa.free();
+    s.vqx.free();
+    if (c.fq.on) {
+        FiniteQ &f = c.fq;
+        if (!f.bq.valid || f.bq.nk != nk || f.bq.nv != nv || f.bq.nc != nc)
+            throw Error{WTB_ERR_STATE, "wtb_hk_eig_q must run with the nv, nc of wtb_hk_eig before wtb_bse_build"};
+        const int norb = c.model.norb, nb = nc + nv;
+        f.esel.alloc((size_t)nk * nb);
+        f.usel.alloc((size_t)nk * nb * norb);
+        f.usel32.alloc((size_t)nk * nb * norb);
+        const long nu = (long)nk * nb * norb;
+        hipLaunchKernelGGL(fq_splice_kernel, dim3((unsigned)((nu + 255) / 256)), dim3(256), 0, c.stream, nk, nb, nv,
+                           norb, b.esel.p, f.bq.esel.p, b.usel.p, f.bq.usel.p, f.esel.p, f.usel.p, f.usel32.p);
+        WTB_LAUNCH_CHECK();
+        esel = f.esel.p;
+        u64 = f.usel.p;
+        u32 = f.usel32.p;
+        const double modq = std::sqrt(f.q[0] * f.q[0] + f.q[1] * f.q[1] + f.q[2] * f.q[2]);
+        if (modq != 0.0) {
+            s.xa.alloc(N);
+            s.vqx.alloc(1);
+            hipLaunchKernelGGL(fq_exchange_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c.stream, nk, nc,
+                               nv, norb, cd_, f.q[0], f.q[1], f.q[2], f.usel.p, s.xa.p, s.vqx.p);
+            WTB_LAUNCH_CHECK();
+        }
+    }
     s.temperature = c.temperature;
     if (s.temperature >= 0.0) {
         s.fdeh.alloc(N);
         hipLaunchKernelGGL(fdeh_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c.stream, nk, nc, nv,
-                           s.temperature, b.esel.p, s.fdeh.p);
+                           s.temperature, esel, s.fdeh.p);
         WTB_LAUNCH_CHECK();
     }
-    if (f64) dispatch_tile<double>(c, cd_, convention == WTB_CONV_TEXTBOOK, b.usel.p, s.h64.p);
-    else dispatch_tile<float>(c, cd_, convention == WTB_CONV_TEXTBOOK, b.usel32.p, s.h32.p);
+    if (f64) dispatch_tile<double>(c, cd_, convention == WTB_CONV_TEXTBOOK, esel, u64, s.h64.p);
+    else dispatch_tile<float>(c, cd_, convention == WTB_CONV_TEXTBOOK, esel, u32, s.h32.p);
     hipLaunchKernelGGL(bse_diag_kernel, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, c.stream, nk, nc, nv,
-                       cd_, b.esel.p, s.temperature >= 0.0 ? (const double *)s.fdeh.p : nullptr, s.diag.p);
+                       cd_, esel, s.temperature >= 0.0 ? (const double *)s.fdeh.p : nullptr,
+                       s.xa.n ? (const cd *)s.xa.p : nullptr, s.vqx.p, s.diag.p);
     WTB_LAUNCH_CHECK();
     if (!f64) {
         // power-of-two scale for the FP16 split: max |H| / hscale in [2^12, 2^13)

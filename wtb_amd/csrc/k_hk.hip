@@ -476,13 +476,17 @@ __global__ void to_f32_kernel(const cd *__restrict__ in, cf *__restrict__ out, s
 }  // namespace
 
 void launch_hk_eig(Ctx &c, int nv, int nc) {
+    hk_eig_on(c, c.kpt.p, c.nk, nv, nc, c.bands);
+}
+
+// eigsys over the nk points kpt ([k][3] on the device) into b
+void hk_eig_on(Ctx &c, const double *kpt, int nk, int nv, int nc, Bands &b) {
     Model &m = c.model;
-    Bands &b = c.bands;
     if (m.norb <= 0) throw Error{WTB_ERR_STATE, "no TB model set"};
-    if (c.nk <= 0) throw Error{WTB_ERR_STATE, "no k-points set"};
+    if (nk <= 0) throw Error{WTB_ERR_STATE, "no k-points set"};
     if (m.norb > kMaxOrb) throw Error{WTB_ERR_UNSUPPORTED, "norb > 32 not supported by the batched eigensolver"};
     if (nv < 1 || nc < 1 || nv + nc > m.norb) throw Error{WTB_ERR_ARG, "invalid nv/nc"};
-    const int n = m.norb, nk = c.nk, nb = nv + nc;
+    const int n = m.norb, nb = nv + nc;
     b.valid = false;
     b.nk = nk; b.nv = nv; b.nc = nc; b.nb = nb;
     b.eall.alloc((size_t)nk * n);
@@ -508,7 +512,7 @@ void launch_hk_eig(Ctx &c, int nv, int nc) {
     // hk_rot_kernel; otherwise one wavefront per k rotates Z in place
     auto run = [&](bool lane_ql) {
         hipLaunchKernelGGL(hk_house_kernel, dim3(nk), dim3(kLanes), sh_house, c.stream, n, m.nvec, m.flags,
-                           m.efermi, c.kpt.p, m.hop.p, m.rcart.p, m.invdeg.p, b.uall.p, b.tri.p);
+                           m.efermi, kpt, m.hop.p, m.rcart.p, m.invdeg.p, b.uall.p, b.tri.p);
         WTB_LAUNCH_CHECK();
         if (lane_ql) {
             hipLaunchKernelGGL(hk_ql_lane_kernel, dim3((unsigned)((nk + kQLLanes - 1) / kQLLanes)), dim3(kQLLanes), 0,

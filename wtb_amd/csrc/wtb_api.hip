@@ -163,6 +163,8 @@ int wtb_set_tb_model(wtb_ctx *ctx, int norb, int nvec, const int *rvec, const in
         m.scissor = scissor;
         m.flags = flags;
         c.bands.valid = false;
+        c.fq.on = false;
+        c.fq.bq.valid = false;
         c.bse.valid = false;
         c.sol.valid = false;
         c.dip_valid = false;
@@ -181,6 +183,8 @@ int wtb_set_kpoints(wtb_ctx *ctx, int nk, const double *kpt, const int *ngrid) {
         c.nk = nk;
         for (int i = 0; i < 3; ++i) c.ngrid[i] = ngrid[i];
         c.bands.valid = false;
+        c.fq.on = false;
+        c.fq.bq.valid = false;
         c.bse.valid = false;
         c.sol.valid = false;
         c.dip_valid = false;
@@ -205,6 +209,44 @@ int wtb_hk_eig(wtb_ctx *ctx, int nv, int nc, double *eig, double *vec, int *nocp
             // device usel[(o*nb + j)*nk + k] is exactly Fortran vector(nk, nb, norb)
             copy(c, vec, b.usel.p, sizeof(cd) * (size_t)nk * nb * n, hipMemcpyDeviceToHost);
         }
+        if (nocp) copy(c, nocp, b.nocp.p, sizeof(int) * nk, hipMemcpyDeviceToHost);
+    });
+}
+
+int wtb_hk_eig_q(wtb_ctx *ctx, const double *q, const double *qpt, int nv, int nc, double *eig, double *vec,
+                 int *nocp) {
+    return guarded(ctx, [&](Ctx &c) {
+        FiniteQ &f = c.fq;
+        c.bse.valid = false;
+        c.sol.valid = false;
+        if (!q) {
+            f.on = false;
+            f.bq.valid = false;
+            return;
+        }
+        if (!qpt) throw Error{WTB_ERR_ARG, "null k+Q mesh"};
+        if (c.nk <= 0) throw Error{WTB_ERR_STATE, "no k-points set"};
+        for (int i = 0; i < 3; ++i)
+            if (!std::isfinite(q[i])) throw Error{WTB_ERR_ARG, "non-finite Q"};
+        const int nk = c.nk;
+        std::vector<double> k(3 * (size_t)nk);
+        for (int i = 0; i < nk; ++i)
+            for (int cc = 0; cc < 3; ++cc) k[3 * i + cc] = qpt[i + (size_t)nk * cc];
+        f.kq.alloc(k.size());
+        copy(c, f.kq.p, k.data(), sizeof(double) * k.size(), hipMemcpyHostToDevice);
+        f.on = false;
+        hk_eig_on(c, f.kq.p, nk, nv, nc, f.bq);
+        for (int i = 0; i < 3; ++i) f.q[i] = q[i];
+        f.on = true;
+        Bands &b = f.bq;
+        const int nb = b.nb, n = c.model.norb;
+        if (eig) {
+            std::vector<double> e((size_t)nk * nb);
+            copy(c, e.data(), b.esel.p, sizeof(double) * e.size(), hipMemcpyDeviceToHost);
+            for (int kk = 0; kk < nk; ++kk)
+                for (int j = 0; j < nb; ++j) eig[kk + (size_t)nk * j] = e[(size_t)kk * nb + j];
+        }
+        if (vec) copy(c, vec, b.usel.p, sizeof(cd) * (size_t)nk * nb * n, hipMemcpyDeviceToHost);
         if (nocp) copy(c, nocp, b.nocp.p, sizeof(int) * nk, hipMemcpyDeviceToHost);
     });
 }
@@ -287,6 +329,7 @@ int wtb_oscillator(wtb_ctx *ctx, int nev, int comp_a, int comp_b, double *act) {
         if (nev <= 0 || nev > c.sol.nev) throw Error{WTB_ERR_ARG, "nev exceeds the solved pairs"};
         if (comp_a < 0 || comp_a > 4 || comp_b < 0 || comp_b > 4) throw Error{WTB_ERR_ARG, "component"};
         if (!act) throw Error{WTB_ERR_ARG, "null output"};
+        if (c.bse.finite_q) throw Error{WTB_ERR_UNSUPPORTED, "oscillator strengths of a finite-Q store"};
         const long N = c.bse.N;
         DBuf<double> a;
         a.alloc(nev);
@@ -342,6 +385,7 @@ int wtb_grad_energies(wtb_ctx *ctx, const double *w, wtb_grad *g, double *hop_re
     return guarded(ctx, [&](Ctx &c) {
         if (c.bse.temperature >= 0.0)
             throw Error{WTB_ERR_UNSUPPORTED, "gradients of the temperature BSE are not available"};
+        if (c.bse.finite_q) throw Error{WTB_ERR_UNSUPPORTED, "gradients of the finite-Q BSE are not available"};
         grad_energies(c, w, g, hop_re_grad, hop_im_grad);
     });
 }
@@ -352,6 +396,7 @@ int wtb_grad_spectrum(wtb_ctx *ctx, int comp, double cshift, double gamma, int n
     return guarded(ctx, [&](Ctx &c) {
         if (c.bse.temperature >= 0.0)
             throw Error{WTB_ERR_UNSUPPORTED, "gradients of the temperature BSE are not available"};
+        if (c.bse.finite_q) throw Error{WTB_ERR_UNSUPPORTED, "gradients of the finite-Q BSE are not available"};
         grad_spectrum(c, comp, cshift, gamma, nw, omega, target, loss, eps2, g, hop_re_grad, hop_im_grad);
     });
 }

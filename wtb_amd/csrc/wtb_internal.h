@@ -157,7 +157,22 @@ struct Bse {
     float hscale = 1.f;  // power of two: max |H| / hscale ~ 2^13 for the FP16 split
     double temperature = -1.0;  // of the build (< 0: none)
     DBuf<double> fdeh;   // f_v - f_c of every state when built with a temperature
+    bool finite_q = false;      // built on the k+Q conduction bands
+    DBuf<cd> xa;         // exchange amplitudes <c(k+Q)|v(k)> of every state (Q != 0)
+    DBuf<double> vqx;    // V(Q) of the exchange term
     bool valid = false;
+};
+
+// Finite-Q exciton (bsebnds, src/module_bse.F90:1206-1566): conduction bands
+// on the k+Q mesh, valence bands on the k mesh
+struct FiniteQ {
+    bool on = false;
+    double q[3] = {0, 0, 0};   // Cartesian Q (the reference's q(2:4))
+    DBuf<double> kq;     // [k][3] k+Q mesh
+    Bands bq;            // eigensolve on the k+Q mesh
+    DBuf<double> esel;   // [k][j]: j < nv from the k mesh, conduction from k+Q
+    DBuf<cd> usel;       // layout of Bands::usel
+    DBuf<cf> usel32;
 };
 
 struct Solution {
@@ -226,6 +241,7 @@ struct Ctx {
     int ngrid[3] = {1, 1, 1};
     DBuf<double> kpt;    // [k][3]
     Bands bands;
+    FiniteQ fq;
     Bse bse;
     Solution sol;
     DBuf<cd> dip;        // [a][N], a = x, y, z, sp, sm
@@ -292,6 +308,7 @@ __device__ __forceinline__ double kdotr(double kx, double ky, double kz, const d
 
 // ---- kernels (launch wrappers) ----
 void launch_hk_eig(Ctx &c, int nv, int nc);
+void hk_eig_on(Ctx &c, const double *kpt, int nk, int nv, int nc, Bands &b);
 void launch_bse_build(Ctx &c, const wtb_coulomb &coul, int precision, int convention);
 void launch_optics(Ctx &c, double cshift);
 void launch_oscillator(Ctx &c, int nev, int a, int b, double *act_dev);

@@ -96,6 +96,16 @@ module wtb_hip
             type(c_ptr), value :: eig, vec, nocp
         end function wtb_hk_eig
 
+        ! BSE_BND=T (bsebnds, module_bse.F90:1206-1566): eigsys on the k+Q mesh qpt(nk,3)
+        ! for the conduction states of the following wtb_bse_build (matrizelbsekq);
+        ! q = c_loc(q(2:4)), c_null_ptr switches finite Q off
+        integer(c_int) function wtb_hk_eig_q(ctx, q, qpt, nv, nc, eig, vec, nocp) bind(C, name="wtb_hk_eig_q")
+            import :: c_int, c_ptr
+            type(c_ptr), value :: ctx, q, qpt
+            integer(c_int), value :: nv, nc
+            type(c_ptr), value :: eig, vec, nocp
+        end function wtb_hk_eig_q
+
         integer(c_int) function wtb_get_bands(ctx, energies, vectors) bind(C, name="wtb_get_bands")
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, energies, vectors

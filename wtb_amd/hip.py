@@ -30,7 +30,8 @@ COMPONENTS = {"x": 0, "y": 1, "z": 2, "sp": 3, "sm": 4}
 EXPORTS = ["wtb_create", "wtb_destroy", "wtb_last_error", "wtb_set_tb_model", "wtb_set_kpoints",
            "wtb_hk_eig", "wtb_get_bands", "wtb_bse_build", "wtb_bse_solve", "wtb_optics",
            "wtb_oscillator", "wtb_bse_get_block", "wtb_bse_matvec", "wtb_bse_diag",
-           "wtb_slepc_solver_compat", "wtb_grad_energies", "wtb_grad_spectrum", "wtb_set_temperature"]
+           "wtb_slepc_solver_compat", "wtb_grad_energies", "wtb_grad_spectrum", "wtb_set_temperature",
+           "wtb_hk_eig_q"]
 
 
 class WTBError(RuntimeError):
@@ -83,6 +84,7 @@ def load():
         "wtb_set_tb_model": [P, I, I, P, P, P, P, P, D, D, I],
         "wtb_set_kpoints": [P, I, P, P],
         "wtb_hk_eig": [P, I, I, P, P, P],
+        "wtb_hk_eig_q": [P, P, P, I, I, P, P, P],
         "wtb_get_bands": [P, P, P],
         "wtb_bse_build": [P, C.POINTER(CoulombParams), I, I],
         "wtb_bse_solve": [P, I, D, I, P, P, C.POINTER(Report)],
@@ -182,6 +184,25 @@ class Context:
         vec = np.zeros((self.nk, nb, self.norb), np.complex128, order="F")
         nocp = np.zeros(self.nk, np.int32)
         self._check(self.lib.wtb_hk_eig(self.h, nv, nc, _ptr(eig), _ptr(vec), _ptr(nocp)))
+        return eig, vec, nocp
+
+    def hk_eig_q(self, q, qpt, nv, nc):
+        """Finite-Q exciton (BSE_BND=T): bands on the k+Q mesh qpt (nk, 3) for
+        the conduction states of the following bse_build, which adds the
+        exchange term when |Q| != 0.  q is the Cartesian Q (3,); q None
+        switches finite Q off.  Returns the k+Q (eig, vec, nocp)."""
+        if q is None:
+            self._check(self.lib.wtb_hk_eig_q(self.h, None, None, nv, nc, None, None, None))
+            return None
+        qv = np.ascontiguousarray(q, np.float64).reshape(3)
+        k = np.asfortranarray(qpt, np.float64)
+        if k.shape != (self.nk, 3):
+            raise ValueError("qpt must be (nk, 3)")
+        nb = nv + nc
+        eig = np.zeros((self.nk, nb), order="F")
+        vec = np.zeros((self.nk, nb, self.norb), np.complex128, order="F")
+        nocp = np.zeros(self.nk, np.int32)
+        self._check(self.lib.wtb_hk_eig_q(self.h, _ptr(qv), _ptr(k), nv, nc, _ptr(eig), _ptr(vec), _ptr(nocp)))
         return eig, vec, nocp
 
     def get_bands(self):

@@ -26,7 +26,8 @@ from typing import Optional
 import numpy as np
 
 __all__ = [
-    "TBModel", "read_tb_file", "write_tb_file", "mp_mesh", "recip_vectors",
+    "TBModel", "read_tb_file", "write_tb_file", "mp_mesh", "mp_mesh_q", "recip_vectors",
+    "temperature_gap_shift",
     "honeycomb_two_band", "random_hexagonal", "random_cubic", "workload",
     "WORKLOADS",
 ]
@@ -170,6 +171,20 @@ def mp_mesh(n1: int, n2: int, n3: int, rlat: np.ndarray,
     i, j, l = np.meshgrid(np.arange(n1), np.arange(n2), np.arange(n3), indexing="ij")
     k = (np.outer(i.ravel(), b[0] / n1) + np.outer(j.ravel(), b[1] / n2)
          + np.outer(l.ravel(), b[2] / n3))
+    return k + kshift[None, :]
+
+
+def mp_mesh_q(n1: int, n2: int, n3: int, rlat: np.ndarray, q,
+              shift=(0.0, 0.0, 0.0), pi: float = PI_F32) -> np.ndarray:
+    """The k+Q mesh of the finite-Q exciton, q the Cartesian Q: the points of
+    mp_mesh in the same order, summed as q + i b1/n1 + j b2/n2 + l b3/n3 +
+    shift (src/module_kmesh.F90:281-334)."""
+    b = recip_vectors(rlat, pi)
+    kshift = b.T @ np.asarray(shift, np.float64)
+    i, j, l = np.meshgrid(np.arange(n1), np.arange(n2), np.arange(n3), indexing="ij")
+    k = np.asarray(q, np.float64)[None, :] + np.outer(i.ravel(), b[0] / n1)
+    k = k + np.outer(j.ravel(), b[1] / n2)
+    k = k + np.outer(l.ravel(), b[2] / n3)
     return k + kshift[None, :]
 
 
