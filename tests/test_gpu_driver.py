@@ -215,3 +215,52 @@ def test_temperature_driver_run():
     mine = mine[np.lexsort(mine.T[::-1])]
     theirs = sp[:, :3][np.lexsort(sp[:, :3].T[::-1])]
     assert np.abs(mine - theirs).max() <= 1.01e-4
+
+
+def test_exciton_bands_driver_run():
+    """BSE_BND=T along one Q segment (Gamma -> b1/2, three points) on the
+    honeycomb 6x6 mesh: every eigenvalue of the finite-Q matrix (with the
+    exchange for Q != 0) per Q in bands_bse.dat, the path coordinate and the
+    segment ends of KLABELS-BSE.dat."""
+    say("case driver exciton bands")
+    if not os.path.exists(DRIVER):
+        pytest.fail("wtb_amd/wtb_bse is not built (make -C wtb_amd/fortran)")
+    kdir = tempfile.mkdtemp(prefix="wtbq")
+    kfile = os.path.join(kdir, "kpath_bse.dat")
+    ks = np.array([[0.0, 0.0, 0.0], [0.5, 0.0, 0.0]])
+    with open(kfile, "w") as fh:
+        fh.write("2\n3\n" + "".join(f"{a} {b} {c}\n" for a, b, c in ks))
+    cards = ["BSE_BND= T", f'KPATH_BSE= "{kfile}"', "SYSDIM= \"2D\"", "NGX= 6", "NGY= 6", "NGZ= 1",
+             "NBANDSC= 1", "NBANDSV= 1", "COULOMB_POT= V2DRK", "R_0= 10.0", "EDIEL= 1.0", "EDIEL_T= 1.0",
+             "EDIEL_B= 1.0", "EDIEL_Z= 1.0", "LC= 1.0", "KTOL= 0.001", "PRECISION= FP64"]
+    d, m = run_driver(cards)
+    assert not os.path.exists(os.path.join(d, "bse_opt_diel.dat"))      # BSE= is off
+    lab = load(os.path.join(d, "KLABELS-BSE.dat"))[:, 0]
+    assert np.allclose(lab, [0.0, 1.0], atol=1e-7)
+    with open(os.path.join(d, "log_bse_kpath.dat")) as fh:
+        log = fh.read()
+    assert log.count("progress:") == 3
+    out = load(os.path.join(d, "bands_bse.dat"))
+    N, nq = 36, 3
+    assert out.shape == (N * nq, 2)
+    mesh = (6, 6, 1)
+    b = T.recip_vectors(m.rlat)
+    kc = [(s[0] * b[0] + s[1] * b[1]) + s[2] * b[2] for s in ks]
+    k = T.mp_mesh(*mesh, m.rlat)
+    om = O.Model(m)
+    ref = O.hk_eig(om, k, 1, 1)
+    c = O.coulomb(kind="V2DRK", r0=10.0, ediel=(1.0, 1.0, 1.0), ez=1.0, w=0.0, lc=1.0, ktol=0.001)
+    moved = 0.0
+    for iq in range(nq):
+        q = kc[0] + (kc[1] - kc[0]) * float(iq) / 2.0
+        kq = T.mp_mesh_q(*mesh, m.rlat, q)
+        refq = O.hk_eig(om, kq, 1, 1)
+        H = O.bse_matrix_q(k, ref["eig"], ref["vec"], refq["eig"], refq["vec"], c, mesh, m.rlat, 1, 1, q)
+        w = np.linalg.eigvalsh(H)
+        rows = out[iq::nq]
+        assert np.abs(rows[:, 0] - 0.5 * iq).max() < 1e-6
+        assert np.abs(rows[:, 1] - w).max() < 1e-9, iq
+        H0 = O.bse_matrix_q(k, ref["eig"], ref["vec"], refq["eig"], refq["vec"], c, mesh, m.rlat, 1, 1, q,
+                            exchange=False)
+        moved = max(moved, np.abs(np.linalg.eigvalsh(H0) - w).max())
+    assert moved > 1e-4                # the exchange shifts the Q != 0 levels

@@ -12,7 +12,10 @@
 ! exciton_lifetime.dat.  BSET=T runs the temperature BSE (bsesolvertemp,
 ! module_bse.F90:2649-3263; main.F90:102-120): TEMP= in K, TA= FA | VE | BE
 ! with ST= and PHAVG= for the gap correction, a 'temperature' line in the log
-! and |d|^2 weighted by f_v - f_c.
+! and |d|^2 weighted by f_v - f_c.  BSE_BND=T / BSET_BND=T compute the
+! exciton bands along the Q path of KPATH_BSE= (bsebnds / bsebndstemp,
+! module_bse.F90:1206-2014): log_bse_kpath.dat, bands_bse.dat and
+! KLABELS-BSE.dat.
 !
 ! Cards of this driver beyond the reference's: NEV= (eigenpairs, default 100
 ! as the reference's SLEPc call), PRECISION= (FP64 | MIXED | FP32), KERNEL=
@@ -36,6 +39,10 @@ module wtb_cards
     logical :: bset = .false.
     double precision :: temp = 0.0d0, st = 0.0d0, phavg = 0.0d0
     character(len=2) :: ta = "FA"
+    ! BSE_BND=T / BSET_BND=T (bsebnds / bsebndstemp): exciton bands along the
+    ! Q path of the KPATH_BSE= file
+    logical :: bsekpath = .false., bsetbnd = .false.
+    character(len=70) :: kpathsbse = "non declared file"
 contains
     ! "CARD= value" lines from standard input, defaults of module_input.F90:53-120
     subroutine read_cards()
@@ -76,6 +83,9 @@ contains
             case ("RNMD=");         read(b, *) renorm
             case ("BSE=");          read(b, *) bse
             case ("BSET=");         read(b, *) bset
+            case ("BSE_BND=");      read(b, *) bsekpath
+            case ("BSET_BND=");     read(b, *) bsetbnd
+            case ("KPATH_BSE=");    kpathsbse = trim(adjustl(b))
             case ("TEMP=");         read(b, *) temp
             case ("TA=");           read(b, *) ta
             case ("ST=");           read(b, *) st
@@ -210,6 +220,85 @@ contains
         end do
         if (twod) ng(3) = 1
     end subroutine rk_grid
+
+    ! k+Q mesh (monhkhorst_packq, module_kmesh.F90:281-334): the points of
+    ! mp_mesh, each summed as q + i b1/n1 + j b2/n2 + l b3/n3 + shift
+    subroutine mp_mesh_q(ng, shift, q, kpt)
+        integer, intent(in) :: ng(3)
+        double precision, intent(in) :: shift(3), q(3)
+        double precision, intent(out) :: kpt(:, :)
+        double precision, parameter :: pi = acos(-1._4)
+        double precision :: b1(3), b2(3), b3(3), v23(3), v31(3), v12(3), vol, ks(3)
+        integer :: i, j, l, n
+        v23 = cross(rlat(2, :), rlat(3, :))
+        v31 = cross(rlat(3, :), rlat(1, :))
+        v12 = cross(rlat(1, :), rlat(2, :))
+        vol = abs((rlat(1, 1) * v23(1)) + (rlat(1, 2) * v23(2)) + (rlat(1, 3) * v23(3)))
+        b1 = ((2.0 * pi) / vol) * v23
+        b2 = ((2.0 * pi) / vol) * v31
+        b3 = ((2.0 * pi) / vol) * v12
+        ks = b1 * shift(1) + b2 * shift(2) + b3 * shift(3)
+        n = 0
+        do i = 0, ng(1) - 1
+            do j = 0, ng(2) - 1
+                do l = 0, ng(3) - 1
+                    n = n + 1
+                    kpt(n, :) = q + (b1 / dble(ng(1))) * dble(i) + (b2 / dble(ng(2))) * dble(j) &
+                                + (b3 / dble(ng(3))) * dble(l) + ks
+                end do
+            end do
+        end do
+    end subroutine mp_mesh_q
+
+    ! Q path (kpathbse, module_kmesh.F90:124-193): segments ks(2j-1) -> ks(2j)
+    ! in fractional coordinates, npts points each; qv(:, 1) is the path
+    ! coordinate normalised to 1, qv(:, 2:4) the Cartesian Q.  The segment
+    ! ends go to OUTPUT//KLABELS-BSE.dat as the reference writes them.
+    subroutine kpath_bse(folder, nks, ks, npts, qv)
+        character(len=*), intent(in) :: folder
+        integer, intent(in) :: nks, npts
+        double precision, intent(in) :: ks(:, :)
+        double precision, intent(out) :: qv(:, :)
+        double precision, parameter :: pi = acos(-1._4)
+        double precision :: b(3, 3), vol, kc(nks, 3), kd(nks / 2), tot, acc, t
+        integer :: i, j, c, u, erro
+        b(1, :) = cross(rlat(2, :), rlat(3, :))
+        b(2, :) = cross(rlat(3, :), rlat(1, :))
+        b(3, :) = cross(rlat(1, :), rlat(2, :))
+        vol = abs((rlat(1, 1) * b(1, 1)) + (rlat(1, 2) * b(1, 2)) + (rlat(1, 3) * b(1, 3)))
+        do i = 1, 3
+            b(i, :) = ((2.0 * pi) / vol) * b(i, :)
+        end do
+        do i = 1, nks
+            do c = 1, 3
+                kc(i, c) = ks(i, 1) * b(1, c) + ks(i, 2) * b(2, c) + ks(i, 3) * b(3, c)
+            end do
+        end do
+        tot = 0.0
+        do j = 1, nks / 2
+            kd(j) = sqrt((kc(2 * j, 1) - kc(2 * j - 1, 1))**2 + (kc(2 * j, 2) - kc(2 * j - 1, 2))**2 &
+                         + (kc(2 * j, 3) - kc(2 * j - 1, 3))**2)
+            tot = tot + kd(j)
+        end do
+        kd = kd / tot
+        open(newunit=u, file=trim(folder)//'KLABELS-BSE.dat', status='unknown', iostat=erro)
+        if (erro /= 0) stop "Error opening KLABELS-BSE output file"
+        acc = 0.0
+        write(u, *) real(acc)
+        do j = 1, nks / 2
+            do i = 1, npts
+                t = dble(i - 1)
+                qv((j - 1) * npts + i, 1) = acc + kd(j) * t / dble(npts - 1)
+                do c = 1, 3
+                    qv((j - 1) * npts + i, 1 + c) = kc(2 * j - 1, c) + (kc(2 * j, c) - kc(2 * j - 1, c)) * t &
+                                                    / dble(npts - 1)
+                end do
+            end do
+            acc = acc + kd(j)
+            write(u, *) real(acc)
+        end do
+        close(u)
+    end subroutine kpath_bse
 
     pure function cross(a, b) result(c)
         double precision, intent(in) :: a(3), b(3)
@@ -429,11 +518,13 @@ program wtb_bse
 
     call read_cards()
     if (bse .and. bset) stop "ERROR: Choose BSE=T or BSET=T not both"
-    if (.not. (bse .or. bset)) stop "wtb_bse: only the BSE=T and BSET=T paths are provided"
-    ! gap correction of the temperature path (bsesolvertemp, module_bse.F90:2857-2875),
-    ! applied with the scissor to the conduction bands
+    if (bsekpath .and. bsetbnd) stop "ERROR: Choose BSE_BND=T or BSET_BND=T not both"
+    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd)) &
+        stop "wtb_bse: only the BSE=T, BSET=T, BSE_BND=T and BSET_BND=T paths are provided"
+    ! gap correction of the temperature paths (bsesolvertemp, module_bse.F90:2857-2875;
+    ! bsebndstemp :1740-1756), applied with the scissor to the conduction bands
     tcor = 0.0d0
-    if (bset) then
+    if (bset .or. bsetbnd) then
         select case (trim(ta))
         case ("VE"); tcor = gap_shift_varshni(st, phavg, temp)
         case ("BE"); tcor = gap_shift_bose(st, phavg, temp)
@@ -480,6 +571,8 @@ program wtb_bse
         stop
     end select
     coul = wtb_coulomb(kind, r0, ediel, ez, w, lc, ktol)
+    if (bsekpath .or. bsetbnd) call exciton_bands()
+    if (.not. (bse .or. bset)) stop
 
     ! ---- GPU path: eigsys loop, optsp, matrizelbse, slepc_solver, dielbse*
     call wtb_check(ctx, wtb_create(ctx, int(device, c_int)), "wtb_create")
@@ -606,6 +699,75 @@ program wtb_bse
         ' eV, max rel residual ', rep%max_rel_residual
 
 contains
+
+    ! BSE_BND=T / BSET_BND=T (bsebnds, bsebndstemp, module_bse.F90:1206-2014):
+    ! for every Q of the path the bands on the k+Q mesh, the kernel of
+    ! matrizelbsekq (exchange for Q != 0; with BSET_BND the factors of
+    ! matrizelbsekqtemp) and its eigenvalues, written to bands_bse.dat as
+    ! (path coordinate, energy) per exciton with a blank line between excitons.
+    ! Every eigenvalue as the reference's ZHEEV when N <= 1024, else the
+    ! lowest NEV= of them.
+    subroutine exciton_bands()
+        type(c_ptr) :: bctx
+        integer :: u, ul, ub, erro, nks, npts, nq, iq, i, dimb, nevb
+        double precision, allocatable :: ks(:, :), qv(:, :), exk(:, :), wq(:)
+        double precision, allocatable, target :: kq(:, :), qc(:)
+        open(newunit=u, file=trim(kpathsbse), status='old', iostat=erro)
+        if (erro /= 0) stop "Error opening bse-kpath input file"
+        read(u, *) nks
+        read(u, *) npts
+        allocate(ks(nks, 3))
+        do i = 1, nks
+            read(u, *) ks(i, 1), ks(i, 2), ks(i, 3)
+        end do
+        close(u)
+        if (nks < 2 .or. npts < 2) stop "wtb_bse: KPATH_BSE= needs at least one segment of two points"
+        nq = (nks / 2) * npts
+        allocate(qv(nq, 4))
+        call kpath_bse(outputfolder, nks, ks, npts, qv)
+        dimb = nk * nc * nv
+        nevb = dimb
+        if (dimb > 1024) nevb = min(nev, dimb)
+        allocate(kq(nk, 3), qc(3), wq(dimb), exk(nevb, nq))
+        open(newunit=ul, file=trim(outputfolder)//"/log_bse_kpath.dat", status='unknown', iostat=erro)
+        if (erro /= 0) stop "Error opening log_bse_kpath output file"
+        open(newunit=ub, file=trim(outputfolder)//"/bands_bse.dat", status='unknown', iostat=erro)
+        if (erro /= 0) stop "Error opening bands_bse output file"
+        if (bsetbnd) write(ul, *) 'temperature', temp
+        write(ul, *) 'grid:', ngrid(1), ngrid(2), ngrid(3)
+        write(ul, *) 'ktol-coulomb:', ktol
+        write(ul, *) 'kmesh shift:', mshift(1), mshift(2), mshift(3)
+        write(ul, *) 'conduction bands, nc:', nc, '  ', 'valence bands, nv:', nv
+        write(ul, *) 'number of kpoints in the path:', '   ', nq
+        call wtb_check(bctx, wtb_create(bctx, int(device, c_int)), "wtb_create")
+        call wtb_check(bctx, wtb_set_tb_model(bctx, norb, nvec, rvec, ffactor, hop, ihop, rlat, efermi, scs + tcor, &
+                       flags), "wtb_set_tb_model")
+        call wtb_check(bctx, wtb_set_kpoints(bctx, nk, kpt, ngrid), "wtb_set_kpoints")
+        call wtb_check(bctx, wtb_hk_eig(bctx, nv, nc, c_null_ptr, c_null_ptr, c_null_ptr), "wtb_hk_eig")
+        write(ul, *) "quantum numbers for exciton basis set finished"
+        if (bsetbnd) call wtb_check(bctx, wtb_set_temperature(bctx, temp), "wtb_set_temperature")
+        do iq = 1, nq
+            qc = qv(iq, 2:4)
+            call mp_mesh_q(ngrid, mshift, qc, kq)
+            call wtb_check(bctx, wtb_hk_eig_q(bctx, c_loc(qc), c_loc(kq), nv, nc, c_null_ptr, c_null_ptr, &
+                           c_null_ptr), "wtb_hk_eig_q")
+            call wtb_check(bctx, wtb_bse_build(bctx, coul, prec, conv), "wtb_bse_build")
+            call wtb_check(bctx, wtb_bse_solve(bctx, nevb, 1.0d-8, 200, wq, c_null_ptr, c_null_ptr), &
+                           "wtb_bse_solve")
+            exk(:, iq) = wq(1:nevb)
+            write(ul, *) 'progress:', iq, '/', nq
+            flush(ul)
+        end do
+        call wtb_check(bctx, wtb_destroy(bctx), "wtb_destroy")
+        do i = 1, nevb
+            do iq = 1, nq
+                write(ub, *) real(qv(iq, 1)), exk(i, iq)
+            end do
+            write(ub, *)
+        end do
+        close(ub)
+        close(ul)
+    end subroutine exciton_bands
 
     ! f_v - f_c at T kelvin, Fermi level 0 (fermidisteh, module_math.F90:410-421):
     ! the Boltzmann constant is a default-real literal there, and T = 0 is 1E-8
