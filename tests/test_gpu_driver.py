@@ -184,6 +184,9 @@ def test_temperature_driver_run():
     with open(os.path.join(d, "log_bse-diel.dat")) as fh:
         first = fh.readline().split()
     assert first[0] == "temperature" and float(first[1]) == temp
+    # no lifetimes on the temperature path (main.F90:187-190)
+    assert not os.path.exists(os.path.join(d, "exciton_lifetime.dat"))
+    assert os.path.exists(os.path.join(d, "sp_diel_xx.dat"))
     tcor = T.temperature_gap_shift("VE", st, ph, temp)
     say("gap correction", tcor)
     assert -2e-3 < tcor < -5e-4
@@ -264,3 +267,54 @@ def test_exciton_bands_driver_run():
                             exchange=False)
         moved = max(moved, np.abs(np.linalg.eigvalsh(H0) - w).max())
     assert moved > 1e-4                # the exchange shifts the Q != 0 levels
+
+
+@pytest.mark.filterwarnings("ignore::RuntimeWarning")      # 0/0 on identically zero components
+def test_single_particle_and_polarised_spectra(honeycomb_run):
+    """SPEC=T with BSE: the single-particle chains (spdielraw, spdielrawpol on
+    sp_optics*.dat) and, with CPOL=T, bsedielrawpol on bse_opt_diel-pol.dat,
+    each against the oracle spectrum of the driver's own 4-decimal file; their
+    optical constants and the polarised lifetimes from the closed forms."""
+    d, m = honeycomb_run
+    say("case driver single-particle and polarised spectra")
+    vc = abs(np.dot(np.cross(m.rlat[0], m.rlat[1]), m.rlat[2]))
+    spinf = 2.0
+    sets = [("sp_optics.dat", "sp_diel_", ("xx", "yy", "zz", "xy", "xz", "yz"), 3),
+            ("sp_optics-pol.dat", "sp_diel-pol_", ("x", "y", "z", "sp", "sm"), 5),
+            ("bse_opt_diel-pol.dat", "bse_diel-pol_", ("x", "y", "z", "sp", "sm"), 5)]
+    for src, prefix, tags, ndiag in sets:
+        tr = load(os.path.join(d, src))
+        for col, tag in enumerate(tags, start=1):
+            got = load(os.path.join(d, f"{prefix}{tag}.dat"))
+            exp = O.spectrum(tr[:, 0], tr[:, col], col <= ndiag, (12, 12, 1), vc, 3.0, 7.0, 400, 0.05, renorm=True)
+            assert got.shape == (400, 3), (prefix, tag)
+            scale = max(1.0, np.abs(exp[:, 1:]).max() * spinf)
+            assert np.abs(got[:, 0] - exp[:, 0]).max() < 1e-12
+            assert np.abs(got[:, 1] - np.float32(spinf * exp[:, 1])).max() < 1e-6 * scale, (prefix, tag)
+            assert np.abs(got[:, 2] - np.float32(spinf * exp[:, 2])).max() < 1e-6 * scale, (prefix, tag)
+    # optical constants of the single-particle and polarised sets
+    for inp, outp, suffix, tags in (("sp_diel_", "sp_", "", ("xx", "yy", "zz", "xy", "xz", "yz")),
+                                    ("sp_diel-pol_", "sp_", "-pol", ("x", "y", "z", "sp", "sm")),
+                                    ("bse_diel-pol_", "bse_", "-pol", ("x", "y", "z", "sp", "sm"))):
+        e = [load(os.path.join(d, f"{inp}{t}.dat")) for t in tags]
+        e1 = np.stack([x[:, 1] for x in e], 1)
+        e2 = np.stack([x[:, 2] for x in e], 1)
+        mod = np.sqrt(e1 * e1 + e2 * e2)
+        n = np.sqrt((mod + e1) / 2)
+        got = load(os.path.join(d, f"{outp}refractive_index{suffix}.dat"))
+        assert got.shape == (400, len(tags) + 1)
+        assert np.abs(got[:, 1:] - n).max() <= 5.1e-5 + 1e-7 * np.abs(n).max(), (outp, suffix)
+        loss = e2 / (e1 * e1 + e2 * e2)
+        got = load(os.path.join(d, f"{outp}en_loss_func{suffix}.dat"))
+        ok = np.isfinite(loss)
+        assert np.abs(got[:, 1:][ok] - loss[ok]).max() <= 5.1e-5 + 1e-7 * np.abs(loss[ok]).max()
+    # polarised lifetimes: the same closed form per column, the first on x + y + z
+    pol = load(os.path.join(d, "bse_opt_diel-pol.dat"))
+    lt = load(os.path.join(d, "exciton_lifetime-pol.dat"))
+    assert lt.shape == (144, 7)
+    cfe, hbar = float(np.float32(7.297e-3)), float(np.float32(6.582e-16))
+    a2 = np.linalg.norm(np.cross(m.rlat[0], m.rlat[1]))
+    for col, f in ((1, pol[:, 1] + pol[:, 2] + pol[:, 3]), (5, pol[:, 4]), (6, pol[:, 5])):
+        sel = f > 1e-3
+        ref = (144 * a2 * hbar) / ((8.0 * PI32 * cfe * pol[:, 0]) * f)
+        assert np.allclose(lt[sel, col], ref[sel], rtol=1e-3), col

@@ -515,6 +515,8 @@ program wtb_bse
     integer, allocatable :: order(:)
     double precision :: egap, spinf, tcor
     double precision, allocatable :: fdeh(:)
+    character(len=2), parameter :: tens(6) = ["xx", "yy", "zz", "xy", "xz", "yz"]
+    character(len=2), parameter :: pols(5) = ["x ", "y ", "z ", "sp", "sm"]
 
     call read_cards()
     if (bse .and. bset) stop "ERROR: Choose BSE=T or BSET=T not both"
@@ -688,12 +690,23 @@ program wtb_bse
     close(302)
 
     ! ---- spectra from the written (4-decimal) file, optical constants, lifetimes
+    ! (main.F90:183-223: with BSE the single-particle chains run too, and the
+    ! lifetimes only without a temperature)
     if (spec) then
         spinf = 1.0
         if (systype == "NP") spinf = 2.0
-        call diel_files(spinf)
-        call optprop_files()
-        call lifetime_file()
+        call diel_set("bse_opt_diel.dat", "bse_diel_", tens, spinf)
+        call optprop_set("bse_diel_", tens, "bse_", "")
+        if (.not. bset) call lifetime_file()
+        if (cpol) then
+            call diel_set("bse_opt_diel-pol.dat", "bse_diel-pol_", pols, spinf)
+            call optprop_set("bse_diel-pol_", pols, "bse_", "-pol")
+            if (.not. bset) call lifetime_pol_file()
+        end if
+        call diel_set("sp_optics.dat", "sp_diel_", tens, spinf)
+        call optprop_set("sp_diel_", tens, "sp_", "")
+        call diel_set("sp_optics-pol.dat", "sp_diel-pol_", pols, spinf)
+        call optprop_set("sp_diel-pol_", pols, "sp_", "-pol")
     end if
     write(*, '(A,I0,A,I0,A,F12.6,A,ES10.3)') 'wtb_bse: N=', dimbse, ' nev=', nevs, ' E1=', ws(1), &
         ' eV, max rel residual ', rep%max_rel_residual
@@ -794,105 +807,151 @@ contains
         gap_shift_bose = -(2.0 * s / (exp(ph / t) - 1.0))
     end function gap_shift_bose
 
-    ! bsedielraw (module_bse.F90:835-1036): re-reads bse_opt_diel.dat
-    subroutine diel_files(spinf)
+    ! Spectra of one transition file (optdiel + imagrenorm per column), each
+    ! column m written to OUTPUT/<prefix><tag(m)>.dat:
+    !   bsedielraw    (module_bse.F90:835-1036)   bse_opt_diel.dat     -> bse_diel_{xx,yy,zz,xy,xz,yz}
+    !   bsedielrawpol (:1043-1199)                bse_opt_diel-pol.dat -> bse_diel-pol_{x,y,z,sp,sm}
+    !   spdielraw     (:4090-4308)                sp_optics.dat        -> sp_diel_{xx,yy,zz,xy,xz,yz}
+    !   spdielrawpol  (:4310-4524)                sp_optics-pol.dat    -> sp_diel-pol_{x,y,z,sp,sm}
+    ! every file re-read with its 4-decimal values, as the reference does; the
+    ! off-diagonal tensor components (xy, xz, yz) take optdiel's non-diagonal form.
+    subroutine diel_set(infile, prefix, tag, spinf)
+        character(len=*), intent(in) :: infile, prefix
+        character(len=2), intent(in) :: tag(:)
         double precision, intent(in) :: spinf
         character(len=2) :: aread
         double precision, allocatable :: e(:), f(:, :), d(:, :, :)
         double precision :: vc, elux
-        integer :: u, j, m, npts
-        character(len=2), parameter :: tag(6) = ["xx", "yy", "zz", "xy", "xz", "yz"]
-        integer, parameter :: unit_of(6) = [300, 303, 305, 301, 302, 304]
+        integer :: u, uo, j, m, npts, nm
+        nm = size(tag)
         npts = int(numbse)
-        allocate(e(dimbse), f(dimbse, 6), d(npts, 3, 6))
-        open(newunit=u, file=trim(outputfolder)//"/bse_opt_diel.dat", status='unknown', iostat=erro)
-        if (erro /= 0) stop "Error opening bse_opt_diel input file"
+        allocate(e(dimbse), f(dimbse, nm), d(npts, 3, nm))
+        open(newunit=u, file=trim(outputfolder)//"/"//infile, status='unknown', iostat=erro)
+        if (erro /= 0) stop "Error opening "//infile//" input file"
         read(u, *) aread
         do j = 1, dimbse
-            read(u, *) e(j), f(j, 1), f(j, 2), f(j, 3), f(j, 4), f(j, 5), f(j, 6)
+            read(u, *) e(j), f(j, :)
         end do
         close(u)
         vc = vcell3d()
         do j = 1, npts
             elux = ebse0 + dble(((ebsef - ebse0) * (j - 1)) / (numbse - 1.))
-            do m = 1, 6
+            do m = 1, nm
                 d(j, 1, m) = elux
-                call optdiel(m <= 3, vc, ngrid, elux, e, f(:, m), cshift, d(j, 2, m), d(j, 3, m))
+                call optdiel(nm == 5 .or. m <= 3, vc, ngrid, elux, e, f(:, m), cshift, d(j, 2, m), d(j, 3, m))
             end do
         end do
         if (renorm) then
-            do m = 1, 6
+            do m = 1, nm
                 call imagrenorm(e, d(:, :, m))
             end do
         end if
-        do m = 1, 6
-            open(unit=unit_of(m), file=trim(outputfolder)//"/bse_diel_"//tag(m)//".dat", status='unknown', &
+        do m = 1, nm
+            open(newunit=uo, file=trim(outputfolder)//"/"//prefix//trim(tag(m))//".dat", status='unknown', &
                  iostat=erro)
-            if (erro /= 0) stop "Error opening bse_diel output file"
-            write(unit_of(m), *) "#", "  ", "energy", "  ", "real", "  ", "imag"
+            if (erro /= 0) stop "Error opening "//prefix//" output file"
+            write(uo, *) "#", "  ", "energy", "  ", "real", "  ", "imag"
             do j = 1, npts
-                write(unit_of(m), *) d(j, 1, m), real(spinf * d(j, 2, m)), real(spinf * d(j, 3, m))
+                write(uo, *) d(j, 1, m), real(spinf * d(j, 2, m)), real(spinf * d(j, 3, m))
             end do
-            close(unit_of(m))
+            close(uo)
         end do
-    end subroutine diel_files
+    end subroutine diel_set
 
-    ! bseoptprop (module_bse.F90:3766-3931): re-reads the six bse_diel files
-    subroutine optprop_files()
+    ! Optical constants of one set of dielectric files (refractive index,
+    ! extinction, reflectivity, absorption, energy loss), OUTPUT/<outp>NAME<suffix>.dat:
+    !   bseoptprop (module_bse.F90:3766-3931), bseoptproppol (:3937-4084),
+    !   spoptprop (:3438-3602), spoptproppol (:3608-3760)
+    subroutine optprop_set(inprefix, tag, outp, suffix)
+        character(len=*), intent(in) :: inprefix, outp, suffix
+        character(len=2), intent(in) :: tag(:)
         character(len=2) :: aread
-        character(len=2), parameter :: tag(6) = ["xx", "yy", "zz", "xy", "xz", "yz"]
-        character(len=25), parameter :: outname(5) = [character(len=25) :: "bse_refractive_index.dat", &
-            "bse_extinction_coef.dat", "bse_reflectibility.dat", "bse_absorption_coef.dat", "bse_en_loss_func.dat"]
-        double precision, allocatable :: en(:), re(:, :), im(:, :)
-        double precision :: flag, val(6)
-        integer :: u(6), uo(5), j, m, q, npts
+        character(len=16), parameter :: outname(5) = [character(len=16) :: "refractive_index", &
+            "extinction_coef", "reflectibility", "absorption_coef", "en_loss_func"]
+        double precision, allocatable :: en(:), re(:, :), im(:, :), val(:)
+        double precision :: flag
+        character(len=16) :: fmt
+        integer :: u(6), uo(5), j, m, q, npts, nm
+        nm = size(tag)
         npts = int(numbse)
-        allocate(en(npts), re(npts, 6), im(npts, 6))
-        do m = 1, 6
-            open(newunit=u(m), file=trim(outputfolder)//"/bse_diel_"//tag(m)//".dat", status='old', iostat=erro)
-            if (erro /= 0) stop "Error opening bse_diel input file"
+        allocate(en(npts), re(npts, nm), im(npts, nm), val(nm))
+        write(fmt, "(A,I0,A)") "(", nm + 1, "F15.4)"
+        do m = 1, nm
+            open(newunit=u(m), file=trim(outputfolder)//"/"//inprefix//trim(tag(m))//".dat", status='old', &
+                 iostat=erro)
+            if (erro /= 0) stop "Error opening "//inprefix//" input file"
             read(u(m), *) aread
         end do
         do j = 1, npts
-            do m = 1, 6
+            do m = 1, nm
                 read(u(m), *) flag, re(j, m), im(j, m)
                 if (m == 1) en(j) = flag
             end do
         end do
-        do m = 1, 6
+        do m = 1, nm
             close(u(m))
         end do
         do q = 1, 5
-            open(newunit=uo(q), file=trim(outputfolder)//"/"//trim(outname(q)), status='unknown', iostat=erro)
+            open(newunit=uo(q), file=trim(outputfolder)//"/"//outp//trim(outname(q))//suffix//".dat", &
+                 status='unknown', iostat=erro)
             if (erro /= 0) stop "Error opening optical property output file"
-            write(uo(q), *) "#", "  ", "energy", "  ", "xx", "  ", "yy", "  ", "zz", "  ", "xy", "  ", "xz", "  ", "yz"
+            if (nm == 6) then
+                write(uo(q), *) "#", "  ", "energy", "  ", "xx", "  ", "yy", "  ", "zz", "  ", "xy", "  ", "xz", &
+                    "  ", "yz"
+            else
+                write(uo(q), *) "#", "  ", "energy", "  ", "x", "  ", "y", "  ", "z", "  ", "sp", "  ", "sm"
+            end if
         end do
         do j = 1, npts
-            do m = 1, 6
+            do m = 1, nm
                 val(m) = refractive(re(j, m), im(j, m))
             end do
-            write(uo(1), "(7F15.4)") en(j), val
-            do m = 1, 6
+            write(uo(1), fmt) en(j), val
+            do m = 1, nm
                 val(m) = extinction(re(j, m), im(j, m))
             end do
-            write(uo(2), "(7F15.4)") en(j), val
-            do m = 1, 6
+            write(uo(2), fmt) en(j), val
+            do m = 1, nm
                 val(m) = reflectivity(re(j, m), im(j, m))
             end do
-            write(uo(3), "(7F15.4)") en(j), val
-            do m = 1, 6
+            write(uo(3), fmt) en(j), val
+            do m = 1, nm
                 val(m) = absorption(re(j, m), im(j, m), en(j))
             end do
-            write(uo(4), "(7F15.4)") en(j), val
-            do m = 1, 6
+            write(uo(4), fmt) en(j), val
+            do m = 1, nm
                 val(m) = energy_loss(re(j, m), im(j, m))
             end do
-            write(uo(5), "(7F15.4)") en(j), val
+            write(uo(5), fmt) en(j), val
         end do
         do q = 1, 5
             close(uo(q))
         end do
-    end subroutine optprop_files
+    end subroutine optprop_set
+
+    ! lifetimepol (module_bse.F90:3353-3432): re-reads bse_opt_diel-pol.dat
+    subroutine lifetime_pol_file()
+        character(len=70) :: cflag
+        double precision :: fo(6), lft(6), vc2
+        integer :: u, uo, j, m
+        vc2 = vcell2d()
+        open(newunit=u, file=trim(outputfolder)//"/bse_opt_diel-pol.dat", status='old', iostat=erro)
+        if (erro /= 0) stop "Error opening bse_opt_diel-pol input file"
+        open(newunit=uo, file=trim(outputfolder)//"/exciton_lifetime-pol.dat", status='unknown', iostat=erro)
+        if (erro /= 0) stop "Error opening exciton_lifetime-pol output file"
+        read(u, *) cflag
+        write(uo, *) "#energy(eV) sum-lft(s)  x-lft(s) y-lft(s) z-lft(s) sp-lft(s) sm-lft(s)"
+        do j = 1, dimbse
+            read(u, *) fo
+            lft(1) = lifetime(sysdim, ngrid, vc2, rlat(1, 1), fo(2) + fo(3) + fo(4), fo(1))
+            do m = 2, 6
+                lft(m) = lifetime(sysdim, ngrid, vc2, rlat(1, 1), fo(m), fo(1))
+            end do
+            write(uo, "(1F15.4,6E15.4)") fo(1), lft
+        end do
+        close(u)
+        close(uo)
+    end subroutine lifetime_pol_file
 
     ! lifetime (module_bse.F90:3271-3349): re-reads bse_opt_diel.dat
     subroutine lifetime_file()
