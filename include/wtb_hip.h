@@ -143,6 +143,18 @@ int wtb_hk_eig(wtb_ctx *ctx, int nv, int nc, double *eig, double *vec, int *nocp
 int wtb_hk_eig_q(wtb_ctx *ctx, const double *q, const double *qpt, int nv, int nc, double *eig, double *vec,
                  int *nocp);
 
+/* Density of states (dostool, module_tools.F90:390-651) of all bands of the
+ * last wtb_hk_eig: en(ndos) is the reference's grid from min(0, lowest band)
+ * to max(0, highest band); dos(ndos,4) column-major holds tdos, spin-up,
+ * spin-down and the orbital-weighted (layercont, :153-175) density, each
+ * (1/nk) sum over states of weight * gaussian(E - E_n(k), sigma) (endos
+ * :103-149), before the spin factor of NP models.  orbweight(norb) may be
+ * NULL (all weights 1, as when ORB_W= names no file).  spin_mode (spincont
+ * :41-99): 0 none (NP), 1 first/second half of the orbitals (SP, and SOC
+ * with DFT=V), 2 odd/even orbitals (other SOC). */
+int wtb_dos(wtb_ctx *ctx, int ndos, double sigma, const double *orbweight, int spin_mode, double *en,
+            double *dos);
+
 /* Full spectrum of every H(k): energies(nk,norb) (scissor applied) and
  * eigenvectors(nk,norb,norb) with ev(k,o,n) = component o of band n. */
 int wtb_get_bands(wtb_ctx *ctx, double *energies, double *vectors);

@@ -267,6 +267,36 @@ def bse_matrix_q(kpt, eig, vec, eigq, vecq, c, ngrid, rlat, nc, nv, q, temp=None
     return H
 
 
+def dos(eall, zall, sigma, ndos, orbweight=None, spin_mode=0):
+    """dostool (src/module_tools.F90:390-651): the grid from min(0, lowest
+    band) to max(0, highest band) and, per grid energy, (1/nk) sum over all
+    states of weight * gaussian (endos :103-149, gaussian
+    src/module_math.F90:330-339 with the single-precision pi); weights 1,
+    spin up / down (spincont :41-99; spin_mode 1 halves, 2 odd/even) and
+    sum_o w_o |z_o|^2 (layercont :153-175).  zall[k, o, n] is component o of
+    band n.  Returns en (ndos,), d (ndos, 4)."""
+    eall = np.asarray(eall, float)
+    nk, n = eall.shape
+    e0 = min(0.0, float(eall[:, 0].min()))
+    e1 = max(0.0, float(eall[:, -1].max()))
+    en = e0 + ((e1 - e0) * np.arange(ndos)) / (ndos - 1.0)
+    p = np.abs(np.asarray(zall)) ** 2
+    w = np.ones(n) if orbweight is None else np.asarray(orbweight, float)
+    h = n // 2
+    up = np.zeros((nk, n))
+    dn = np.zeros((nk, n))
+    if spin_mode == 1:
+        up, dn = p[:, :h, :].sum(1), p[:, h:2 * h, :].sum(1)
+    elif spin_mode == 2:
+        up, dn = p[:, 0:2 * h:2, :].sum(1), p[:, 1:2 * h:2, :].sum(1)
+    lw = np.einsum("kon,o->kn", p, w)
+    pi32 = float(np.float32(np.pi))
+    g = (1.0 / (sigma * math.sqrt(2.0 * pi32))) * np.exp(-((en[:, None] - eall.ravel()[None, :]) ** 2)
+                                                       / (2.0 * sigma * sigma))
+    wts = np.stack([np.ones(nk * n), up.ravel(), dn.ravel(), lw.ravel()], 1)
+    return en, (g @ wts) / nk
+
+
 def dipoles(model: Model, kpt, eig, vec, nc, nv, cshift):
     nk = kpt.shape[0]
     N = nk * nc * nv

@@ -318,3 +318,26 @@ def test_single_particle_and_polarised_spectra(honeycomb_run):
         sel = f > 1e-3
         ref = (144 * a2 * hbar) / ((8.0 * PI32 * cfe * pol[:, 0]) * f)
         assert np.allclose(lt[sel, col], ref[sel], rtol=1e-3), col
+
+
+def test_dos_driver_run():
+    """DOS=T alone (no BSE): dos.dat of the honeycomb model against the
+    oracle density (spin factor 2 of an NP model) and its running trapezoid
+    integral, written one row late as dostool writes it."""
+    say("case driver dos")
+    if not os.path.exists(DRIVER):
+        pytest.fail("wtb_amd/wtb_bse is not built (make -C wtb_amd/fortran)")
+    cards = ["DOS= T", "NEDOS= 401", "SIGMA= 0.1", "NGX= 12", "NGY= 12", "NGZ= 1"]
+    d, m = run_driver(cards)
+    assert not os.path.exists(os.path.join(d, "bse_opt_diel.dat"))
+    out = load(os.path.join(d, "dos.dat"))
+    assert out.shape == (401, 4)
+    k = T.mp_mesh(12, 12, 1, m.rlat)
+    r = O.hk_eig(O.Model(m), k, 1, 1, full=True)
+    en, dd = O.dos(r["eall"], r["zall"], 0.1, 401)
+    assert np.abs(out[:, 0] - en).max() <= 5.1e-5
+    assert np.abs(out[:, 1] - 2.0 * dd[:, 0]).max() <= 5.1e-5
+    assert np.abs(out[:, 3] - 2.0 * dd[:, 3]).max() <= 5.1e-5
+    tdos = 2.0 * dd[:, 0]
+    run = np.r_[0.0, 0.0, np.cumsum(0.5 * (tdos[1:-1] + tdos[:-2]) * (en[-1] - en[0]) / 400.0)]
+    assert np.abs(out[:, 2] - run).max() <= 1e-4 * max(1.0, run.max())

@@ -502,3 +502,41 @@ def test_tb_file_roundtrip(tmp_path):
     assert np.abs(m2.hop - m.hop).max() < 1e-15
     assert abs(m2.efermi - m.efermi) < 1e-15
     assert m.hermiticity_error() < 1e-15
+
+
+# ----------------------------- DOS ----------------------------------------
+
+def test_dos_flat_bands_and_weight_identities():
+    """Two flat bands at -D/2 and +D/2 (onsite only): the grid runs exactly
+    from -D/2 to D/2 and its end points carry norm (1 + exp(-D^2 / 2 s^2));
+    with unit weights on all orbitals the weighted density is the total, and
+    up + down is the total for either spin ordering."""
+    D, s = 2.0, 0.3
+    m = T.honeycomb_two_band(delta=D, t=0.0)
+    k = O.mp_mesh(4, 4, 1, m.rlat)
+    r = O.hk_eig(O.Model(m, flags=0), k, 1, 1, full=True)
+    en, d = O.dos(r["eall"], r["zall"], s, 41)
+    assert en[0] == -D / 2 and abs(en[-1] - D / 2) < 1e-15
+    norm = 1.0 / (s * math.sqrt(2.0 * float(np.float32(np.pi))))
+    assert abs(d[0, 0] - norm * (1.0 + math.exp(-D * D / (2 * s * s)))) < 1e-14
+    assert abs(d[20, 0] - 2.0 * norm * math.exp(-(D / 2) ** 2 / (2 * s * s))) < 1e-14
+    mh = hex_model((4, 4, 1))
+    kh = O.mp_mesh(4, 4, 1, mh.rlat)
+    rh = O.hk_eig(O.Model(mh, flags=0), kh, 2, 2, full=True)
+    for mode in (1, 2):
+        en, d = O.dos(rh["eall"], rh["zall"], 0.1, 301, spin_mode=mode)
+        assert np.abs(d[:, 1] + d[:, 2] - d[:, 0]).max() < 1e-12 * d[:, 0].max()
+        assert np.abs(d[:, 3] - d[:, 0]).max() < 1e-12 * d[:, 0].max()
+    # weights of the first half of the orbitals: the spin-up density of the halves ordering
+    w = np.r_[np.ones(11), np.zeros(11)]
+    en, d = O.dos(rh["eall"], rh["zall"], 0.1, 301, orbweight=w, spin_mode=1)
+    assert np.abs(d[:, 3] - d[:, 1]).max() < 1e-12 * d[:, 0].max()
+    # the integral over the window: sum over states of the Gaussian mass
+    # inside [e0, e1] (erf), against the trapezoid of the curve
+    en, d = O.dos(rh["eall"], rh["zall"], 0.05, 4001)
+    z0 = (en[0] - rh["eall"].ravel()) / (0.05 * math.sqrt(2.0))
+    z1 = (en[-1] - rh["eall"].ravel()) / (0.05 * math.sqrt(2.0))
+    from scipy.special import erf
+    mass = 0.5 * (erf(z1) - erf(z0)).sum() / 16
+    assert 20.0 < mass < 22.0
+    assert abs(np.trapezoid(d[:, 0], en) - mass) < 1e-3 * mass

@@ -251,6 +251,11 @@ int wtb_hk_eig_q(wtb_ctx *ctx, const double *q, const double *qpt, int nv, int n
     });
 }
 
+int wtb_dos(wtb_ctx *ctx, int ndos, double sigma, const double *orbweight, int spin_mode, double *en,
+            double *dos) {
+    return guarded(ctx, [&](Ctx &c) { launch_dos(c, ndos, sigma, orbweight, spin_mode, en, dos); });
+}
+
 int wtb_get_bands(wtb_ctx *ctx, double *energies, double *vectors) {
     return guarded(ctx, [&](Ctx &c) {
         Bands &b = c.bands;

@@ -43,6 +43,12 @@ module wtb_cards
     ! Q path of the KPATH_BSE= file
     logical :: bsekpath = .false., bsetbnd = .false.
     character(len=70) :: kpathsbse = "non declared file"
+    ! DOS=T (dostool): NEDOS= points, SIGMA= Gaussian width, ORB_W= orbital
+    ! weights file, DFT= spin ordering of SOC models
+    logical :: doscalc = .false.
+    double precision :: numdos = 6001.0d0, sme = 0.08d0
+    character(len=70) :: orbw = "non declared file"
+    character(len=1) :: dft = "V"
 contains
     ! "CARD= value" lines from standard input, defaults of module_input.F90:53-120
     subroutine read_cards()
@@ -84,6 +90,11 @@ contains
             case ("BSE=");          read(b, *) bse
             case ("BSET=");         read(b, *) bset
             case ("BSE_BND=");      read(b, *) bsekpath
+            case ("DOS=");          read(b, *) doscalc
+            case ("NEDOS=");        read(b, *) numdos
+            case ("SIGMA=");        read(b, *) sme
+            case ("ORB_W=");        orbw = trim(adjustl(b))
+            case ("DFT=");          read(b, *) dft
             case ("BSET_BND=");     read(b, *) bsetbnd
             case ("KPATH_BSE=");    kpathsbse = trim(adjustl(b))
             case ("TEMP=");         read(b, *) temp
@@ -521,8 +532,8 @@ program wtb_bse
     call read_cards()
     if (bse .and. bset) stop "ERROR: Choose BSE=T or BSET=T not both"
     if (bsekpath .and. bsetbnd) stop "ERROR: Choose BSE_BND=T or BSET_BND=T not both"
-    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd)) &
-        stop "wtb_bse: only the BSE=T, BSET=T, BSE_BND=T and BSET_BND=T paths are provided"
+    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd .or. doscalc)) &
+        stop "wtb_bse: only the DOS=T, BSE=T, BSET=T, BSE_BND=T and BSET_BND=T paths are provided"
     ! gap correction of the temperature paths (bsesolvertemp, module_bse.F90:2857-2875;
     ! bsebndstemp :1740-1756), applied with the scissor to the conduction bands
     tcor = 0.0d0
@@ -550,6 +561,8 @@ program wtb_bse
 
     flags = WTB_FLAG_GAUGE_LAPACK
     if (compat) flags = flags + WTB_FLAG_COMPAT_F32
+    if (doscalc) call dos_file()
+    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd)) stop
     select case (trim(precision))
     case ("FP64");  prec = WTB_FP64
     case ("MIXED"); prec = WTB_MIXED
@@ -781,6 +794,60 @@ contains
         close(ub)
         close(ul)
     end subroutine exciton_bands
+
+    ! DOS=T (dostool, module_tools.F90:390-651): OUTPUT//dos.dat with the
+    ! total, integrated, spin (SP / SOC) and orbital-weighted densities
+    subroutine dos_file()
+        type(c_ptr) :: dctx
+        integer :: u, erro, i, nd, mode
+        double precision, allocatable :: en(:), ds(:, :)
+        double precision, allocatable, target :: ow(:)
+        double precision :: spf, sumdos, deltaen
+        type(c_ptr) :: owp
+        nd = int(numdos)
+        allocate(en(nd), ds(nd, 4), ow(norb))
+        owp = c_null_ptr
+        open(newunit=u, file=trim(orbw), status='old', iostat=erro)
+        if (erro == 0) then
+            do i = 1, norb
+                read(u, *) ow(i)
+            end do
+            close(u)
+            owp = c_loc(ow)
+        end if
+        mode = 2
+        if (systype == "NP") mode = 0
+        if (systype == "SP" .or. (systype == "SOC" .and. dft == "V")) mode = 1
+        call wtb_check(dctx, wtb_create(dctx, int(device, c_int)), "wtb_create")
+        call wtb_check(dctx, wtb_set_tb_model(dctx, norb, nvec, rvec, ffactor, hop, ihop, rlat, efermi, scs, flags), &
+                       "wtb_set_tb_model")
+        call wtb_check(dctx, wtb_set_kpoints(dctx, nk, kpt, ngrid), "wtb_set_kpoints")
+        call wtb_check(dctx, wtb_hk_eig(dctx, 1, 1, c_null_ptr, c_null_ptr, c_null_ptr), "wtb_hk_eig")
+        call wtb_check(dctx, wtb_dos(dctx, nd, sme, owp, mode, en, ds), "wtb_dos")
+        call wtb_check(dctx, wtb_destroy(dctx), "wtb_destroy")
+        open(newunit=u, file=trim(outputfolder)//"dos.dat", status='unknown', iostat=erro)
+        if (erro /= 0) stop "Error opening dos output file"
+        spf = 1.0
+        if (systype == "NP") then
+            spf = 2.0
+            write(u, *) "#energy tdos sumdos awdos"
+        else
+            write(u, *) "#energy tdos sumdos updos dndos awdos"
+        end if
+        ds(:, 1) = spf * ds(:, 1)
+        ds(:, 4) = spf * ds(:, 4)
+        deltaen = (en(nd) - en(1)) / (numdos - 1.)
+        sumdos = 0.0
+        do i = 1, nd
+            if (systype == "NP") then
+                write(u, "(4F15.4)") en(i), ds(i, 1), sumdos, ds(i, 4)
+            else
+                write(u, "(6F15.4)") en(i), ds(i, 1), sumdos, ds(i, 2), -ds(i, 3), ds(i, 4)
+            end if
+            if (i > 1) sumdos = sumdos + (deltaen * (ds(i - 1, 1) + ds(i, 1)) * 0.5)
+        end do
+        close(u)
+    end subroutine dos_file
 
     ! f_v - f_c at T kelvin, Fermi level 0 (fermidisteh, module_math.F90:410-421):
     ! the Boltzmann constant is a default-real literal there, and T = 0 is 1E-8

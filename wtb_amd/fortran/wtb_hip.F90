@@ -106,6 +106,15 @@ module wtb_hip
             type(c_ptr), value :: eig, vec, nocp
         end function wtb_hk_eig_q
 
+        ! dostool (module_tools.F90:390-651): en(ndos), dos(ndos,4) = tdos, up, down, weighted
+        integer(c_int) function wtb_dos(ctx, ndos, sigma, orbweight, spin_mode, en, dos) bind(C, name="wtb_dos")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx, orbweight
+            integer(c_int), value :: ndos, spin_mode
+            real(c_double), value :: sigma
+            real(c_double), intent(out) :: en(*), dos(*)
+        end function wtb_dos
+
         integer(c_int) function wtb_get_bands(ctx, energies, vectors) bind(C, name="wtb_get_bands")
             import :: c_int, c_ptr
             type(c_ptr), value :: ctx, energies, vectors

@@ -31,7 +31,7 @@ EXPORTS = ["wtb_create", "wtb_destroy", "wtb_last_error", "wtb_set_tb_model", "w
            "wtb_hk_eig", "wtb_get_bands", "wtb_bse_build", "wtb_bse_solve", "wtb_optics",
            "wtb_oscillator", "wtb_bse_get_block", "wtb_bse_matvec", "wtb_bse_diag",
            "wtb_slepc_solver_compat", "wtb_grad_energies", "wtb_grad_spectrum", "wtb_set_temperature",
-           "wtb_hk_eig_q"]
+           "wtb_hk_eig_q", "wtb_dos"]
 
 
 class WTBError(RuntimeError):
@@ -85,6 +85,7 @@ def load():
         "wtb_set_kpoints": [P, I, P, P],
         "wtb_hk_eig": [P, I, I, P, P, P],
         "wtb_hk_eig_q": [P, P, P, I, I, P, P, P],
+        "wtb_dos": [P, I, D, P, I, P, P],
         "wtb_get_bands": [P, P, P],
         "wtb_bse_build": [P, C.POINTER(CoulombParams), I, I],
         "wtb_bse_solve": [P, I, D, I, P, P, C.POINTER(Report)],
@@ -204,6 +205,16 @@ class Context:
         nocp = np.zeros(self.nk, np.int32)
         self._check(self.lib.wtb_hk_eig_q(self.h, _ptr(qv), _ptr(k), nv, nc, _ptr(eig), _ptr(vec), _ptr(nocp)))
         return eig, vec, nocp
+
+    def dos(self, ndos, sigma, orbweight=None, spin_mode=0):
+        """Density of states of the bands of the last hk_eig (dostool):
+        returns (en (ndos,), dos (ndos, 4): tdos, up, down, weighted)."""
+        en = np.zeros(ndos)
+        d = np.zeros((ndos, 4), order="F")
+        ow = None if orbweight is None else np.ascontiguousarray(orbweight, np.float64)
+        self._check(self.lib.wtb_dos(self.h, int(ndos), float(sigma), None if ow is None else _ptr(ow),
+                                     int(spin_mode), _ptr(en), _ptr(d)))
+        return en, d
 
     def get_bands(self):
         e = np.zeros((self.nk, self.norb), order="F")
