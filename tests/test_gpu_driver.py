@@ -341,3 +341,30 @@ def test_dos_driver_run():
     tdos = 2.0 * dd[:, 0]
     run = np.r_[0.0, 0.0, np.cumsum(0.5 * (tdos[1:-1] + tdos[:-2]) * (en[-1] - en[0]) / 400.0)]
     assert np.abs(out[:, 2] - run).max() <= 1e-4 * max(1.0, run.max())
+
+
+def test_single_particle_driver_run():
+    """DIEL=T alone (spoptics) with TMCOEF=T and SPEC=T: sp_optics.dat and the
+    tm_coef files (k, band indices, then the same sorted rows), the
+    single-particle spectra, and no BSE or polarised files (CPOL=F)."""
+    say("case driver single particle")
+    if not os.path.exists(DRIVER):
+        pytest.fail("wtb_amd/wtb_bse is not built (make -C wtb_amd/fortran)")
+    cards = ["DIEL= T", "TMCOEF= T", "SPEC= T", "SYSDIM= \"2D\"", "NGX= 12", "NGY= 12", "NGZ= 1", "NBANDSC= 1",
+             "NBANDSV= 1", "CSHIFT= 0.05", "ENSPECI= 3.0", "ENSPECF= 7.0", "NESPEC= 400"]
+    d, m = run_driver(cards)
+    for absent in ("bse_opt_diel.dat", "log_bse-diel.dat", "bse_diel_xx.dat", "sp_diel-pol_x.dat"):
+        assert not os.path.exists(os.path.join(d, absent)), absent
+    sp = load(os.path.join(d, "sp_optics.dat"))
+    assert sp.shape == (144, 7)
+    tm = np.loadtxt(os.path.join(d, "tm_coef.dat"), skiprows=4, ndmin=2)
+    assert tm.shape == (144, 13)
+    assert np.abs(tm[:, 6:] - sp).max() <= 1.01e-4                    # F10.4 against F15.4
+    assert np.all(tm[:, 3] == 1) and np.all(tm[:, 4] == 2) and np.all(tm[:, 5] == 1)   # nocp, c, v
+    k = T.mp_mesh(12, 12, 1, m.rlat)
+    kset = {tuple(np.round(x, 4)) for x in k.astype(np.float32).astype(float)}
+    assert all(tuple(np.round(r, 4)) in kset for r in tm[:, :3])
+    vc = abs(np.dot(np.cross(m.rlat[0], m.rlat[1]), m.rlat[2]))
+    got = load(os.path.join(d, "sp_diel_xx.dat"))
+    exp = O.spectrum(sp[:, 0], sp[:, 1], True, (12, 12, 1), vc, 3.0, 7.0, 400, 0.05, renorm=True)
+    assert np.abs(got[:, 2] - np.float32(2.0 * exp[:, 2])).max() < 1e-6 * max(1.0, 2 * np.abs(exp[:, 2]).max())

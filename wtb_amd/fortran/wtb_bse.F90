@@ -46,6 +46,8 @@ module wtb_cards
     ! DOS=T (dostool): NEDOS= points, SIGMA= Gaussian width, ORB_W= orbital
     ! weights file, DFT= spin ordering of SOC models
     logical :: doscalc = .false.
+    ! DIEL=T (spoptics): single-particle optics alone; TMCOEF=T: tm_coef files
+    logical :: spdiel = .false., tmcoef = .false.
     double precision :: numdos = 6001.0d0, sme = 0.08d0
     character(len=70) :: orbw = "non declared file"
     character(len=1) :: dft = "V"
@@ -91,6 +93,8 @@ contains
             case ("BSET=");         read(b, *) bset
             case ("BSE_BND=");      read(b, *) bsekpath
             case ("DOS=");          read(b, *) doscalc
+            case ("DIEL=");         read(b, *) spdiel
+            case ("TMCOEF=");       read(b, *) tmcoef
             case ("NEDOS=");        read(b, *) numdos
             case ("SIGMA=");        read(b, *) sme
             case ("ORB_W=");        orbw = trim(adjustl(b))
@@ -532,8 +536,8 @@ program wtb_bse
     call read_cards()
     if (bse .and. bset) stop "ERROR: Choose BSE=T or BSET=T not both"
     if (bsekpath .and. bsetbnd) stop "ERROR: Choose BSE_BND=T or BSET_BND=T not both"
-    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd .or. doscalc)) &
-        stop "wtb_bse: only the DOS=T, BSE=T, BSET=T, BSE_BND=T and BSET_BND=T paths are provided"
+    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd .or. doscalc .or. spdiel)) &
+        stop "wtb_bse: only the DOS=T, DIEL=T, BSE=T, BSET=T, BSE_BND=T and BSET_BND=T paths are provided"
     ! gap correction of the temperature paths (bsesolvertemp, module_bse.F90:2857-2875;
     ! bsebndstemp :1740-1756), applied with the scissor to the conduction bands
     tcor = 0.0d0
@@ -562,7 +566,7 @@ program wtb_bse
     flags = WTB_FLAG_GAUGE_LAPACK
     if (compat) flags = flags + WTB_FLAG_COMPAT_F32
     if (doscalc) call dos_file()
-    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd)) stop
+    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd .or. spdiel)) stop
     select case (trim(precision))
     case ("FP64");  prec = WTB_FP64
     case ("MIXED"); prec = WTB_MIXED
@@ -587,7 +591,7 @@ program wtb_bse
     end select
     coul = wtb_coulomb(kind, r0, ediel, ez, w, lc, ktol)
     if (bsekpath .or. bsetbnd) call exciton_bands()
-    if (.not. (bse .or. bset)) stop
+    if (.not. (bse .or. bset .or. spdiel)) stop
 
     ! ---- GPU path: eigsys loop, optsp, matrizelbse, slepc_solver, dielbse*
     call wtb_check(ctx, wtb_create(ctx, int(device, c_int)), "wtb_create")
@@ -596,11 +600,12 @@ program wtb_bse
     call wtb_check(ctx, wtb_set_kpoints(ctx, nk, kpt, ngrid), "wtb_set_kpoints")
     call wtb_check(ctx, wtb_hk_eig(ctx, nv, nc, c_loc(eigv), c_null_ptr, c_loc(nocpk)), "wtb_hk_eig")
     call wtb_check(ctx, wtb_optics(ctx, cshift, c_loc(dip)), "wtb_optics")
+    ws = 0.0d0
+    act = 0.0d0
+    if (bse .or. bset) then
     if (bset) call wtb_check(ctx, wtb_set_temperature(ctx, temp), "wtb_set_temperature")
     call wtb_check(ctx, wtb_bse_build(ctx, coul, prec, conv), "wtb_bse_build")
-    ws = 0.0d0
     call wtb_check(ctx, wtb_bse_solve(ctx, nevs, 1.0d-8, 200, ws, c_null_ptr, c_loc(rep)), "wtb_bse_solve")
-    act = 0.0d0
     call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_X, WTB_X, act(:, 1)), "wtb_oscillator xx")
     call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_Y, WTB_Y, act(:, 2)), "wtb_oscillator yy")
     call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_Z, WTB_Z, act(:, 3)), "wtb_oscillator zz")
@@ -613,6 +618,7 @@ program wtb_bse
         call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_SP, WTB_SP, act(:, 7)), "wtb_oscillator sp")
         call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_SM, WTB_SM, act(:, 8)), "wtb_oscillator sm")
     end if
+    end if
     call wtb_check(ctx, wtb_destroy(ctx), "wtb_destroy")
 
     ! ---- log_bse-diel.dat (module_bse.F90:2084-2111, 2271-2273, 2470-2474)
@@ -620,6 +626,7 @@ program wtb_bse
     do k = 1, nk
         egap = min(egap, eigv(k, nv + 1) - eigv(k, nv))
     end do
+    if (bse .or. bset) then
     open(unit=300, file=trim(outputfolder)//"/log_bse-diel.dat", status='unknown', iostat=erro)
     if (erro /= 0) stop "Error opening log_bse-diel output file"
     if (bset) write(300, *) 'temperature', temp
@@ -633,6 +640,7 @@ program wtb_bse
     write(300, *) 'solver: nev', nevs, ' iterations', rep%iterations, ' converged', rep%nconv, &
                   ' max rel residual', rep%max_rel_residual, ' solve ms', rep%t_solve_ms
     close(300)
+    end if
 
     ! ---- sp_optics.dat / sp_optics-pol.dat: transitions sorted by energy
     ! (module_bse.F90:2300-2375); float32 columns as the reference's vecres
@@ -679,16 +687,44 @@ program wtb_bse
     if (erro /= 0) stop "Error opening sp_optics-pol output file"
     write(401, *) "#", "  ", "energy", "  ", "xx", "  ", "yy", "  ", "zz", "  ", "xy", "  ", "xz", "  ", "yz"
     write(403, *) "#", "  ", "energy", "  ", "xx", "  ", "yy", "  ", "zz", "  ", "sp", "  ", "sm"
+    if (tmcoef) then
+        ! tm_coef.dat / tm_coef-pol.dat (module_bse.F90:2093-2096,2300-2375; spoptics :4880-4960)
+        open(unit=402, file=trim(outputfolder)//"/tm_coef.dat", status='unknown', iostat=erro)
+        if (erro /= 0) stop "Error opening tm_coef output file"
+        open(unit=404, file=trim(outputfolder)//"/tm_coef-pol.dat", status='unknown', iostat=erro)
+        if (erro /= 0) stop "Error opening tm_coef-pol output file"
+        write(402, *) "number of kpoints:", nk
+        write(402, *) "number of conduction states", nc
+        write(402, *) "number of valence states", nv
+        write(402, *) "#", " ", "kx", " ", "ky", " ", "kz", " ", "nocp", " ", "nc", " ", "nv", "  ", "energy", "  ", &
+            "xx", "  ", "yy", "  ", "zz", "  ", "xy", "  ", "xz", "  ", "yz"
+        write(404, *) "number of kpoints:", nk
+        write(404, *) "number of conduction states", nc
+        write(404, *) "number of valence states", nv
+        write(404, *) "#", " ", "kx", " ", "ky", " ", "kz", " ", "nocp", " ", "nc", " ", "nv", "  ", "energy", "  ", &
+            "xx", "  ", "yy", "  ", "zz", "  ", "sp", "  ", "sm"
+    end if
     do i = 1, dimbse
         ib = order(i)
         write(401, "(7F15.4)") vecres(ib, 7:13)
         write(403, "(6F15.4)") vecres(ib, 7), vecres(ib, 8), vecres(ib, 9), vecres(ib, 10), vecres(ib, 14), &
                                vecres(ib, 15)
+        if (tmcoef) then
+            write(402, "(3F10.4,3I10.0,7F10.4)") vecres(ib, 1:3), int(vecres(ib, 4)), int(vecres(ib, 5)), &
+                int(vecres(ib, 6)), vecres(ib, 7:13)
+            write(404, "(3F10.4,3I10.0,6F10.4)") vecres(ib, 1:3), int(vecres(ib, 4)), int(vecres(ib, 5)), &
+                int(vecres(ib, 6)), vecres(ib, 7:10), vecres(ib, 14), vecres(ib, 15)
+        end if
     end do
     close(401)
     close(403)
+    if (tmcoef) then
+        close(402)
+        close(404)
+    end if
 
     ! ---- bse_opt_diel.dat / bse_opt_diel-pol.dat (module_bse.F90:2140-2141, 2548-2569)
+    if (bse .or. bset) then
     open(unit=301, file=trim(outputfolder)//"/bse_opt_diel.dat", status='unknown', iostat=erro)
     if (erro /= 0) stop "Error opening bse_opt_diel output file"
     open(unit=302, file=trim(outputfolder)//"/bse_opt_diel-pol.dat", status='unknown', iostat=erro)
@@ -701,6 +737,7 @@ program wtb_bse
     end do
     close(301)
     close(302)
+    end if
 
     ! ---- spectra from the written (4-decimal) file, optical constants, lifetimes
     ! (main.F90:183-223: with BSE the single-particle chains run too, and the
@@ -708,6 +745,8 @@ program wtb_bse
     if (spec) then
         spinf = 1.0
         if (systype == "NP") spinf = 2.0
+    end if
+    if (spec .and. (bse .or. bset)) then
         call diel_set("bse_opt_diel.dat", "bse_diel_", tens, spinf)
         call optprop_set("bse_diel_", tens, "bse_", "")
         if (.not. bset) call lifetime_file()
@@ -716,13 +755,17 @@ program wtb_bse
             call optprop_set("bse_diel-pol_", pols, "bse_", "-pol")
             if (.not. bset) call lifetime_pol_file()
         end if
+    end if
+    if (spec) then
         call diel_set("sp_optics.dat", "sp_diel_", tens, spinf)
         call optprop_set("sp_diel_", tens, "sp_", "")
+    end if
+    if (spec .and. (cpol .or. bse .or. bset)) then
         call diel_set("sp_optics-pol.dat", "sp_diel-pol_", pols, spinf)
         call optprop_set("sp_diel-pol_", pols, "sp_", "-pol")
     end if
-    write(*, '(A,I0,A,I0,A,F12.6,A,ES10.3)') 'wtb_bse: N=', dimbse, ' nev=', nevs, ' E1=', ws(1), &
-        ' eV, max rel residual ', rep%max_rel_residual
+    if (bse .or. bset) write(*, '(A,I0,A,I0,A,F12.6,A,ES10.3)') 'wtb_bse: N=', dimbse, ' nev=', nevs, ' E1=', &
+        ws(1), ' eV, max rel residual ', rep%max_rel_residual
 
 contains
 
