@@ -164,6 +164,13 @@ def hk_eig(model: Model, kpt, nv, nc, full=False):
     return res
 
 
+def bessel_k01(x):
+    """(K0(x), K1(x)) of the oracle's quadrature (used by V1DT)."""
+    a, b = C.c_double(), C.c_double()
+    lib().or_bessel_k01(C.c_double(float(x)), C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
 def coulomb_v(c, k1, k2, ngrid, rlat):
     return lib().or_coulomb_v(C.byref(c), _p(np.ascontiguousarray(k1, float)),
                               _p(np.ascontiguousarray(k2, float)),

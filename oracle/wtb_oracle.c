@@ -6,6 +6,8 @@
  * WTB sources).  Where the reference evaluates an expression in default
  * (single) precision, the OR_COMPAT_F32 flag reproduces that rounding.
  */
+/* j0, j1 of the C library are POSIX (X/Open) functions */
+#define _XOPEN_SOURCE 700
 #include "wtb_oracle.h"
 
 #include <math.h>
@@ -314,6 +316,37 @@ static double vnorm(const double *k1, const double *k2) {
     return sqrt(dx * dx + dy * dy + dz * dz);                  /* modvec, module_math.F90:221 */
 }
 
+/* K0(x), K1(x), x > 0 (the reference's calck0/calck1, Cody's rational
+ * approximations, module_math.F90:453+): here from K_n(x) = int_0^inf
+ * exp(-x cosh t) cosh(n t) dt by the trapezoidal rule with step 1/20, which
+ * converges geometrically for this analytic integrand; pinned against
+ * scipy.special.k0/k1 in the tests. */
+void or_bessel_k01(double x, double *k0, double *k1) {
+    const double h = 0.05;
+    double s0 = 0.5 * exp(-x), s1 = s0;
+    for (int j = 1; j < 100000; ++j) {
+        double ch = cosh(h * j), a = x * ch;
+        if (a > 745.0) break;
+        s0 += exp(-a);
+        s1 += exp(-a) * ch;
+    }
+    *k0 = h * s0;
+    *k1 = h * s1;
+}
+
+/* integralJ0pot, module_math.F90:429-449: 100-point trapezoid of r J0(g r) ln r
+ * on r_i = (i - 1)/99 rmax + 1e-4 (a default-real literal) */
+static double or_j0_log_integral(double rmax, double g) {
+    double r[100], f[100], res = 0.0;
+    for (int i = 0; i < 100; ++i) {
+        r[i] = ((double)i / 99.0) * rmax + (double)1e-4f;
+        f[i] = r[i] * j0(r[i] * g) * log(r[i]);
+    }
+    double h = r[1] - r[0];
+    for (int i = 0; i < 99; ++i) res = res + (h / 2.0) * (f[i] + f[i + 1]);
+    return res;
+}
+
 double or_coulomb_v(const or_coulomb *c, const double *k1, const double *k2,
                     const int *ngrid, const double *rlat) {
     const double cic = or_cic(), pi = or_pi_f32();
@@ -383,6 +416,17 @@ double or_coulomb_v(const or_coulomb *c, const double *k1, const double *k2,
         if (q < c->ktol) return 0.0;
         return (vbz * cic) * exp(-c->w * q) * (1.0 / (c->ez * q));
     }
+    case OR_V1DT: { /* v1dt, module_coulomb.F90:391-442 */
+        double vbz = 1.0 / (nk * or_vcell3d(rlat));
+        double qx = fabs(dk[0]), qyz = sqrt(dk[1] * dk[1] + dk[2] * dk[2]);
+        if (qx < c->ktol && qyz >= c->ktol) return (vbz * cic) * or_j0_log_integral(c->lc, qyz);
+        if (qx < c->ktol) return (vbz * cic) * ((c->lc * c->lc) / 16.0) * (2.0 * log(c->lc) - 1.0);
+        double kk0, kk1;
+        or_bessel_k01(qx * c->lc, &kk0, &kk1);
+        double c1 = qyz * c->lc * j1(qyz * c->lc) * kk0;
+        double c2 = qx * c->lc * j0(qyz * c->lc) * kk1;
+        return (vbz * cic) * (1.0 + c1 + c2);
+    }
     default:
         return NAN;
     }
@@ -397,7 +441,7 @@ int or_bse_matrix(int nk, int nc, int nv, int norb, const double *kpt, const dou
                   const double *rlat, int textbook, double complex *h) {
     int nb = nc + nv;
     long N = (long)nk * nc * nv;
-    if (c->kind < OR_V2DK || c->kind > OR_V0DT) return -7;
+    if (c->kind < OR_V2DK || c->kind > OR_V1DT) return -7;
 #pragma omp parallel for schedule(dynamic, 8)
     for (long i = 0; i < N; ++i) {
         int ki = (int)(i / (nc * nv)), ci = (int)((i / nv) % nc), vi = (int)(i % nv);

@@ -62,6 +62,7 @@ typedef struct {
 /* constants as the reference evaluates them (single-precision literals) */
 double or_pi_f32(void);
 double or_cic(void);
+void or_bessel_k01(double x, double *k0, double *k1);
 
 void or_recvec(const double *rlat, double *blat);
 void or_mp_mesh(int n1, int n2, int n3, const double *shift, const double *rlat, double *kpt);

@@ -115,7 +115,7 @@ def test_bse_matrix(precision, conv):
     ctx.close()
 
 
-@pytest.mark.parametrize("kind", ["V2DK", "V3D", "V2DT", "V2DT2", "V2DOH", "V0DT"])
+@pytest.mark.parametrize("kind", ["V2DK", "V3D", "V2DT", "V2DT2", "V2DOH", "V0DT", "V1DT"])
 def test_bse_matrix_potentials(kind):
     say("case potential", kind)
     mesh = (6, 6, 1)

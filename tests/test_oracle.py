@@ -540,3 +540,37 @@ def test_dos_flat_bands_and_weight_identities():
     mass = 0.5 * (erf(z1) - erf(z0)).sum() / 16
     assert 20.0 < mass < 22.0
     assert abs(np.trapezoid(d[:, 0], en) - mass) < 1e-3 * mass
+
+
+# ----------------------------- V1DT ---------------------------------------
+
+def test_bessel_k01_against_scipy():
+    from scipy.special import k0, k1
+    for x in (1e-3, 0.01, 0.1, 0.5, 1.0, 3.0, 10.0, 40.0, 200.0):
+        a, b = O.bessel_k01(x)
+        assert abs(a - k0(x)) <= 1e-14 * k0(x), x
+        assert abs(b - k1(x)) <= 1e-14 * k1(x), x
+
+
+def test_v1dt_branches():
+    """v1dt's three branches: the q -> 0 closed form, the J0 log integral for
+    q_x = 0 (a 100-point trapezoid, checked with scipy's J0), and the Bessel
+    form 1 + q_yz lc J1 K0 + |q_x| lc J0 K1 with scipy's special functions."""
+    from scipy.special import j0, j1, k0, k1
+    rlat = np.eye(3) * np.array([3.0, 20.0, 20.0])
+    g = np.array([10, 1, 1], np.int32)
+    lc = 6.0
+    c = O.coulomb("V1DT", lc=lc, ktol=0.001)
+    pre = O.cic() / (10 * 3.0 * 400.0)
+    z = np.zeros(3)
+    assert abs(O.coulomb_v(c, z, z, g, rlat) - pre * (lc * lc / 16.0) * (2 * math.log(lc) - 1)) < 1e-12 * abs(pre)
+    qyz = 0.3
+    r = np.arange(100) / 99.0 * lc + float(np.float32(1e-4))
+    f = r * j0(r * qyz) * np.log(r)
+    ref = pre * np.sum((r[1] - r[0]) / 2.0 * (f[:-1] + f[1:]))
+    assert abs(O.coulomb_v(c, np.array([0.0, qyz, 0.0]), z, g, rlat) - ref) < 1e-12 * abs(ref)
+    for qx, qy in ((0.2, 0.0), (0.05, 0.4), (1.3, 0.7)):
+        a, b = qx * lc, qy * lc
+        ref = pre * (1.0 + b * j1(b) * k0(a) + a * j0(b) * k1(a))
+        got = O.coulomb_v(c, np.array([qx, qy, 0.0]), z, g, rlat)
+        assert abs(got - ref) < 1e-12 * abs(ref), (qx, qy)

@@ -29,6 +29,44 @@ struct CoulombDev {
     double rc, cr;
 };
 
+// Modified Bessel functions K0(x), K1(x) (x > 0) from the integral
+// representation K_n(x) = int_0^inf exp(-x cosh t) cosh(n t) dt, summed by the
+// trapezoidal rule with step 1/5.  The integrand is analytic in the strip
+// |Im t| < pi/2, so the rule converges like exp(-pi^2 / h) ~ 1e-21 relative;
+// the sum stops where x cosh t passes 745 (exp underflows there).
+__host__ __device__ inline void bessel_k01(double x, double &k0, double &k1) {
+    const double h = 0.2;
+    const double e0 = exp(-x);
+    double s0 = 0.5 * e0, s1 = 0.5 * e0;
+    for (int j = 1; j < 1024; ++j) {
+        const double ch = cosh(h * (double)j), a = x * ch;
+        if (a > 745.0) break;
+        const double e = exp(-a);
+        s0 += e;
+        s1 += e * ch;
+    }
+    k0 = h * s0;
+    k1 = h * s1;
+}
+
+// int r J0(g r) ln r dr by the reference's 100-point trapezoid on
+// r_i = (i - 1)/99 rmax + 1e-4 (integralJ0pot, src/module_math.F90:429-449;
+// eta is a single-precision literal there)
+__host__ __device__ inline double j0_log_integral(double rmax, double g) {
+    const int n = 100;
+    const double eta = (double)1e-4f;
+    const double r1 = (0.0 / (n - 1.0)) * rmax + eta, r2 = (1.0 / (n - 1.0)) * rmax + eta;
+    const double h = r2 - r1;
+    double res = 0.0, fprev = r1 * j0(r1 * g) * log(r1);
+    for (int i = 2; i <= n; ++i) {
+        const double r = (((double)i - 1.0) / (n - 1.0)) * rmax + eta;
+        const double f = r * j0(r * g) * log(r);
+        res = res + (h / 2.0) * (fprev + f);
+        fprev = f;
+    }
+    return res;
+}
+
 __host__ __device__ inline double coulomb_v(const CoulombDev &c, double dx, double dy, double dz) {
     const double q = sqrt(dx * dx + dy * dy + dz * dz);
     switch (c.kind) {
@@ -53,6 +91,16 @@ __host__ __device__ inline double coulomb_v(const CoulombDev &c, double dx, doub
     case WTB_V0DT:   // v0dt, module_coulomb.F90:197-238
         if (q < c.ktol) return (c.vbz3d * c.cic) * 0.5 * c.cr * c.cr;
         return (c.vbz3d * c.cic) * (1.0 / (q * q)) * (1.0 - cos(c.cr * q));
+    case WTB_V1DT: { // v1dt, module_coulomb.F90:391-442: q_x against |q_yz|, length lc
+        const double qx = fabs(dx), qyz = sqrt(dy * dy + dz * dz), pre = c.vbz3d * c.cic;
+        if (qx < c.ktol && qyz >= c.ktol) return pre * j0_log_integral(c.lc, qyz);
+        if (qx < c.ktol) return pre * ((c.lc * c.lc) / 16.0) * (2.0 * log(c.lc) - 1.0);
+        double k0, k1;
+        bessel_k01(qx * c.lc, k0, k1);
+        const double c1 = qyz * c.lc * j1(qyz * c.lc) * k0;
+        const double c2 = qx * c.lc * j0(qyz * c.lc) * k1;
+        return pre * (1.0 + c1 + c2);
+    }
     case WTB_V2DT2: {// v2dt2, module_coulomb.F90:243-282
         if (q < c.ktol) return 0.0;
         const double qxy = sqrt(dx * dx + dy * dy);

@@ -501,8 +501,7 @@ __global__ void fdeh_kernel(int nk, int nc, int nv, double temp, const double *_
 void launch_bse_build(Ctx &c, const wtb_coulomb &coul, int precision, int convention) {
     Bands &b = c.bands;
     if (!b.valid) throw Error{WTB_ERR_STATE, "wtb_hk_eig must run before wtb_bse_build"};
-    if (coul.kind == WTB_V1DT) throw Error{WTB_ERR_UNSUPPORTED, "V1DT Coulomb potential is not available"};
-    if (coul.kind < WTB_V2DK || coul.kind > WTB_V0DT) throw Error{WTB_ERR_ARG, "Wrong Coulomb Potential"};
+    if (coul.kind < WTB_V2DK || coul.kind > WTB_V1DT) throw Error{WTB_ERR_ARG, "Wrong Coulomb Potential"};
     if (precision != WTB_FP64 && precision != WTB_MIXED && precision != WTB_FP32)
         throw Error{WTB_ERR_ARG, "unknown precision"};
     if (b.nc > 4 || b.nv > 4) throw Error{WTB_ERR_UNSUPPORTED, "nc, nv <= 4 on the assembly kernel"};

@@ -391,6 +391,7 @@ int wtb_grad_energies(wtb_ctx *ctx, const double *w, wtb_grad *g, double *hop_re
         if (c.bse.temperature >= 0.0)
             throw Error{WTB_ERR_UNSUPPORTED, "gradients of the temperature BSE are not available"};
         if (c.bse.finite_q) throw Error{WTB_ERR_UNSUPPORTED, "gradients of the finite-Q BSE are not available"};
+        if (c.bse.coul.kind == WTB_V1DT) throw Error{WTB_ERR_UNSUPPORTED, "gradients with V1DT are not available"};
         grad_energies(c, w, g, hop_re_grad, hop_im_grad);
     });
 }
@@ -402,6 +403,7 @@ int wtb_grad_spectrum(wtb_ctx *ctx, int comp, double cshift, double gamma, int n
         if (c.bse.temperature >= 0.0)
             throw Error{WTB_ERR_UNSUPPORTED, "gradients of the temperature BSE are not available"};
         if (c.bse.finite_q) throw Error{WTB_ERR_UNSUPPORTED, "gradients of the finite-Q BSE are not available"};
+        if (c.bse.coul.kind == WTB_V1DT) throw Error{WTB_ERR_UNSUPPORTED, "gradients with V1DT are not available"};
         grad_spectrum(c, comp, cshift, gamma, nw, omega, target, loss, eps2, g, hop_re_grad, hop_im_grad);
     });
 }
