@@ -143,6 +143,13 @@ int wtb_hk_eig(wtb_ctx *ctx, int nv, int nc, double *eig, double *vec, int *nocp
 int wtb_hk_eig_q(wtb_ctx *ctx, const double *q, const double *qpt, int nv, int nc, double *eig, double *vec,
                  int *nocp);
 
+/* Optical activity over the Brillouin zone (OPT_BZ=T: spoptpolbz,
+ * module_bse.F90:4532-4797): for every k of the mesh, act(nk,5) column-major
+ * = sum over the nc*nv band pairs of |d_a|^2, a = x, y, z, sp, sm, with
+ * d_a = <c|dH/dk_a|v> / (E_c - E_v + i sme) (optspbz, module_optics.F90:257-311)
+ * and sp, sm = (d_x +- i d_y)/sqrt(2).  The bands are those of wtb_hk_eig. */
+int wtb_optics_bz(wtb_ctx *ctx, double sme, double *act);
+
 /* Density of states (dostool, module_tools.F90:390-651) of all bands of the
  * last wtb_hk_eig: en(ndos) is the reference's grid from min(0, lowest band)
  * to max(0, highest band); dos(ndos,4) column-major holds tdos, spin-up,

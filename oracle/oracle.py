@@ -274,6 +274,27 @@ def bse_matrix_q(kpt, eig, vec, eigq, vecq, c, ngrid, rlat, nc, nv, q, temp=None
     return H
 
 
+def optics_bz(model: Model, kpt, eig, vec, nc, nv, sme):
+    """spoptpolbz (src/module_bse.F90:4532-4797): per k the sums over the band
+    pairs (c outer, v inner) of |d_a|^2 with d_a = <c|dH/dk_a|v> / (E_c - E_v
+    + i sme) (optspbz, src/module_optics.F90:257-311) and sp, sm = (d_x +-
+    i d_y) / sqrt(2) with the default-real 1/sqrt(2.).  Returns (nk, 5)."""
+    nk = kpt.shape[0]
+    norm = float(np.float32(1.0) / np.sqrt(np.float32(2.0)))
+    act = np.zeros((nk, 5))
+    for k in range(nk):
+        h = dhk(model, kpt[k])
+        for c in range(nc):
+            uc = vec[k, nv + c]
+            for v in range(nv):
+                uv = vec[k, v]
+                den = complex(eig[k, nv + c] - eig[k, v], sme)
+                dx, dy, dz = (uc.conj() @ h[a] @ uv / den for a in range(3))
+                sp, sm = norm * (dx + 1j * dy), norm * (dx - 1j * dy)
+                act[k] += [abs(dx) ** 2, abs(dy) ** 2, abs(dz) ** 2, abs(sp) ** 2, abs(sm) ** 2]
+    return act
+
+
 def dos(eall, zall, sigma, ndos, orbweight=None, spin_mode=0):
     """dostool (src/module_tools.F90:390-651): the grid from min(0, lowest
     band) to max(0, highest band) and, per grid energy, (1/nk) sum over all

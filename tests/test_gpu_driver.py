@@ -368,3 +368,24 @@ def test_single_particle_driver_run():
     got = load(os.path.join(d, "sp_diel_xx.dat"))
     exp = O.spectrum(sp[:, 0], sp[:, 1], True, (12, 12, 1), vc, 3.0, 7.0, 400, 0.05, renorm=True)
     assert np.abs(got[:, 2] - np.float32(2.0 * exp[:, 2])).max() < 1e-6 * max(1.0, 2 * np.abs(exp[:, 2]).max())
+
+
+def test_bz_activity_driver_run():
+    """OPT_BZ=T alone (spoptpolbz): bz_act_{x,y,z,sp,sm}.dat and
+    dichroism.dat per mesh point against the oracle sums."""
+    say("case driver opt bz")
+    if not os.path.exists(DRIVER):
+        pytest.fail("wtb_amd/wtb_bse is not built (make -C wtb_amd/fortran)")
+    cards = ["OPT_BZ= T", "NGX= 6", "NGY= 6", "NGZ= 1", "NBANDSC= 1", "NBANDSV= 1", "CSHIFT= 0.05"]
+    d, m = run_driver(cards)
+    k = T.mp_mesh(6, 6, 1, m.rlat)
+    om = O.Model(m)
+    r = O.hk_eig(om, k, 1, 1)
+    ref = O.optics_bz(om, k, r["eig"], r["vec"], 1, 1, 0.05)
+    for a, tag in enumerate(("x", "y", "z", "sp", "sm")):
+        got = load(os.path.join(d, f"bz_act_{tag}.dat"))
+        assert got.shape == (36, 4)
+        assert np.abs(got[:, :3] - k.astype(np.float32)).max() < 1e-6
+        assert np.abs(got[:, 3] - ref[:, a]).max() <= 1e-9 * max(1.0, np.abs(ref[:, a]).max()), tag
+    dic = load(os.path.join(d, "dichroism.dat"))
+    assert np.abs(dic[:, 3] - (ref[:, 3] - ref[:, 4]) / (ref[:, 3] + ref[:, 4])).max() < 1e-9

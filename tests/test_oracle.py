@@ -574,3 +574,23 @@ def test_v1dt_branches():
         ref = pre * (1.0 + b * j1(b) * k0(a) + a * j0(b) * k1(a))
         got = O.coulomb_v(c, np.array([qx, qy, 0.0]), z, g, rlat)
         assert abs(got - ref) < 1e-12 * abs(ref), (qx, qy)
+
+
+def test_optics_bz_against_the_dipoles():
+    """The per-k activity sums: (E_c - E_v)^2 + eta^2 times |d_a|^2 of
+    optics_bz equals |<c|dH/dk_a|v>|^2, which is dipoles' |d_a|^2 divided by
+    Re(1/(E_c - E_v + i eta))^2; sp + sm = x + y (the circular pair is unitary)."""
+    m = hex_model((3, 3, 1))
+    k = O.mp_mesh(3, 3, 1, m.rlat)
+    om = O.Model(m, flags=O.GAUGE_LAPACK)
+    r = O.hk_eig(om, k, 1, 1)
+    eta = 0.07
+    act = O.optics_bz(om, k, r["eig"], r["vec"], 1, 1, eta)
+    d = O.dipoles(om, k, r["eig"], r["vec"], 1, 1, eta)
+    de = r["eig"][:, 1] - r["eig"][:, 0]
+    re = de / (de * de + eta * eta)
+    for a in range(3):
+        raw = np.abs(d[a]) ** 2 / re ** 2
+        assert np.allclose(act[:, a] * (de * de + eta * eta), raw, rtol=1e-10)
+    norm2 = float(np.float32(1.0) / np.sqrt(np.float32(2.0))) ** 2
+    assert np.allclose(act[:, 3] + act[:, 4], 2 * norm2 * (act[:, 0] + act[:, 1]), rtol=1e-12)

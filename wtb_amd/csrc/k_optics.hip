@@ -21,7 +21,7 @@ __device__ __forceinline__ cd cadd(cd a, cd b) { return make_double2(a.x + b.x, 
 constexpr int kL = 64;
 
 __global__ void __launch_bounds__(kL)
-dipole_kernel(int nk, int n, int nvec, int nv, int nc, int compat, double eta, double isq2,
+dipole_kernel(int nk, int n, int nvec, int nv, int nc, int compat, int bz, double eta, double isq2,
               const double *__restrict__ kpt, const cd *__restrict__ hop, const double *__restrict__ rcart,
               const double *__restrict__ invdeg, const double *__restrict__ esel, const cd *__restrict__ usel,
               cd *__restrict__ d, long N) {
@@ -77,13 +77,32 @@ dipole_kernel(int nk, int n, int nvec, int nv, int nc, int compat, double eta, d
                 s.x += uc[o].x * row.x + uc[o].y * row.y;   // conj(uc[o]) * row
                 s.y += uc[o].x * row.y - uc[o].y * row.x;
             }
-            r[a] = make_double2(factor * s.x, factor * s.y);
+            if (bz) {   // <c|dH/dk_a|v> / (de + i eta) (spoptpolbz, src/module_bse.F90:4676-4682)
+                const double den = de * de + eta * eta;
+                r[a] = make_double2((s.x * de + s.y * eta) / den, (s.y * de - s.x * eta) / den);
+            } else {
+                r[a] = make_double2(factor * s.x, factor * s.y);
+            }
             d[(size_t)a * N + i] = r[a];
         }
         // circular components (x +- i y)/sqrt(2)
         d[3 * N + i] = make_double2((r[0].x - r[1].y) * isq2, (r[0].y + r[1].x) * isq2);
         d[4 * N + i] = make_double2((r[0].x + r[1].y) * isq2, (r[0].y - r[1].x) * isq2);
     }
+}
+
+// per k: sum over the nc nv transitions of |d_a|^2, a = x, y, z, sp, sm, in
+// the reference's c-outer / v-inner order; act is (nk, 5) column-major
+__global__ void bz_sum_kernel(int nk, int ncv, long N, const cd *__restrict__ d, double *__restrict__ act) {
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= 5L * nk) return;
+    const int a = (int)(t / nk), k = (int)(t % nk);
+    double s = 0.0;
+    for (int i = 0; i < ncv; ++i) {
+        const cd x = d[(size_t)a * N + (size_t)k * ncv + i];
+        s = s + (x.x * x.x + x.y * x.y);
+    }
+    act[(size_t)a * nk + k] = s;
 }
 
 __global__ void __launch_bounds__(256)
@@ -134,10 +153,36 @@ void launch_optics(Ctx &c, double cshift) {
     if (shmem > 160 * 1024) throw Error{WTB_ERR_UNSUPPORTED, "too many R vectors for the dipole kernel"};
     c.dip.alloc(5 * (size_t)N);
     hipLaunchKernelGGL(dipole_kernel, dim3(nk), dim3(kL), shmem, c.stream, nk, n, c.model.nvec, b.nv, b.nc,
-                       (int)compat, eta, isq2, c.kpt.p, c.model.hop.p, c.model.rcart.p, c.model.invdeg.p, b.esel.p,
-                       b.usel.p, c.dip.p, N);
+                       (int)compat, 0, eta, isq2, c.kpt.p, c.model.hop.p, c.model.rcart.p, c.model.invdeg.p,
+                       b.esel.p, b.usel.p, c.dip.p, N);
     WTB_LAUNCH_CHECK();
     c.dip_valid = true;
+}
+
+void launch_optics_bz(Ctx &c, double sme, double *act_host) {
+    Bands &b = c.bands;
+    if (!b.valid) throw Error{WTB_ERR_STATE, "wtb_hk_eig must run before wtb_optics_bz"};
+    if (!act_host) throw Error{WTB_ERR_ARG, "null output"};
+    const int n = c.model.norb, nk = b.nk;
+    const long N = (long)nk * b.nc * b.nv;
+    // spoptpolbz's norm = 1.0/sqrt(2.) is a default-real expression
+    const double isq2 = (double)(1.0f / sqrtf(2.0f));
+    size_t shmem = (3 * (size_t)n * n + (size_t)b.nb * n + c.model.nvec) * sizeof(cd);
+    if (shmem > 160 * 1024) throw Error{WTB_ERR_UNSUPPORTED, "too many R vectors for the dipole kernel"};
+    const bool compat = c.model.flags & WTB_FLAG_COMPAT_F32;
+    DBuf<cd> d;
+    DBuf<double> act;
+    d.alloc(5 * (size_t)N);
+    act.alloc(5 * (size_t)nk);
+    hipLaunchKernelGGL(dipole_kernel, dim3(nk), dim3(kL), shmem, c.stream, nk, n, c.model.nvec, b.nv, b.nc,
+                       (int)compat, 1, sme, isq2, c.kpt.p, c.model.hop.p, c.model.rcart.p, c.model.invdeg.p,
+                       b.esel.p, b.usel.p, d.p, N);
+    WTB_LAUNCH_CHECK();
+    hipLaunchKernelGGL(bz_sum_kernel, dim3((unsigned)((5L * nk + 255) / 256)), dim3(256), 0, c.stream, nk,
+                       b.nc * b.nv, N, d.p, act.p);
+    WTB_LAUNCH_CHECK();
+    WTB_HIP_CHECK(hipMemcpyAsync(act_host, act.p, sizeof(double) * 5 * nk, hipMemcpyDeviceToHost, c.stream));
+    WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
 }
 
 void launch_oscillator(Ctx &c, int nev, int a, int b, double *act_dev) {

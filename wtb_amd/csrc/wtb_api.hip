@@ -251,6 +251,10 @@ int wtb_hk_eig_q(wtb_ctx *ctx, const double *q, const double *qpt, int nv, int n
     });
 }
 
+int wtb_optics_bz(wtb_ctx *ctx, double sme, double *act) {
+    return guarded(ctx, [&](Ctx &c) { launch_optics_bz(c, sme, act); });
+}
+
 int wtb_dos(wtb_ctx *ctx, int ndos, double sigma, const double *orbweight, int spin_mode, double *en,
             double *dos) {
     return guarded(ctx, [&](Ctx &c) { launch_dos(c, ndos, sigma, orbweight, spin_mode, en, dos); });

@@ -311,6 +311,7 @@ void launch_hk_eig(Ctx &c, int nv, int nc);
 void hk_eig_on(Ctx &c, const double *kpt, int nk, int nv, int nc, Bands &b);
 void launch_bse_build(Ctx &c, const wtb_coulomb &coul, int precision, int convention);
 void launch_optics(Ctx &c, double cshift);
+void launch_optics_bz(Ctx &c, double sme, double *act_host);
 void launch_dos(Ctx &c, int ndos, double sigma, const double *orbweight, int spin_mode, double *en_host,
                 double *dos_host);
 void launch_oscillator(Ctx &c, int nev, int a, int b, double *act_dev);

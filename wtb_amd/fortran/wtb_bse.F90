@@ -48,6 +48,8 @@ module wtb_cards
     logical :: doscalc = .false.
     ! DIEL=T (spoptics): single-particle optics alone; TMCOEF=T: tm_coef files
     logical :: spdiel = .false., tmcoef = .false.
+    ! OPT_BZ=T (spoptpolbz): per-k optical activity maps
+    logical :: sppolbz = .false.
     double precision :: numdos = 6001.0d0, sme = 0.08d0
     character(len=70) :: orbw = "non declared file"
     character(len=1) :: dft = "V"
@@ -94,6 +96,7 @@ contains
             case ("BSE_BND=");      read(b, *) bsekpath
             case ("DOS=");          read(b, *) doscalc
             case ("DIEL=");         read(b, *) spdiel
+            case ("OPT_BZ=");       read(b, *) sppolbz
             case ("TMCOEF=");       read(b, *) tmcoef
             case ("NEDOS=");        read(b, *) numdos
             case ("SIGMA=");        read(b, *) sme
@@ -536,7 +539,7 @@ program wtb_bse
     call read_cards()
     if (bse .and. bset) stop "ERROR: Choose BSE=T or BSET=T not both"
     if (bsekpath .and. bsetbnd) stop "ERROR: Choose BSE_BND=T or BSET_BND=T not both"
-    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd .or. doscalc .or. spdiel)) &
+    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd .or. doscalc .or. spdiel .or. sppolbz)) &
         stop "wtb_bse: only the DOS=T, DIEL=T, BSE=T, BSET=T, BSE_BND=T and BSET_BND=T paths are provided"
     ! gap correction of the temperature paths (bsesolvertemp, module_bse.F90:2857-2875;
     ! bsebndstemp :1740-1756), applied with the scissor to the conduction bands
@@ -566,6 +569,7 @@ program wtb_bse
     flags = WTB_FLAG_GAUGE_LAPACK
     if (compat) flags = flags + WTB_FLAG_COMPAT_F32
     if (doscalc) call dos_file()
+    if (sppolbz) call bz_activity_files()
     if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd .or. spdiel)) stop
     select case (trim(precision))
     case ("FP64");  prec = WTB_FP64
@@ -837,6 +841,41 @@ contains
         close(ub)
         close(ul)
     end subroutine exciton_bands
+
+    ! OPT_BZ=T (spoptpolbz, module_bse.F90:4532-4797): per k the activity
+    ! sums of x, y, z, sp and sm in bz_act_*.dat and (sp - sm)/(sp + sm) in
+    ! dichroism.dat, with the mesh point in default real
+    subroutine bz_activity_files()
+        type(c_ptr) :: bctx
+        double precision, allocatable :: act(:, :)
+        character(len=2), parameter :: tg(5) = ["x ", "y ", "z ", "sp", "sm"]
+        integer :: u(6), erro, j, a
+        allocate(act(nk, 5))
+        call wtb_check(bctx, wtb_create(bctx, int(device, c_int)), "wtb_create")
+        call wtb_check(bctx, wtb_set_tb_model(bctx, norb, nvec, rvec, ffactor, hop, ihop, rlat, efermi, scs, flags), &
+                       "wtb_set_tb_model")
+        call wtb_check(bctx, wtb_set_kpoints(bctx, nk, kpt, ngrid), "wtb_set_kpoints")
+        call wtb_check(bctx, wtb_hk_eig(bctx, nv, nc, c_null_ptr, c_null_ptr, c_null_ptr), "wtb_hk_eig")
+        call wtb_check(bctx, wtb_optics_bz(bctx, cshift, act), "wtb_optics_bz")
+        call wtb_check(bctx, wtb_destroy(bctx), "wtb_destroy")
+        do a = 1, 5
+            open(newunit=u(a), file=trim(outputfolder)//"/bz_act_"//trim(tg(a))//".dat", status='unknown', &
+                 iostat=erro)
+            if (erro /= 0) stop "Error opening bz_act output file"
+        end do
+        open(newunit=u(6), file=trim(outputfolder)//"/dichroism.dat", status='unknown', iostat=erro)
+        if (erro /= 0) stop "Error opening dichroism output file"
+        do j = 1, nk
+            do a = 1, 5
+                write(u(a), *) real(kpt(j, 1)), real(kpt(j, 2)), real(kpt(j, 3)), act(j, a)
+            end do
+            write(u(6), *) real(kpt(j, 1)), real(kpt(j, 2)), real(kpt(j, 3)), &
+                (act(j, 4) - act(j, 5)) / (act(j, 4) + act(j, 5))
+        end do
+        do a = 1, 6
+            close(u(a))
+        end do
+    end subroutine bz_activity_files
 
     ! DOS=T (dostool, module_tools.F90:390-651): OUTPUT//dos.dat with the
     ! total, integrated, spin (SP / SOC) and orbital-weighted densities

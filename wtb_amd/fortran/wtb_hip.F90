@@ -106,6 +106,14 @@ module wtb_hip
             type(c_ptr), value :: eig, vec, nocp
         end function wtb_hk_eig_q
 
+        ! spoptpolbz (module_bse.F90:4532-4797): act(nk,5) = per-k sums of |d_a|^2, a = x, y, z, sp, sm
+        integer(c_int) function wtb_optics_bz(ctx, sme, act) bind(C, name="wtb_optics_bz")
+            import :: c_int, c_ptr, c_double
+            type(c_ptr), value :: ctx
+            real(c_double), value :: sme
+            real(c_double), intent(out) :: act(*)
+        end function wtb_optics_bz
+
         ! dostool (module_tools.F90:390-651): en(ndos), dos(ndos,4) = tdos, up, down, weighted
         integer(c_int) function wtb_dos(ctx, ndos, sigma, orbweight, spin_mode, en, dos) bind(C, name="wtb_dos")
             import :: c_int, c_ptr, c_double

@@ -31,7 +31,7 @@ EXPORTS = ["wtb_create", "wtb_destroy", "wtb_last_error", "wtb_set_tb_model", "w
            "wtb_hk_eig", "wtb_get_bands", "wtb_bse_build", "wtb_bse_solve", "wtb_optics",
            "wtb_oscillator", "wtb_bse_get_block", "wtb_bse_matvec", "wtb_bse_diag",
            "wtb_slepc_solver_compat", "wtb_grad_energies", "wtb_grad_spectrum", "wtb_set_temperature",
-           "wtb_hk_eig_q", "wtb_dos"]
+           "wtb_hk_eig_q", "wtb_dos", "wtb_optics_bz"]
 
 
 class WTBError(RuntimeError):
@@ -86,6 +86,7 @@ def load():
         "wtb_hk_eig": [P, I, I, P, P, P],
         "wtb_hk_eig_q": [P, P, P, I, I, P, P, P],
         "wtb_dos": [P, I, D, P, I, P, P],
+        "wtb_optics_bz": [P, D, P],
         "wtb_get_bands": [P, P, P],
         "wtb_bse_build": [P, C.POINTER(CoulombParams), I, I],
         "wtb_bse_solve": [P, I, D, I, P, P, C.POINTER(Report)],
@@ -205,6 +206,13 @@ class Context:
         nocp = np.zeros(self.nk, np.int32)
         self._check(self.lib.wtb_hk_eig_q(self.h, _ptr(qv), _ptr(k), nv, nc, _ptr(eig), _ptr(vec), _ptr(nocp)))
         return eig, vec, nocp
+
+    def optics_bz(self, sme):
+        """Per-k optical activity (OPT_BZ=T): (nk, 5) sums of |d_a|^2 for
+        a = x, y, z, sp, sm over the selected band pairs."""
+        act = np.zeros((self.nk, 5), order="F")
+        self._check(self.lib.wtb_optics_bz(self.h, float(sme), _ptr(act)))
+        return act
 
     def dos(self, ndos, sigma, orbweight=None, spin_mode=0):
         """Density of states of the bands of the last hk_eig (dostool):
