@@ -190,7 +190,9 @@ constexpr int pair_slots() { return TK * (TK + 1) + 1; }
 // same factor, so the store stays exactly Hermitian.
 // xa (finite Q != 0, else null): the exchange term above, vqx[0] = V(Q), added
 // before the factor (matrizelbsekqtemp, src/module_bse.F90:664-780).
-template <class R, int NC, int NV, int TK>
+// EXT: compiled with the temperature and exchange terms; the plain kernel
+// (EXT = false) carries neither, so their registers do not cost it occupancy.
+template <class R, int NC, int NV, int TK, bool EXT>
 __global__ void __launch_bounds__(kThreads)
 bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
                  const double *__restrict__ kpt, const double *__restrict__ esel,
@@ -199,7 +201,7 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
                  const double *__restrict__ vqx) {
     using C = typename CT<R>::C;
     constexpr int NB = NC + NV, NCV = NC * NV, NC2 = NC * NC, NV2 = NV * NV;
-    const double vx = xa ? vqx[0] : 0.0;
+    const double vx = (EXT && xa) ? vqx[0] : 0.0;
     constexpr int S = TK * TK / kThreads;              // pairs per thread
     constexpr int NPP = pair_slots<TK>();
     static_assert(kThreads % TK == 0 && S >= 1, "tile must split evenly over the workgroup");
@@ -315,13 +317,13 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
                         const int k = k0 + kl;
                         double de = esel[(size_t)k * NB + NV + ci] - esel[(size_t)k * NB + vi];
                         double w = (double)vq;
-                        if (xa) w -= vx * (xa[I].x * xa[I].x + xa[I].y * xa[I].y);
-                        val = mk<C>(fdeh ? de + w * fdeh[I] : de + w, 0.0);
+                        if (EXT && xa) w -= vx * (xa[I].x * xa[I].x + xa[I].y * xa[I].y);
+                        val = mk<C>((EXT && fdeh) ? de + w * fdeh[I] : de + w, 0.0);
                     }
                 } else {
                     val = pair_value(vq, a[cj], b[vj]);
-                    if (xa) sub_exchange(val, xa, vx, I, J);
-                    if (fdeh) {
+                    if (EXT && xa) sub_exchange(val, xa, vx, I, J);
+                    if (EXT && fdeh) {
                         const R f = (R)fdeh[I < J ? I : J];
                         val.x *= f;
                         val.y *= f;
@@ -357,8 +359,8 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
 #pragma unroll
                 for (int vj = 0; vj < NV; ++vj) {
                     C val = pair_value(vq, a[cj], b[vj]);
-                    if (xa) sub_exchange(val, xa, vx, I, Jb + cj * NV + vj);
-                    if (fdeh) {     // J < I off the diagonal tiles: the factor of J
+                    if (EXT && xa) sub_exchange(val, xa, vx, I, Jb + cj * NV + vj);
+                    if (EXT && fdeh) {     // J < I off the diagonal tiles: the factor of J
                         const R f = (R)fdeh[Jb + cj * NV + vj];
                         val.x *= f;
                         val.y *= f;
@@ -401,8 +403,13 @@ void launch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const double *esel
     unsigned *hm = sizeof(R) == sizeof(float) ? c.bse.hmax.p : nullptr;
     const double *fd = c.bse.temperature >= 0.0 ? c.bse.fdeh.p : nullptr;
     const cd *xa = c.bse.xa.n ? c.bse.xa.p : nullptr;
-    hipLaunchKernelGGL((bse_build_kernel<R, NC, NV, TK>), dim3((unsigned)ntile), dim3(kThreads), shmem, c.stream,
-                       nk, norb, ld, textbook, cd_, c.kpt.p, esel, usel, H, hm, fd, xa, c.bse.vqx.p);
+    if (fd || xa)
+        hipLaunchKernelGGL((bse_build_kernel<R, NC, NV, TK, true>), dim3((unsigned)ntile), dim3(kThreads), shmem,
+                           c.stream, nk, norb, ld, textbook, cd_, c.kpt.p, esel, usel, H, hm, fd, xa, c.bse.vqx.p);
+    else
+        hipLaunchKernelGGL((bse_build_kernel<R, NC, NV, TK, false>), dim3((unsigned)ntile), dim3(kThreads), shmem,
+                           c.stream, nk, norb, ld, textbook, cd_, c.kpt.p, esel, usel, H, hm, nullptr, nullptr,
+                           nullptr);
     WTB_LAUNCH_CHECK();
     if (ld > N) {
         const long tot = (ld - N) * N;
