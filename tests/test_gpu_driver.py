@@ -56,7 +56,8 @@ def honeycomb_run():
     cards = ["BSE= T", "SPEC= T", "DTDIAG= T", "CPOL= T", "SYSDIM= \"2D\"", "NGX= 12", "NGY= 12", "NGZ= 1",
              "NBANDSC= 1", "NBANDSV= 1", "COULOMB_POT= V2DRK", "R_0= 10.0", "EDIEL= 1.0", "EDIEL_T= 1.0",
              "EDIEL_B= 1.0", "EDIEL_Z= 1.0", "LC= 1.0", "KTOL= 0.001", "CSHIFT= 0.05", "ENSPECI= 3.0",
-             "ENSPECF= 7.0", "NESPEC= 400", "NEV= 144", "PRECISION= FP64"]
+             "ENSPECF= 7.0", "NESPEC= 400", "NEV= 144", "PRECISION= FP64", "BSE_WF= T", "EXC_WF_I= 1",
+             "EXC_WF_F= 4"]
     return run_driver(cards)
 
 
@@ -389,3 +390,30 @@ def test_bz_activity_driver_run():
         assert np.abs(got[:, 3] - ref[:, a]).max() <= 1e-9 * max(1.0, np.abs(ref[:, a]).max()), tag
     dic = load(os.path.join(d, "dichroism.dat"))
     assert np.abs(dic[:, 3] - (ref[:, 3] - ref[:, 4]) / (ref[:, 3] + ref[:, 4])).max() < 1e-9
+
+
+def test_exciton_wavefunction_files(honeycomb_run):
+    """BSE_WF=T: exc_wf_1..4.dat hold (nocp, k, c and v band, Re, Im) per
+    state; the amplitudes' |psi|^2 summed over each degenerate level inside
+    1..4 equal the diagonal of the oracle's projector on that level."""
+    d, m = honeycomb_run
+    say("case driver exciton wavefunctions")
+    mesh, k, ref, w, X, dip = oracle_case(m)
+    levels = np.split(np.arange(144), np.where(np.diff(w) > 1e-6)[0] + 1)
+    inside = [lv for lv in levels if lv[-1] < 4]
+    assert inside
+    rows = {}
+    for n in range(1, 5):
+        f = os.path.join(d, f"exc_wf_{n}.dat")
+        a = load(f)
+        assert a.shape == (144, 8)
+        assert np.all(a[:, 0] == 1) and np.all(a[:, 4] == 2) and np.all(a[:, 5] == 1)
+        assert np.abs(a[:, 1:4] - k).max() <= 5.1e-5
+        with open(f) as fh:
+            head = fh.read().splitlines()[1].split()
+        assert abs(float(head[-1]) - w[n - 1]) < 1e-8
+        rows[n] = a[:, 6] + 1j * a[:, 7]
+    for lv in inside:
+        got = sum(np.abs(rows[i + 1]) ** 2 for i in lv)
+        want = (np.abs(X[:, lv]) ** 2).sum(1)
+        assert np.abs(got - want).max() < 1e-3 * want.max(), lv

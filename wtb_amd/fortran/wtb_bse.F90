@@ -50,6 +50,9 @@ module wtb_cards
     logical :: spdiel = .false., tmcoef = .false.
     ! OPT_BZ=T (spoptpolbz): per-k optical activity maps
     logical :: sppolbz = .false.
+    ! BSE_WF=T (excwf2): exciton amplitudes of states EXC_WF_I= .. EXC_WF_F=
+    logical :: bsewf = .false.
+    integer :: excwf0 = 1, excwff = 2
     double precision :: numdos = 6001.0d0, sme = 0.08d0
     character(len=70) :: orbw = "non declared file"
     character(len=1) :: dft = "V"
@@ -97,6 +100,9 @@ contains
             case ("DOS=");          read(b, *) doscalc
             case ("DIEL=");         read(b, *) spdiel
             case ("OPT_BZ=");       read(b, *) sppolbz
+            case ("BSE_WF=");       read(b, *) bsewf
+            case ("EXC_WF_I=");     read(b, *) excwf0
+            case ("EXC_WF_F=");     read(b, *) excwff
             case ("TMCOEF=");       read(b, *) tmcoef
             case ("NEDOS=");        read(b, *) numdos
             case ("SIGMA=");        read(b, *) sme
@@ -533,6 +539,7 @@ program wtb_bse
     integer, allocatable :: order(:)
     double precision :: egap, spinf, tcor
     double precision, allocatable :: fdeh(:)
+    complex(c_double_complex), allocatable, target :: hev(:, :)
     character(len=2), parameter :: tens(6) = ["xx", "yy", "zz", "xy", "xz", "yz"]
     character(len=2), parameter :: pols(5) = ["x ", "y ", "z ", "sp", "sm"]
 
@@ -609,7 +616,12 @@ program wtb_bse
     if (bse .or. bset) then
     if (bset) call wtb_check(ctx, wtb_set_temperature(ctx, temp), "wtb_set_temperature")
     call wtb_check(ctx, wtb_bse_build(ctx, coul, prec, conv), "wtb_bse_build")
-    call wtb_check(ctx, wtb_bse_solve(ctx, nevs, 1.0d-8, 200, ws, c_null_ptr, c_loc(rep)), "wtb_bse_solve")
+    if (bsewf) then
+        allocate(hev(dimbse, nevs))
+        call wtb_check(ctx, wtb_bse_solve(ctx, nevs, 1.0d-8, 200, ws, c_loc(hev), c_loc(rep)), "wtb_bse_solve")
+    else
+        call wtb_check(ctx, wtb_bse_solve(ctx, nevs, 1.0d-8, 200, ws, c_null_ptr, c_loc(rep)), "wtb_bse_solve")
+    end if
     call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_X, WTB_X, act(:, 1)), "wtb_oscillator xx")
     call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_Y, WTB_Y, act(:, 2)), "wtb_oscillator yy")
     call wtb_check(ctx, wtb_oscillator(ctx, nevs, WTB_Z, WTB_Z, act(:, 3)), "wtb_oscillator zz")
@@ -741,6 +753,7 @@ program wtb_bse
     end do
     close(301)
     close(302)
+    if (bsewf) call exciton_wf_files()
     end if
 
     ! ---- spectra from the written (4-decimal) file, optical constants, lifetimes
@@ -841,6 +854,40 @@ contains
         close(ub)
         close(ul)
     end subroutine exciton_bands
+
+    ! BSE_WF=T (excwf2, module_bse.F90:374-408; called at :2463-2468): one
+    ! OUTPUT//exc_wf_<n>.dat per exciton n = EXC_WF_I .. EXC_WF_F (of the NEV
+    ! solved).  The reference's row format "(I,3F15.4,2I,2E15.4)" has no
+    ! integer width, which only some compilers accept; the width 12 used here
+    ! is the default such compilers apply to a 4-byte integer.
+    subroutine exciton_wf_files()
+        character(len=200) :: file1
+        integer :: n, u, erro, kk, cc, vv, j
+        do n = excwf0, min(excwff, nevs)
+            write(file1, '(a7,I0,a4)') 'exc_wf_', n, '.dat'
+            open(newunit=u, file=trim(outputfolder)//trim(file1), status='unknown', iostat=erro)
+            if (erro /= 0) stop "Error opening wf BSE output file"
+            write(u, *) "#", " ", "excitonic state", n
+            write(u, *) "#", " ", "exciton energy:", ws(n)
+            write(u, *) "#", " ", "number occupied states", nocpk(1), "(not valid for metallic systems)"
+            write(u, *) "#", " ", "Number of conduction states", "", nc
+            write(u, *) "#", " ", "Number of valence states", "", nv
+            write(u, *) "#", " ", "Number of kpoints", "", nk
+            write(u, *) "#", " ", "nocpk", " ", "kx", " ", "ky", " ", "kz", " ", "nc", " ", "nv", " ", "re_wf", " ", &
+                "imag_wf"
+            j = 0
+            do kk = 1, nk
+                do cc = 1, nc
+                    do vv = 1, nv
+                        j = j + 1
+                        write(u, "(I12,3F15.4,2I12,2E15.4)") nocpk(kk), kpt(kk, 1), kpt(kk, 2), kpt(kk, 3), &
+                            nocpk(kk) + cc, nocpk(kk) - nv + vv, real(hev(j, n)), aimag(hev(j, n))
+                    end do
+                end do
+            end do
+            close(u)
+        end do
+    end subroutine exciton_wf_files
 
     ! OPT_BZ=T (spoptpolbz, module_bse.F90:4532-4797): per k the activity
     ! sums of x, y, z, sp and sm in bz_act_*.dat and (sp - sm)/(sp + sm) in
