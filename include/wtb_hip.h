@@ -42,7 +42,7 @@ extern "C" {
 #define WTB_V2DOH 6
 #define WTB_V2DRK 7
 #define WTB_V0DT  8
-#define WTB_V1DT  9              /* not available: WTB_ERR_UNSUPPORTED */
+#define WTB_V1DT  9              /* build and solve; the gradients return WTB_ERR_UNSUPPORTED */
 
 /* precision of the BSE kernel store and the Krylov matvecs */
 #define WTB_FP64   0             /* complex FP64 store, FP64 MFMA matvec                     */
@@ -252,8 +252,14 @@ int wtb_grad_spectrum(wtb_ctx *ctx, int comp, double cshift, double gamma, int n
                       double *hop_im_grad);
 
 /* Drop-in for slepc_solver(HX,EX,HE,NX,NE) (module_slepc_solver.F90:12-97):
- * uploads the caller's full host matrix HX(NX,NX), returns the NE
- * smallest-magnitude eigenpairs.  EX(NE) ascending in |E|; HE(NX,NE). */
+ * uploads the caller's full Hermitian host matrix HX(NX,NX) (FP64, device 0)
+ * and returns the NE eigenpairs of smallest magnitude (EPS_SMALLEST_MAGNITUDE,
+ * :65), tolerance 1e-8, EX(NE) ascending in |E| (ties in ascending E),
+ * HE(NX,NE).  Up to NX = 1024 all eigenpairs are computed and the set is
+ * chosen among them, so an indefinite spectrum gets its interior pairs.
+ * Above, the block solver's lowest algebraic pairs are returned, which are
+ * the smallest-magnitude set for a non-negative spectrum (bound excitons);
+ * a negative lowest eigenvalue gives WTB_ERR_UNSUPPORTED. */
 int wtb_slepc_solver_compat(const double *HX, double *EX, double *HE, int NX, int NE);
 
 #ifdef __cplusplus

@@ -315,3 +315,33 @@ def test_optics_and_oscillator():
         assert np.abs(act - ar).max() < 1e-9 * max(1, np.abs(ar).max())
         assert np.all(act[16:] == 0)
     ctx.close()
+
+
+def test_mixed_solve_honours_tol():
+    """MIXED counts a pair as converged only at the caller's tolerance: at
+    1e-5 one int8 refinement step suffices; at 1e-8 the solver keeps
+    iterating on the wanted columns with the accurate (int8-digit) product
+    until the residual against the FP32 store is below 1e-8; with too few
+    iterations allowed it reports WTB_ERR_NOCONV instead of claiming
+    convergence at the reduced-precision floor."""
+    say("case mixed tol semantics")
+    mesh = (20, 20, 1)
+    m = hex_model(mesh)
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, 2, 2)
+    ctx.bse_build(COUL, hip.FP64)
+    E64, _, _ = ctx.bse_solve(32, tol=1e-10)
+    ctx.bse_build(COUL, hip.MIXED)
+    H32 = ctx.get_block(0, ctx.N, 0, ctx.N)            # the FP32 store, exactly
+    for tol in (1e-5, 1e-8):
+        E, X, rep = ctx.bse_solve(32, tol=tol)
+        r = np.linalg.norm(H32 @ X - X * E, axis=0) / np.abs(E)
+        say("tol", tol, "report", rep["max_rel_residual"], "numpy residual", r.max(), "iterations", rep["iterations"])
+        assert rep["nconv"] == 32 and rep["max_rel_residual"] < tol
+        assert r.max() < 1.5 * tol
+        assert np.abs(E - E64).max() < 1e-4
+    with pytest.raises(hip.WTBError) as e:
+        ctx.bse_solve(32, tol=1e-8, max_it=2)
+    assert e.value.code == -3
+    E, X, rep = ctx.bse_solve(32, tol=1e-8, max_it=2, check=False)
+    assert rep["nconv"] < 32 and rep["max_rel_residual"] >= 1e-8
+    ctx.close()

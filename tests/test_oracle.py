@@ -594,3 +594,186 @@ def test_optics_bz_against_the_dipoles():
         assert np.allclose(act[:, a] * (de * de + eta * eta), raw, rtol=1e-10)
     norm2 = float(np.float32(1.0) / np.sqrt(np.float32(2.0))) ** 2
     assert np.allclose(act[:, 3] + act[:, 4], 2 * norm2 * (act[:, 0] + act[:, 1]), rtol=1e-12)
+
+
+# ------------------- truncated and screened potentials --------------------
+# Each branch is checked against the Fourier integral of the potential it
+# represents, evaluated by quadrature, or against another potential it
+# reduces to.  The lattices give |a1| != |a2| != |a3| and a tilted a3, so a
+# wrong cell length, cutoff or cell volume fails.
+
+def _vq(c, dk, rlat, ngrid):
+    return O.coulomb_v(c, np.asarray(dk, float), np.zeros(3), np.array(ngrid, np.int32), rlat)
+
+
+def test_v2dt_is_the_slab_truncated_coulomb_integral():
+    """v2dt (module_coulomb.F90:135-193) is the 3-D transform of 1/r cut to
+    |z| < rc = rlat(3,3)/2 without the 4 pi: (1/g) int_0^rc cos(gz z) e^{-g z} dz
+    for in-plane g > 0; for g = 0 the transform's g-independent part
+    -int_0^rc z cos(gz z) dz (the 1/g term of a charged sheet removed), which
+    is -rc^2/2 at gz = 0."""
+    from scipy.integrate import quad
+    rlat = np.array([[3.1, 0.0, 0.0], [-1.2, 2.9, 0.0], [0.4, 0.3, 18.0]])
+    ngrid = (7, 5, 1)
+    pre = O.cic() / (35 * abs(np.linalg.det(rlat)))
+    rc = 9.0
+    c = O.coulomb("V2DT", ktol=0.001)
+    for gx, gy, gz in ((0.3, 0.0, 0.0), (0.05, 0.02, 0.4), (0.7, -0.4, 0.9), (0.01, 0.0, 2.5)):
+        g = math.hypot(gx, gy)
+        f, _ = quad(lambda z: math.cos(gz * z) * math.exp(-g * z), 0.0, rc, limit=400, epsabs=0, epsrel=1e-12)
+        ref = pre * f / g
+        got = _vq(c, (gx, gy, gz), rlat, ngrid)
+        got_neg = _vq(c, (gx, gy, -gz), rlat, ngrid)      # |gz| enters
+        assert abs(got - ref) < 1e-10 * abs(ref), (gx, gy, gz)
+        assert got_neg == got
+    for gz in (0.05, 0.8, 3.0):
+        f, _ = quad(lambda z: z * math.cos(gz * z), 0.0, rc, limit=400, epsabs=0, epsrel=1e-12)
+        ref = -pre * f
+        assert abs(_vq(c, (0.0, 0.0, gz), rlat, ngrid) - ref) < 1e-10 * abs(ref), gz
+    assert abs(_vq(c, (0.0, 0.0, 0.0), rlat, ngrid) - (-pre * rc * rc / 2)) < 1e-13 * abs(pre * rc * rc)
+
+
+def test_v0dt_is_the_sphere_truncated_coulomb_integral():
+    """v0dt (module_coulomb.F90:197-238): the transform of 1/r cut to a sphere
+    of radius cr = min |a_i| / 2 without the 4 pi, int_0^cr sin(q r)/q dr,
+    and cr^2/2 = int_0^cr r dr for q below KTOL."""
+    from scipy.integrate import quad
+    rlat = np.array([[4.2, 0.0, 0.0], [0.0, 3.0, 0.5], [0.3, 0.0, 5.0]])
+    ngrid = (4, 4, 4)
+    pre = O.cic() / (64 * abs(np.linalg.det(rlat)))
+    cr = 0.5 * min(np.linalg.norm(rlat, axis=1))
+    c = O.coulomb("V0DT", ktol=0.001)
+    for dk in ((0.3, 0.0, 0.0), (0.2, -0.5, 0.7), (1.9, 0.4, 0.0)):
+        q = np.linalg.norm(dk)
+        f, _ = quad(lambda r: math.sin(q * r) / q, 0.0, cr, epsabs=0, epsrel=1e-13)
+        assert abs(_vq(c, dk, rlat, ngrid) - pre * f) < 1e-11 * abs(pre * f), dk
+    assert abs(_vq(c, (2e-4, 0, 0), rlat, ngrid) - pre * cr * cr / 2) < 1e-13 * abs(pre * cr * cr)
+
+
+def test_v2dohono_reduces_to_rytova_keldysh_without_polarisability():
+    """v2dohono (module_coulomb.F90:354-387) is the Rytova-Keldysh potential
+    with r0 = 0 in a uniform medium of permittivity EDIEL_Z at height W_COUL;
+    the two functions are independent restatements.  Scaling: 1/EDIEL_Z and
+    e^{-q W_COUL}; zero below KTOL."""
+    m = T.honeycomb_two_band()
+    rlat = m.rlat.copy()
+    rlat[1] = [-1.0, 2.4, 0.0]                                   # |a1| != |a2|
+    ngrid = (9, 6, 1)
+    for ez, w in ((1.0, 0.0), (2.5, 0.0), (3.0, 0.8)):
+        oh = O.coulomb("V2DOH", ez=ez, w=w, ediel=(7.0, 5.0, 3.0), lc=4.0, r0=11.0)
+        rk = O.coulomb("V2DRK", r0=0.0, ediel=(ez, ez, ez), ez=ez, w=w, lc=4.0)
+        for dk in ((0.01, 0.0, 0.0), (0.2, 0.3, 0.0), (1.1, -0.6, 0.0)):
+            a, b = _vq(oh, dk, rlat, ngrid), _vq(rk, dk, rlat, ngrid)
+            assert abs(a - b) < 1e-13 * abs(b), (ez, w, dk)
+    base = _vq(O.coulomb("V2DOH", ez=1.0, w=0.0), (0.4, 0.0, 0.0), rlat, ngrid)
+    assert abs(_vq(O.coulomb("V2DOH", ez=4.0, w=0.0), (0.4, 0.0, 0.0), rlat, ngrid) - base / 4) < 1e-14 * abs(base)
+    assert abs(_vq(O.coulomb("V2DOH", ez=1.0, w=2.0), (0.4, 0.0, 0.0), rlat, ngrid)
+               - base * math.exp(-0.8)) < 1e-14 * abs(base)
+    assert _vq(O.coulomb("V2DOH", ez=1.0, w=0.0), (5e-4, 0.0, 0.0), rlat, ngrid) == 0.0
+
+
+def test_v2dk_keldysh_form_and_small_q_value():
+    """v2dk (module_coulomb.F90:8-60).  Above KTOL it is the Keldysh form
+    cic / (N A ed q (1 + r0 q)) with ed = (EDIEL_T + EDIEL_B)/2 and
+    r0 = (EDIEL - 1) LC / (EDIEL_T + EDIEL_B): the Rytova-Keldysh function in a
+    uniform medium ed with polarisability length ed r0 and W_COUL = 0.  Below
+    KTOL the regularised value vbz (cic/ed) (1.76 a0 sqrt(n1 n2)/(2 pi) + r0),
+    a0 the mean of |a1| and |a2|, 1.76 and pi single precision: its intercept
+    at EDIEL = 1 (r0 = 0) and its slope in LC pin the two terms."""
+    rlat = np.array([[3.0, 0.0, 0.0], [-1.3, 2.5, 0.0], [0.0, 0.0, 15.0]])
+    ngrid = (8, 6, 1)
+    area = np.linalg.norm(np.cross(rlat[0], rlat[1]))
+    et, eps, eb, lc = 2.0, 5.0, 4.0, 6.5
+    ed = 0.5 * (et + eb)
+    r0 = (eps - 1.0) * lc / (et + eb)
+    k = O.coulomb("V2DK", ediel=(et, eps, eb), lc=lc)
+    rk = O.coulomb("V2DRK", r0=ed * r0, ediel=(ed, ed, ed), ez=ed, w=0.0, lc=lc)
+    for dk in ((0.002, 0.0, 0.0), (0.3, 0.1, 0.0), (1.4, -0.9, 0.0)):
+        a, b = _vq(k, dk, rlat, ngrid), _vq(rk, dk, rlat, ngrid)
+        assert abs(a - b) < 1e-13 * abs(b), dk
+    vbz = 1.0 / (48 * area)
+    a0 = 0.5 * (np.linalg.norm(rlat[0]) + np.linalg.norm(rlat[1]))
+    small = (5e-4, 0.0, 0.0)
+    v1 = _vq(O.coulomb("V2DK", ediel=(et, 1.0, eb), lc=lc), small, rlat, ngrid)
+    ref1 = vbz * (O.cic() / ed) * float(np.float32(1.76)) * a0 * math.sqrt(48.0) / (2 * O.pi_f32())
+    assert abs(v1 - ref1) < 1e-13 * abs(ref1)
+    va = _vq(O.coulomb("V2DK", ediel=(et, eps, eb), lc=lc), small, rlat, ngrid)
+    vb = _vq(O.coulomb("V2DK", ediel=(et, eps, eb), lc=2 * lc), small, rlat, ngrid)
+    slope = vbz * (O.cic() / ed) * (eps - 1.0) / (et + eb)
+    assert abs((vb - va) / lc - slope) < 1e-11 * abs(slope)
+    assert abs((va - v1) - slope * lc) < 1e-11 * abs(slope * lc)
+
+
+# ------------------- the reference's own TB input --------------------------
+
+def test_reference_mos2_file_read_and_hermitian():
+    """examples/example-3/tb_mos2.dat (copied as tests/golden/tb_mos2.dat): the
+    reader gives SOC, norb = 22, nvec = 147 with six R of degeneracy 2, and
+    H(-R) = H(R)^dagger for every R, so every H(k) is Hermitian; the oracle's
+    H(k) (with 1/deg) equals the model's own Fourier sum and the bands leave
+    E_F in the gap at every k (nocp = 14)."""
+    from golden.make_bse_golden import MOS2_FILE
+    m = T.read_tb_file(MOS2_FILE)
+    assert m.systype == "SOC" and m.norb == 22 and m.nvec == 147
+    assert abs(m.efermi - (-1.2008)) < 1e-12 and m.scissor == 0.0
+    assert np.count_nonzero(m.deg == 2) == 6 and np.count_nonzero(m.deg == 1) == 141
+    assert tuple(m.rvec[0]) == (-8, -4, 0) and tuple(m.rvec[-1]) == (8, 4, 0)
+    where = {tuple(r): i for i, r in enumerate(m.rvec)}
+    for i, r in enumerate(m.rvec):
+        j = where[tuple(-r)]
+        assert np.array_equal(m.hop[j], m.hop[i].conj().T)
+        assert m.deg[i] == m.deg[j]
+    om = O.Model(m, flags=0)
+    k = O.mp_mesh(5, 5, 1, m.rlat)
+    for kk in (0, 7, 24):
+        h = O.hk(om, k[kk])
+        ph = np.exp(1j * (m.rvec @ m.rlat) @ k[kk])
+        href = np.einsum("r,rmn->mn", ph / m.deg, m.hop) - m.efermi * np.eye(22)
+        assert np.abs(h - href).max() < 1e-12
+        assert np.abs(h - h.conj().T).max() < 1e-13
+    r = O.hk_eig(O.Model(m), k, 2, 2)
+    assert r["status"] == 0 and np.all(r["nocp"] == 14)
+
+
+def test_sp_tb_file_merges_up_and_down_blocks(tmp_path):
+    """An SP file (module_hamiltonian.F90:63-162) holds an up block and a down
+    block, each with its own R list; the reader appends the down list to the
+    up list (nvec = nvec_up + nvec_down) with the up hoppings in the first
+    norb orbitals and the down hoppings in the last, so H(k) is the block
+    diagonal of the two spin Hamiltonians.  Written here from two halves of a
+    seeded model (the down block with its R list reversed) and read back."""
+    base = hex_model((3, 3, 1))
+    n = 6
+    up = base.hop[:, :n, :n].copy()
+    dn = base.hop[::-1, n:2 * n, n:2 * n].copy()
+    rv_dn = base.rvec[::-1].copy()
+    p = tmp_path / "tb_sp.dat"
+    with open(p, "w") as f:
+        f.write("SP\n   0.10\n   0.25\n")
+        for row in base.rlat:
+            f.write(" ".join(f"{x:.7f}" for x in row) + "\n")
+        for blk, rv, label in ((up, base.rvec, "up"), (dn, rv_dn, "down")):
+            f.write(f" {label} block\n{n}\n{base.nvec}\n")
+            f.write(" ".join(str(int(x)) for x in base.deg) + "\n")
+            for i in range(base.nvec):
+                for b in range(n):
+                    for a in range(n):
+                        v = blk[i, a, b]
+                        r = rv[i]
+                        f.write(f"{r[0]} {r[1]} {r[2]} {a + 1} {b + 1} {v.real:.17e} {v.imag:.17e}\n")
+    m = T.read_tb_file(str(p))
+    nv = base.nvec
+    assert m.systype == "SP" and m.norb == 2 * n and m.nvec == 2 * nv
+    assert m.scissor == 0.1 and m.efermi == 0.25
+    assert np.array_equal(m.rvec[:nv], base.rvec) and np.array_equal(m.rvec[nv:], rv_dn)
+    assert np.array_equal(m.hop[:nv, :n, :n], up) and np.array_equal(m.hop[nv:, n:, n:], dn)
+    assert np.all(m.hop[:nv, n:, :] == 0) and np.all(m.hop[:nv, :, n:] == 0)
+    assert np.all(m.hop[nv:, :n, :] == 0) and np.all(m.hop[nv:, :, :n] == 0)
+    # H(k) is block diagonal: the up and down spin Hamiltonians
+    k = O.mp_mesh(3, 3, 1, m.rlat)[4]
+    h = O.hk(O.Model(m, flags=0), k)
+    ph = np.exp(1j * (base.rvec @ m.rlat) @ k)
+    hu = np.einsum("r,rmn->mn", ph, up) - 0.25 * np.eye(n)
+    hd = np.einsum("r,rmn->mn", ph[::-1], dn) - 0.25 * np.eye(n)
+    assert np.abs(h[:n, :n] - hu).max() < 1e-12 and np.abs(h[n:, n:] - hd).max() < 1e-12
+    assert np.all(h[:n, n:] == 0)

@@ -440,21 +440,51 @@ int wtb_slepc_solver_compat(const double *HX, double *EX, double *HE, int NX, in
         c.bse.ld = ld;
         c.bse.precision = WTB_FP64;
         c.bse.valid = true;
-        wtb_report r{};
-        lobpcg(c, NE, 1e-8, 200, WTB_FP64, &r);
-        std::vector<double> e(NE);
-        std::vector<cd> x((size_t)N * NE);
-        copy(c, e.data(), c.sol.evals.p, sizeof(double) * NE, hipMemcpyDeviceToHost);
-        copy(c, x.data(), c.sol.evecs.p, sizeof(cd) * x.size(), hipMemcpyDeviceToHost);
-        // order by |E| as EPS_SMALLEST_MAGNITUDE
-        std::vector<int> ord(NE);
-        for (int i = 0; i < NE; ++i) ord[i] = i;
+        // EPS_SMALLEST_MAGNITUDE (module_slepc_solver.F90:65): the NE eigenvalues
+        // of least |E|.  Up to 1024 every eigenpair comes from the dense solver
+        // and the set is chosen from all of them, so an indefinite spectrum
+        // gets its interior pairs.  Above, the block solver returns the lowest
+        // algebraic pairs: the same set whenever the spectrum is non-negative
+        // (the BSE case, bound excitons above zero); a negative lowest
+        // eigenvalue is refused rather than answered with the wrong set.
+        std::vector<double> e;
+        std::vector<cd> x;
+        int nget = NE;
+        if (N <= 1024) {
+            nget = (int)N;
+            DBuf<cd> A, Z;
+            DBuf<double> W;
+            A.alloc((size_t)N * N);
+            Z.alloc((size_t)N * N);
+            W.alloc(N);
+            bse_block_to_c64(c, 0, N, 0, N, A.p, N);
+            heev_lowest(c, (int)N, (int)N, A.p, (int)N, W.p, Z.p, (int)N);
+            e.resize(N);
+            x.resize((size_t)N * N);
+            copy(c, e.data(), W.p, sizeof(double) * N, hipMemcpyDeviceToHost);
+            copy(c, x.data(), Z.p, sizeof(cd) * x.size(), hipMemcpyDeviceToHost);
+        } else {
+            wtb_report r{};
+            lobpcg(c, NE, 1e-8, 200, WTB_FP64, &r);
+            e.resize(NE);
+            x.resize((size_t)N * NE);
+            copy(c, e.data(), c.sol.evals.p, sizeof(double) * NE, hipMemcpyDeviceToHost);
+            copy(c, x.data(), c.sol.evecs.p, sizeof(cd) * x.size(), hipMemcpyDeviceToHost);
+            // a Ritz value bounds the lowest eigenvalue from above, so a
+            // negative one proves the matrix indefinite, converged or not
+            if (e[0] < 0.0)
+                throw Error{WTB_ERR_UNSUPPORTED, "indefinite matrix above N = 1024: the smallest-magnitude "
+                                                 "pairs are interior eigenvalues"};
+            if (r.nconv < NE) throw Error{WTB_ERR_NOCONV, "not converged"};
+        }
+        // ascending |E|, ties by position (stable)
+        std::vector<int> ord(nget);
+        for (int i = 0; i < nget; ++i) ord[i] = i;
         std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return std::fabs(e[a]) < std::fabs(e[b]); });
         for (int i = 0; i < NE; ++i) {
             EX[i] = e[ord[i]];
             if (HE) std::memcpy(HE + 2 * (size_t)N * i, &x[(size_t)N * ord[i]], sizeof(cd) * N);
         }
-        if (r.nconv < NE) throw Error{WTB_ERR_NOCONV, "not converged"};
     });
     wtb_destroy(ctx);
     return st;
