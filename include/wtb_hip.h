@@ -213,6 +213,13 @@ int wtb_bse_get_block(wtb_ctx *ctx, long i0, long ni, long j0, long nj, double *
 int wtb_bse_matvec(wtb_ctx *ctx, int ncol, const double *X, double *Y, int precision);
 int wtb_bse_diag(wtb_ctx *ctx, double *diag);
 
+/* Device time in ms of the kernels of the last wtb_hk_eig, wtb_bse_build,
+ * wtb_optics and wtb_oscillator calls (ms[0..3]), measured with HIP events on
+ * the context stream.  No counterpart in the reference, whose MSG timers
+ * (mpi_p.F90:134-160) print per-stage CPU time; its callers are benchmarks
+ * and roofline reports. */
+int wtb_stage_times(wtb_ctx *ctx, double *ms);
+
 /* Gradients of L = sum_n w[n] E_n over the nev states of the last
  * wtb_bse_solve (Hellmann-Feynman contractions x^H dH x evaluated per (k,k')
  * tile, chained through the TB eigenvector adjoint).  No counterpart in the

@@ -364,6 +364,20 @@ def matvec_rows(h, x):
     return y
 
 
+def krylov_schur(h, nev, tol=1e-8, max_it=200, seed=1, vectors=True):
+    """Krylov-Schur for the nev least-|E| pairs of the row-major Hermitian h
+    (slepc_solver's settings): (E, X or None, restarts, matvecs, nconv)."""
+    h = np.ascontiguousarray(h, np.complex128)
+    N = h.shape[0]
+    e = np.zeros(nev)
+    x = np.zeros((nev, N), np.complex128) if vectors else None
+    its = C.c_int(0)
+    nmv = C.c_long(0)
+    nconv = lib().or_krylov_schur(C.c_long(N), _p(h), int(nev), C.c_double(tol), int(max_it), C.c_uint(seed),
+                                  _p(e), _p(x) if vectors else None, C.byref(its), C.byref(nmv))
+    return e, (x.T if vectors else None), its.value, nmv.value, nconv
+
+
 def num_threads():
     return lib().or_num_threads()
 

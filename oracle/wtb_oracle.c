@@ -658,3 +658,188 @@ int or_num_threads(void) {
     return 1;
 #endif
 }
+
+/* ------------------------- Krylov-Schur (CPU baseline) --------------------- */
+
+/* y = h x for one vector (row-major N x N h), four independent partial sums
+ * per row so the loop is bound by the stream of h, not by one add chain. */
+static void mv1(long N, const double complex *h, const double complex *x, double complex *y) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        const double *hi = (const double *)(h + i * N);
+        const double *xx = (const double *)x;
+        double ar[4] = {0, 0, 0, 0}, ai[4] = {0, 0, 0, 0};
+        long j = 0;
+        for (; j + 4 <= N; j += 4)
+            for (int u = 0; u < 4; ++u) {
+                const double hr = hi[2 * (j + u)], hm = hi[2 * (j + u) + 1];
+                const double xr = xx[2 * (j + u)], xm = xx[2 * (j + u) + 1];
+                ar[u] += hr * xr - hm * xm;
+                ai[u] += hr * xm + hm * xr;
+            }
+        for (; j < N; ++j) {
+            const double hr = hi[2 * j], hm = hi[2 * j + 1], xr = xx[2 * j], xm = xx[2 * j + 1];
+            ar[0] += hr * xr - hm * xm;
+            ai[0] += hr * xm + hm * xr;
+        }
+        y[i] = ((ar[0] + ar[1]) + (ar[2] + ar[3])) + I * ((ai[0] + ai[1]) + (ai[2] + ai[3]));
+    }
+}
+
+/* c[0..m) = V[:, 0..m)^H w, V column-major (ldv = N) */
+static void proj(long N, int m, const double complex *V, const double complex *w, double complex *c) {
+#pragma omp parallel for schedule(static)
+    for (int q = 0; q < m; ++q) {
+        const double complex *v = V + (size_t)q * N;
+        double sr = 0, si = 0;
+        for (long i = 0; i < N; ++i) {
+            const double vr = creal(v[i]), vi = cimag(v[i]), wr = creal(w[i]), wi = cimag(w[i]);
+            sr += vr * wr + vi * wi;
+            si += vr * wi - vi * wr;
+        }
+        c[q] = sr + I * si;
+    }
+}
+
+/* w -= V[:, 0..m) c */
+static void sub(long N, int m, const double complex *V, const double complex *c, double complex *w) {
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < N; ++i) {
+        double complex s = 0;
+        for (int q = 0; q < m; ++q) s += V[(size_t)q * N + i] * c[q];
+        w[i] -= s;
+    }
+}
+
+static double nrm(long N, const double complex *w) {
+    double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (long i = 0; i < N; ++i) s += creal(w[i]) * creal(w[i]) + cimag(w[i]) * cimag(w[i]);
+    return sqrt(s);
+}
+
+/* The nev eigenpairs of least |E| of a dense Hermitian matrix h (row-major
+ * N x N) by the Krylov-Schur method as slepc_solver configures it
+ * (module_slepc_solver.F90:29-74): EPS_HEP, EPS_SMALLEST_MAGNITUDE, ncv =
+ * nev + mpd (mpd = nev below 1000, nev/5 above), relative tolerance tol on
+ * ||H x - l x|| / |l| (SLEPc's EPS_CONV_REL), max_it restarts.  For a
+ * Hermitian matrix Krylov-Schur is thick-restart Lanczos: the basis is kept
+ * orthonormal by two classical Gram-Schmidt passes per step, the projected
+ * matrix is diagonalised at every restart (Jacobi), its Ritz pairs are sorted
+ * by |theta|, the residual of pair i is |beta s_last,i|, and a restart keeps
+ * the converged pairs plus half of the others (SLEPc's default keep = 0.5)
+ * with the last Lanczos vector coupled to them in arrowhead form.  The start
+ * vector is a seeded random vector.
+ * Outputs: e (nev) and x (N x nev column-major) in order of |E|; *its
+ * restarts; *nmv matrix-vector products.  Returns the number of converged
+ * pairs (nev on success). */
+int or_krylov_schur(long N, const double complex *h, int nev, double tol, int max_it, unsigned seed,
+                    double *e, double complex *x, int *its, long *nmv) {
+    const int mpd = nev < 1000 ? nev : nev / 5;
+    int ncv = nev + mpd;
+    if (ncv > N) ncv = (int)N;
+    double complex *V = (double complex *)malloc(sizeof(double complex) * (size_t)N * (ncv + 1));
+    double complex *T = (double complex *)calloc((size_t)ncv * ncv, sizeof(double complex));
+    double complex *A = (double complex *)malloc(sizeof(double complex) * (size_t)ncv * ncv);
+    double complex *S = (double complex *)malloc(sizeof(double complex) * (size_t)ncv * ncv);
+    double complex *c1 = (double complex *)malloc(sizeof(double complex) * (ncv + 1));
+    double complex *c2 = (double complex *)malloc(sizeof(double complex) * (ncv + 1));
+    double complex *Vn = (double complex *)malloc(sizeof(double complex) * (size_t)N * ncv);
+    double *th = (double *)malloc(sizeof(double) * ncv);
+    int *ord = (int *)malloc(sizeof(int) * ncv);
+    /* seeded start vector (xorshift) */
+    unsigned s = seed ? seed : 1u;
+    for (long i = 0; i < N; ++i) {
+        s ^= s << 13; s ^= s >> 17; s ^= s << 5;
+        double a = (double)s / 4294967296.0 - 0.5;
+        s ^= s << 13; s ^= s >> 17; s ^= s << 5;
+        double b = (double)s / 4294967296.0 - 0.5;
+        V[i] = a + I * b;
+    }
+    double n0 = nrm(N, V);
+    for (long i = 0; i < N; ++i) V[i] /= n0;
+    int k = 0, nconv = 0, it;
+    long mv = 0;
+    double beta = 0;
+    for (it = 0; it < max_it; ++it) {
+        /* expand the basis from column k to ncv; T[:, j] from the projections */
+        for (int j = k; j < ncv; ++j) {
+            double complex *w = V + (size_t)(j + 1) * N;
+            mv1(N, h, V + (size_t)j * N, w);
+            ++mv;
+            proj(N, j + 1, V, w, c1);
+            sub(N, j + 1, V, c1, w);
+            proj(N, j + 1, V, w, c2);
+            sub(N, j + 1, V, c2, w);
+            for (int q = 0; q <= j; ++q) c1[q] += c2[q];
+            /* column j of the projected matrix; rows below k of the kept part
+               are the arrowhead, set at the restart */
+            for (int q = (j == k ? k : j - 1); q <= j; ++q) {
+                T[q * ncv + j] = (q == j) ? creal(c1[q]) : c1[q];
+                T[j * ncv + q] = conj(T[q * ncv + j]);
+            }
+            if (j == k)
+                for (int q = 0; q < k; ++q) {      /* arrowhead row from the kept pairs */
+                    T[q * ncv + j] = c1[q];
+                    T[j * ncv + q] = conj(c1[q]);
+                }
+            beta = nrm(N, w);
+            if (beta == 0.0) beta = 1e-300;
+            for (long i = 0; i < N; ++i) w[i] /= beta;
+            if (j + 1 < ncv) {
+                T[j * ncv + j + 1] = beta;
+                T[(j + 1) * ncv + j] = beta;
+            }
+        }
+        /* Ritz pairs, ordered by |theta| (EPS_SMALLEST_MAGNITUDE) */
+        memcpy(A, T, sizeof(double complex) * (size_t)ncv * ncv);
+        or_heev_jacobi(ncv, A, th, S);
+        for (int i = 0; i < ncv; ++i) ord[i] = i;
+        for (int i = 1; i < ncv; ++i) {          /* insertion sort by |theta| (stable) */
+            int t = ord[i], q = i;
+            while (q > 0 && fabs(th[ord[q - 1]]) > fabs(th[t])) { ord[q] = ord[q - 1]; --q; }
+            ord[q] = t;
+        }
+        nconv = 0;
+        for (int i = 0; i < ncv && nconv < nev; ++i) {
+            const double res = fabs(beta) * cabs(S[(size_t)(ncv - 1) * ncv + ord[i]]);
+            if (res < tol * fabs(th[ord[i]])) ++nconv;
+            else break;
+        }
+        if (nconv >= nev || it == max_it - 1) break;
+        /* restart: keep the converged pairs and half of the rest */
+        int keep = nconv + (int)(0.5 * (ncv - nconv));
+        if (keep >= ncv) keep = ncv - 1;
+        if (keep < 1) keep = 1;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < N; ++i)
+            for (int q = 0; q < keep; ++q) {
+                double complex acc = 0;
+                for (int r = 0; r < ncv; ++r) acc += V[(size_t)r * N + i] * S[(size_t)r * ncv + ord[q]];
+                Vn[(size_t)q * N + i] = acc;
+            }
+        memcpy(V, Vn, sizeof(double complex) * (size_t)N * keep);
+        memmove(V + (size_t)keep * N, V + (size_t)ncv * N, sizeof(double complex) * N);
+        memset(T, 0, sizeof(double complex) * (size_t)ncv * ncv);
+        for (int q = 0; q < keep; ++q) T[q * ncv + q] = th[ord[q]];
+        /* the coupling of the kept pairs to the continuation vector enters
+           through the projections of its first product (arrowhead row) */
+        k = keep;
+    }
+    const int nout = nconv < nev ? nconv : nev;
+    for (int q = 0; q < nev; ++q) {
+        e[q] = th[ord[q]];
+        if (x) {
+#pragma omp parallel for schedule(static)
+            for (long i = 0; i < N; ++i) {
+                double complex acc = 0;
+                for (int r = 0; r < ncv; ++r) acc += V[(size_t)r * N + i] * S[(size_t)r * ncv + ord[q]];
+                x[(size_t)q * N + i] = acc;
+            }
+        }
+    }
+    *its = it + 1;
+    *nmv = mv;
+    free(V); free(T); free(A); free(S); free(c1); free(c2); free(Vn); free(th); free(ord);
+    return nout;
+}

@@ -109,6 +109,14 @@ void or_matvec_rows(long ni, long N, int ncol, const double complex *h, const do
                     double complex *y);
 int or_num_threads(void);
 
+/* Krylov-Schur (thick-restart Lanczos) for the nev eigenpairs of least |E| of
+ * the dense Hermitian h (row-major N x N), with slepc_solver's settings
+ * (module_slepc_solver.F90:29-74): ncv = nev + mpd, relative tolerance tol,
+ * max_it restarts.  e (nev), x (N x nev column-major, may be NULL) in order
+ * of |E|.  Returns the converged count; its = restarts, nmv = products. */
+int or_krylov_schur(long N, const double complex *h, int nev, double tol, int max_it, unsigned seed,
+                    double *e, double complex *x, int *its, long *nmv);
+
 #ifdef __cplusplus
 }
 #endif

@@ -226,6 +226,17 @@ __global__ void add_diag_kernel(long N, int ncol, const double *__restrict__ d, 
     Y[o + 1] += d[i] * X[o + 1];
 }
 
+// the FP64 diagonal of a FP32 store (whose stored diagonal is zero)
+void add_diag(Ctx &c, int ncol, const double *x, double *y) {
+    const long N = c.bse.N;
+    for (int c0 = 0; c0 < ncol; c0 += 65535) {
+        int nc = std::min(65535, ncol - c0);
+        hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)((N + 255) / 256), nc), dim3(256), 0, c.stream, N, nc,
+                           c.bse.diag.p, x + 2 * (size_t)c0 * N, y + 2 * (size_t)c0 * N);
+        WTB_LAUNCH_CHECK();
+    }
+}
+
 }  // namespace
 
 // Y = H X with the arithmetic of `precision`:
@@ -254,12 +265,7 @@ void matvec(Ctx &c, int precision, int ncol, const cd *X, cd *Y) {
         else
             launch_hx<float, float>(c.stream, N, s.ld, ncol, (const float *)s.h32.p, x, y, c.kclock,
                                     KernelClock::F32_ON_F32);
-        for (int c0 = 0; c0 < ncol; c0 += 65535) {
-            int nc = std::min(65535, ncol - c0);
-            hipLaunchKernelGGL(add_diag_kernel, dim3((unsigned)((N + 255) / 256), nc), dim3(256), 0, c.stream, N, nc,
-                               s.diag.p, x + 2 * (size_t)c0 * N, y + 2 * (size_t)c0 * N);
-            WTB_LAUNCH_CHECK();
-        }
+        add_diag(c, ncol, x, y);
     }
 }
 

@@ -290,27 +290,36 @@ void launch_f16split(hipStream_t st, int ncb, long N, long ld, int nc, float hin
 // Y = H_offdiag X on the FP32 store with split-FP16 MFMA; the caller adds the
 // FP64 diagonal.  hscale (set at assembly): power of two with
 // max|H| / hscale in [2^12, 2^13).
-void matvec_f16split(Ctx &c, int ncol, const cd *X, cd *Y) {
+namespace {
+
+// Column blocks of up to `blk` columns (<= 128) through the split planes
+// (4 blk ld FP16) and column scales (2 blk floats) given by the caller.
+void f16split_blocks(Ctx &c, int ncol, const cd *X, cd *Y, int blk, _Float16 *planes, float *scales) {
     Bse &s = c.bse;
     const long N = s.N, ld = s.ld;
-    for (int c0 = 0; c0 < ncol; c0 += 128) {
-        const int nc = std::min(ncol - c0, 128);
+    for (int c0 = 0; c0 < ncol; c0 += blk) {
+        const int nc = std::min(ncol - c0, blk);
         const double *x = (const double *)X + 2 * (size_t)c0 * N;
         double *y = (double *)Y + 2 * (size_t)c0 * N;
-        c.ws_split.alloc((size_t)4 * nc * ld);
-        c.ws_colscale.alloc(2 * (size_t)nc);
-        hipLaunchKernelGGL(col_scales_kernel, dim3(nc), dim3(kCS), 0, c.stream, N, x, s.hscale, c.ws_colscale.p,
-                           c.ws_colscale.p + nc);
+        hipLaunchKernelGGL(col_scales_kernel, dim3(nc), dim3(kCS), 0, c.stream, N, x, s.hscale, scales, scales + nc);
         hipLaunchKernelGGL(split_x_kernel, dim3((unsigned)((ld + 255) / 256), nc), dim3(256), 0, c.stream, N, ld,
-                           nc, x, c.ws_colscale.p, c.ws_split.p);
+                           nc, x, scales, planes);
         WTB_LAUNCH_CHECK();
         const int ncb = (nc + 15) / 16;
         c.kclock.begin(KernelClock::F16SPLIT, c.stream);
-        launch_f16split<kKT, kWPB>(c.stream, ncb, N, ld, nc, 1.0f / s.hscale, s.h32.p, c.ws_split.p,
-                                  c.ws_colscale.p + nc, y);
+        launch_f16split<kKT, kWPB>(c.stream, ncb, N, ld, nc, 1.0f / s.hscale, s.h32.p, planes, scales + nc, y);
         c.kclock.end(KernelClock::F16SPLIT, c.stream, 8.0 * (double)ld * N, 8.0 * (double)N * N * nc);
         WTB_LAUNCH_CHECK();
     }
+}
+
+}  // namespace
+
+void matvec_f16split(Ctx &c, int ncol, const cd *X, cd *Y) {
+    const int blk = std::min(ncol, 128);
+    c.ws_split.alloc((size_t)4 * blk * c.bse.ld);
+    c.ws_colscale.alloc(2 * (size_t)blk);
+    f16split_blocks(c, ncol, X, Y, 128, c.ws_split.p, c.ws_colscale.p);
 }
 
 }  // namespace wtb

@@ -193,7 +193,9 @@ int wtb_set_kpoints(wtb_ctx *ctx, int nk, const double *kpt, const int *ngrid) {
 
 int wtb_hk_eig(wtb_ctx *ctx, int nv, int nc, double *eig, double *vec, int *nocp) {
     return guarded(ctx, [&](Ctx &c) {
+        StageTimer tm(c, 0);
         launch_hk_eig(c, nv, nc);
+        tm.stop();
         c.bse.valid = false;
         c.sol.valid = false;
         c.dip_valid = false;
@@ -288,7 +290,9 @@ int wtb_bse_build(wtb_ctx *ctx, const wtb_coulomb *coulomb, int precision, int c
         if (!coulomb) throw Error{WTB_ERR_ARG, "null Coulomb parameters"};
         if (convention != WTB_CONV_REFERENCE && convention != WTB_CONV_TEXTBOOK)
             throw Error{WTB_ERR_ARG, "unknown overlap convention"};
+        StageTimer tm(c, 1);
         launch_bse_build(c, *coulomb, precision, convention);
+        tm.stop();
         WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
     });
 }
@@ -323,7 +327,9 @@ int wtb_bse_solve(wtb_ctx *ctx, int nev, double tol, int max_it, double *E, doub
 
 int wtb_optics(wtb_ctx *ctx, double cshift, double *d) {
     return guarded(ctx, [&](Ctx &c) {
+        StageTimer tm(c, 2);
         launch_optics(c, cshift);
+        tm.stop();
         if (d) {
             const long N = (long)c.bands.nk * c.bands.nc * c.bands.nv;
             copy(c, d, c.dip.p, sizeof(cd) * 5 * (size_t)N, hipMemcpyDeviceToHost);
@@ -342,7 +348,9 @@ int wtb_oscillator(wtb_ctx *ctx, int nev, int comp_a, int comp_b, double *act) {
         const long N = c.bse.N;
         DBuf<double> a;
         a.alloc(nev);
+        StageTimer tm(c, 3);
         launch_oscillator(c, nev, comp_a, comp_b, a.p);
+        tm.stop();
         std::fill(act, act + N, 0.0);
         WTB_HIP_CHECK(hipMemcpyAsync(act, a.p, sizeof(double) * nev, hipMemcpyDeviceToHost, c.stream));
         WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
@@ -379,6 +387,13 @@ int wtb_bse_matvec(wtb_ctx *ctx, int ncol, const double *X, double *Y, int preci
         matvec(c, precision, ncol, x.p, y.p);
         WTB_HIP_CHECK(hipMemcpyAsync(Y, y.p, sizeof(cd) * N * ncol, hipMemcpyDeviceToHost, c.stream));
         WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+    });
+}
+
+int wtb_stage_times(wtb_ctx *ctx, double *ms) {
+    return guarded(ctx, [&](Ctx &c) {
+        if (!ms) throw Error{WTB_ERR_ARG, "null output"};
+        for (int i = 0; i < 4; ++i) ms[i] = c.stage_ms[i];
     });
 }
 

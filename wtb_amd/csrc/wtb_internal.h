@@ -261,6 +261,32 @@ struct Ctx {
     SolverWs lws;
     KernelClock kclock;
     double temperature = -1.0;  // wtb_set_temperature: < 0 off (kelvin otherwise)
+    // device time (ms) of the last wtb_hk_eig, wtb_bse_build, wtb_optics and
+    // wtb_oscillator, from HIP events on the stream around their kernels
+    double stage_ms[4] = {0, 0, 0, 0};
+};
+
+// Brackets the kernels of one entry point with HIP events on the context
+// stream; stop() waits for the second event and stores the elapsed time.
+struct StageTimer {
+    Ctx &c;
+    int slot;
+    hipEvent_t a = nullptr, b = nullptr;
+    StageTimer(Ctx &cx, int s) : c(cx), slot(s) {
+        if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) return;
+        (void)hipEventRecord(a, c.stream);
+    }
+    void stop() {
+        if (!a || !b) return;
+        (void)hipEventRecord(b, c.stream);
+        if (hipEventSynchronize(b) != hipSuccess) return;
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, a, b) == hipSuccess) c.stage_ms[slot] = ms;
+    }
+    ~StageTimer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
 };
 
 // k.R = (kx Rx + ky Ry) + kz Rz, evaluated without FMA contraction in the

@@ -31,7 +31,7 @@ EXPORTS = ["wtb_create", "wtb_destroy", "wtb_last_error", "wtb_set_tb_model", "w
            "wtb_hk_eig", "wtb_get_bands", "wtb_bse_build", "wtb_bse_solve", "wtb_optics",
            "wtb_oscillator", "wtb_bse_get_block", "wtb_bse_matvec", "wtb_bse_diag",
            "wtb_slepc_solver_compat", "wtb_grad_energies", "wtb_grad_spectrum", "wtb_set_temperature",
-           "wtb_hk_eig_q", "wtb_dos", "wtb_optics_bz"]
+           "wtb_hk_eig_q", "wtb_dos", "wtb_optics_bz", "wtb_stage_times"]
 
 
 class WTBError(RuntimeError):
@@ -99,6 +99,7 @@ def load():
         "wtb_grad_energies": [P, P, C.POINTER(GradScalars), P, P],
         "wtb_grad_spectrum": [P, I, D, D, I, P, P, P, P, C.POINTER(GradScalars), P, P],
         "wtb_set_temperature": [P, D],
+        "wtb_stage_times": [P, P],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -275,6 +276,12 @@ class Context:
         Y = np.zeros_like(X, order="F")
         self._check(self.lib.wtb_bse_matvec(self.h, X.shape[1], _ptr(X), _ptr(Y), int(precision)))
         return Y
+
+    def stage_times(self):
+        """device ms of the last hk_eig, bse_build, optics and oscillator calls"""
+        ms = np.zeros(4)
+        self._check(self.lib.wtb_stage_times(self.h, _ptr(ms)))
+        return dict(zip(("hk_eig", "bse_build", "optics", "oscillator"), ms.tolist()))
 
     def diag(self):
         d = np.zeros(self.N)
