@@ -213,6 +213,19 @@ int wtb_bse_get_block(wtb_ctx *ctx, long i0, long ni, long j0, long nj, double *
 int wtb_bse_matvec(wtb_ctx *ctx, int ncol, const double *X, double *Y, int precision);
 int wtb_bse_diag(wtb_ctx *ctx, double *diag);
 
+/* Y = H X like wtb_bse_matvec, without any device allocation: X and Y are
+ * DEVICE arrays (complex(8) (N, ncol) column-major, checked to be device
+ * memory), precision WTB_MIXED, WTB_FP32 or WTB_FP64 (WTB_INT8X is refused).
+ * The split-FP16 product of WTB_MIXED takes its FP16 planes and column
+ * scales from a static device array of 32 MiB + 4 KiB that each GPU holds
+ * until the process ends; up to 128 columns per launch, fewer above
+ * N = 32768, and N above 4194304 is refused.  Every call writes the part of
+ * the array it reads; calls that overlap on one GPU share the array and
+ * race.  No counterpart in the reference: for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc). */
+int wtb_bse_matvec_noalloc(wtb_ctx *ctx, int ncol, const double *X, double *Y, int precision);
+
 /* Device time in ms of the kernels of the last wtb_hk_eig, wtb_bse_build,
  * wtb_optics and wtb_oscillator calls (ms[0..3]), measured with HIP events on
  * the context stream.  No counterpart in the reference, whose MSG timers

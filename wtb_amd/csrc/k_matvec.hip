@@ -269,4 +269,29 @@ void matvec(Ctx &c, int precision, int ncol, const cd *X, cd *Y) {
     }
 }
 
+// matvec's products without device allocation: the FP64 and FP32 kernels use
+// none, the split-FP16 product takes its planes from the scratch array
+void matvec_noalloc(Ctx &c, int precision, int ncol, const cd *X, cd *Y) {
+    Bse &s = c.bse;
+    if (!s.valid) throw Error{WTB_ERR_STATE, "no BSE matrix built"};
+    if (ncol <= 0) return;
+    const double *x = (const double *)X;
+    double *y = (double *)Y;
+    if (precision == WTB_INT8X) throw Error{WTB_ERR_UNSUPPORTED, "the int8-digit product needs a workspace"};
+    if (s.precision == WTB_FP64) {
+        launch_hx<double, double>(c.stream, s.N, s.ld, ncol, (const double *)s.h64.p, x, y, c.kclock,
+                                  KernelClock::F64_ON_F64);
+        return;
+    }
+    if (precision == WTB_FP64)
+        launch_hx<float, double>(c.stream, s.N, s.ld, ncol, (const float *)s.h32.p, x, y, c.kclock,
+                                 KernelClock::F64_ON_F32);
+    else if (precision == WTB_MIXED)
+        matvec_f16split_noalloc(c, ncol, X, Y);
+    else
+        launch_hx<float, float>(c.stream, s.N, s.ld, ncol, (const float *)s.h32.p, x, y, c.kclock,
+                                KernelClock::F32_ON_F32);
+    add_diag(c, ncol, x, y);
+}
+
 }  // namespace wtb

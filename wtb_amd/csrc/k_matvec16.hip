@@ -22,6 +22,7 @@
 #include "wtb_internal.h"
 
 #include <cmath>
+#include <mutex>
 
 namespace wtb {
 namespace {
@@ -313,6 +314,19 @@ void f16split_blocks(Ctx &c, int ncol, const cd *X, cd *Y, int blk, _Float16 *pl
     }
 }
 
+// Scratch of the allocation-free entry point wtb_bse_matvec_noalloc: the FP16
+// split planes of X (4 ld bytes-pairs per column) in the first 32 MiB, the
+// column scales (2 floats per column) after them.  Every call writes the
+// planes and scales it reads, so nothing in it needs clearing; calls that
+// overlap on one GPU share it and race.
+constexpr size_t kScratchPlaneBytes = size_t(32) << 20;
+constexpr size_t kScratchBytes = kScratchPlaneBytes + 4096;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {nullptr};
+
 }  // namespace
 
 void matvec_f16split(Ctx &c, int ncol, const cd *X, cd *Y) {
@@ -320,6 +334,37 @@ void matvec_f16split(Ctx &c, int ncol, const cd *X, cd *Y) {
     c.ws_split.alloc((size_t)4 * blk * c.bse.ld);
     c.ws_colscale.alloc(2 * (size_t)blk);
     f16split_blocks(c, ncol, X, Y, 128, c.ws_split.p, c.ws_colscale.p);
+}
+
+// Device address of this GPU's copy of the scratch array, looked up once per
+// device with that device current (the host-side name is only a stand-in).
+void scratch_setup(Ctx &c) {
+    if (c.device < 0 || c.device >= kMaxDevices) throw Error{WTB_ERR_ARG, "device ordinal out of range"};
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[c.device]) {
+        void *ptr = nullptr;
+        hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !ptr)
+            throw Error{WTB_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e)};
+        g_scratch_ptr[c.device] = ptr;
+    }
+    c.scratch = g_scratch_ptr[c.device];
+}
+
+// The split-FP16 product through the scratch array: no device allocation.
+// Up to 128 columns per launch, fewer when 128 columns' planes exceed 32 MiB
+// (ld > 32768); ld > 4 Mi (one column's planes above 32 MiB) is refused.
+void matvec_f16split_noalloc(Ctx &c, int ncol, const cd *X, cd *Y) {
+    if (!c.scratch) throw Error{WTB_ERR_STATE, "allocation-free scratch not set up on this device"};
+    const size_t per_col = 4 * (size_t)c.bse.ld * sizeof(_Float16);
+    const size_t fit = kScratchPlaneBytes / per_col;
+    if (fit < 1)
+        throw Error{WTB_ERR_UNSUPPORTED, "N = " + std::to_string(c.bse.N) +
+                                             " exceeds the allocation-free scratch (32 MiB of FP16 planes: ld <= 4194304)"};
+    const int blk = (int)std::min<size_t>(128, fit);
+    unsigned char *base = static_cast<unsigned char *>(c.scratch);
+    f16split_blocks(c, ncol, X, Y, blk, reinterpret_cast<_Float16 *>(base),
+                    reinterpret_cast<float *>(base + kScratchPlaneBytes));
 }
 
 }  // namespace wtb

@@ -98,6 +98,7 @@ int wtb_create(wtb_ctx **out, int device) {
         WTB_HIP_CHECK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
         WTB_BLAS_CHECK(rocblas_create_handle(&c.blas));
         WTB_BLAS_CHECK(rocblas_set_stream(c.blas, c.stream));
+        scratch_setup(c);
     });
     if (st != WTB_OK) {
         delete ctx;
@@ -386,6 +387,28 @@ int wtb_bse_matvec(wtb_ctx *ctx, int ncol, const double *X, double *Y, int preci
         WTB_HIP_CHECK(hipMemcpyAsync(x.p, X, sizeof(cd) * N * ncol, hipMemcpyHostToDevice, c.stream));
         matvec(c, precision, ncol, x.p, y.p);
         WTB_HIP_CHECK(hipMemcpyAsync(Y, y.p, sizeof(cd) * N * ncol, hipMemcpyDeviceToHost, c.stream));
+        WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+    });
+}
+
+int wtb_bse_matvec_noalloc(wtb_ctx *ctx, int ncol, const double *X, double *Y, int precision) {
+    return guarded(ctx, [&](Ctx &c) {
+        if (!c.bse.valid) throw Error{WTB_ERR_STATE, "no BSE matrix"};
+        if (ncol <= 0 || !X || !Y) throw Error{WTB_ERR_ARG, "invalid matvec arguments"};
+        if (precision != WTB_FP64 && precision != WTB_MIXED && precision != WTB_FP32)
+            throw Error{WTB_ERR_ARG, "precision must be WTB_FP64, WTB_MIXED or WTB_FP32"};
+        // X and Y must be device memory: a host pointer would fault the kernels
+        for (const void *p : {static_cast<const void *>(X), static_cast<const void *>(Y)}) {
+            hipPointerAttribute_t at;
+            hipError_t e = hipPointerGetAttributes(&at, p);
+            if (e != hipSuccess) {
+                (void)hipGetLastError();
+                throw Error{WTB_ERR_ARG, "X and Y must be device pointers"};
+            }
+            if (at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged)
+                throw Error{WTB_ERR_ARG, "X and Y must be device pointers"};
+        }
+        matvec_noalloc(c, precision, ncol, reinterpret_cast<const cd *>(X), reinterpret_cast<cd *>(Y));
         WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
     });
 }

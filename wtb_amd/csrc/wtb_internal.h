@@ -264,6 +264,9 @@ struct Ctx {
     // device time (ms) of the last wtb_hk_eig, wtb_bse_build, wtb_optics and
     // wtb_oscillator, from HIP events on the stream around their kernels
     double stage_ms[4] = {0, 0, 0, 0};
+    // device address of this GPU's scratch array of the allocation-free
+    // matvec (set by scratch_setup in wtb_create)
+    void *scratch = nullptr;
 };
 
 // Brackets the kernels of one entry point with HIP events on the context
@@ -348,6 +351,16 @@ void matvec(Ctx &c, int precision, int ncol, const cd *X, cd *Y);
 
 // Y = H_offdiag X on the FP32 store with split-FP16 MFMA (mixed precision).
 void matvec_f16split(Ctx &c, int ncol, const cd *X, cd *Y);
+
+// The same product with its split planes and column scales in the file-scope
+// scratch array of k_matvec16.hip (no device allocation); scratch_setup looks
+// up that array's device address for the context's device.
+void matvec_f16split_noalloc(Ctx &c, int ncol, const cd *X, cd *Y);
+void scratch_setup(Ctx &c);
+
+// Y = H X without device allocation (split-FP16 products through the scratch
+// array for WTB_MIXED); X, Y device arrays
+void matvec_noalloc(Ctx &c, int precision, int ncol, const cd *X, cd *Y);
 
 // Y = H_offdiag X on the FP32 store with exact int8 digit products (the
 // refinement product of the mixed-precision solver).

@@ -31,7 +31,7 @@ EXPORTS = ["wtb_create", "wtb_destroy", "wtb_last_error", "wtb_set_tb_model", "w
            "wtb_hk_eig", "wtb_get_bands", "wtb_bse_build", "wtb_bse_solve", "wtb_optics",
            "wtb_oscillator", "wtb_bse_get_block", "wtb_bse_matvec", "wtb_bse_diag",
            "wtb_slepc_solver_compat", "wtb_grad_energies", "wtb_grad_spectrum", "wtb_set_temperature",
-           "wtb_hk_eig_q", "wtb_dos", "wtb_optics_bz", "wtb_stage_times"]
+           "wtb_hk_eig_q", "wtb_dos", "wtb_optics_bz", "wtb_stage_times", "wtb_bse_matvec_noalloc"]
 
 
 class WTBError(RuntimeError):
@@ -100,6 +100,7 @@ def load():
         "wtb_grad_spectrum": [P, I, D, D, I, P, P, P, P, C.POINTER(GradScalars), P, P],
         "wtb_set_temperature": [P, D],
         "wtb_stage_times": [P, P],
+        "wtb_bse_matvec_noalloc": [P, I, P, P, I],
     }
     for name, args in sig.items():
         f = getattr(lib, name)
@@ -282,6 +283,12 @@ class Context:
         ms = np.zeros(4)
         self._check(self.lib.wtb_stage_times(self.h, _ptr(ms)))
         return dict(zip(("hk_eig", "bse_build", "optics", "oscillator"), ms.tolist()))
+
+    def matvec_noalloc(self, x_dev, y_dev, ncol, precision=MIXED):
+        """Y = H X on device arrays (addresses of complex128 (N, ncol)
+        column-major buffers) without any device allocation by the library"""
+        self._check(self.lib.wtb_bse_matvec_noalloc(self.h, int(ncol), C.c_void_p(x_dev), C.c_void_p(y_dev),
+                                                     int(precision)))
 
     def diag(self):
         d = np.zeros(self.N)
