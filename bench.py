@@ -210,13 +210,13 @@ def main():
               "(the multi-GPU sharding is described in DESIGN.md, not implemented)", file=sys.stderr)
         sys.exit(2)
 
+    # torch only for the optional copy calibration: the library's calls return
+    # after their own stream has drained, and torch brings a second HIP runtime
     global _torch
-    try:
+    if args.calibrate_copy:
         import torch
         if torch.cuda.is_available():
             _torch = torch
-    except Exception:
-        _torch = None
 
     from wtb_amd import hip, tbmodel as T
     t = time.perf_counter()

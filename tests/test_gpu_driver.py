@@ -236,7 +236,8 @@ def test_exciton_bands_driver_run():
         fh.write("2\n3\n" + "".join(f"{a} {b} {c}\n" for a, b, c in ks))
     cards = ["BSE_BND= T", f'KPATH_BSE= "{kfile}"', "SYSDIM= \"2D\"", "NGX= 6", "NGY= 6", "NGZ= 1",
              "NBANDSC= 1", "NBANDSV= 1", "COULOMB_POT= V2DRK", "R_0= 10.0", "EDIEL= 1.0", "EDIEL_T= 1.0",
-             "EDIEL_B= 1.0", "EDIEL_Z= 1.0", "LC= 1.0", "KTOL= 0.001", "PRECISION= FP64"]
+             "EDIEL_B= 1.0", "EDIEL_Z= 1.0", "LC= 1.0", "KTOL= 0.001", "PRECISION= FP64", "BSE_WF= T",
+             "EXC_WF_I= 1", "EXC_WF_F= 3"]
     d, m = run_driver(cards)
     assert not os.path.exists(os.path.join(d, "bse_opt_diel.dat"))      # BSE= is off
     lab = load(os.path.join(d, "KLABELS-BSE.dat"))[:, 0]
@@ -264,6 +265,27 @@ def test_exciton_bands_driver_run():
         rows = out[iq::nq]
         assert np.abs(rows[:, 0] - 0.5 * iq).max() < 1e-6
         assert np.abs(rows[:, 1] - w).max() < 1e-9, iq
+        # BSE_WF=T (excwfi2): exc_wf_<n>_<iq>.dat for n = 1..3, headed by the
+        # Cartesian Q; |psi|^2 summed over each level inside 1..3 against the
+        # oracle's projector on that level
+        wv, Xq = np.linalg.eigh(H)
+        levels = [lv for lv in np.split(np.arange(N), np.where(np.diff(wv) > 1e-6)[0] + 1) if lv[-1] < 3]
+        amp = {}
+        for n in range(1, 4):
+            f = os.path.join(d, f"exc_wf_{n}_{iq + 1}.dat")
+            with open(f) as fh:
+                head = fh.read().splitlines()
+            assert np.allclose([float(x) for x in head[0].split()[-3:]], np.float32(q), atol=1e-6)
+            assert abs(float(head[2].split()[-1]) - wv[n - 1]) < 1e-8
+            a = load(f)
+            assert a.shape == (N, 8)
+            assert np.abs(a[:, 1:4] - k).max() <= 5.1e-5
+            assert np.all(a[:, 4] == 2) and np.all(a[:, 5] == 1)
+            amp[n] = a[:, 6] + 1j * a[:, 7]
+        for lv in levels:
+            got = sum(np.abs(amp[i + 1]) ** 2 for i in lv)
+            want = (np.abs(Xq[:, lv]) ** 2).sum(1)
+            assert np.abs(got - want).max() < 1e-3 * want.max(), (iq, lv)
         H0 = O.bse_matrix_q(k, ref["eig"], ref["vec"], refq["eig"], refq["vec"], c, mesh, m.rlat, 1, 1, q,
                             exchange=False)
         moved = max(moved, np.abs(np.linalg.eigvalsh(H0) - w).max())
@@ -390,6 +412,28 @@ def test_bz_activity_driver_run():
         assert np.abs(got[:, 3] - ref[:, a]).max() <= 1e-9 * max(1.0, np.abs(ref[:, a]).max()), tag
     dic = load(os.path.join(d, "dichroism.dat"))
     assert np.abs(dic[:, 3] - (ref[:, 3] - ref[:, 4]) / (ref[:, 3] + ref[:, 4])).max() < 1e-9
+
+
+def test_log_file(honeycomb_run):
+    """log.dat (CALC_DATA, here the run folder): the cards in param_out's
+    layout (module_input.F90:422-556), the Begin line, the finished stages
+    in main.F90's order, the End line."""
+    d, m = honeycomb_run
+    say("case driver log.dat")
+    with open(os.path.join(d, "log.dat")) as fh:
+        text = fh.read()
+    lines = [ln.strip() for ln in text.splitlines()]
+    for card in ("NTHREADS= 1", "SYSDIM= 2D", "BSE= T", "BSET= F", "SPEC= T", "PP_ONLY= F", "BSE_WF= T",
+                 "DTDIAG= T", "CPOL= T", "NGX= 12", "NGY= 12", "NGZ= 1", "NBANDSC= 1", "NBANDSV= 1",
+                 "ENSPECI=   3.0000", "ENSPECF=   7.0000", "NESPEC= 400", "COULOMB_POT= V2DRK",
+                 "CSHIFT=   0.0500", "R_0=  10.0000", "EXC_WF_F= 4"):
+        assert card in lines, card
+    order = ["Begin", "BSE and Single particle dielectric calculation finished", "BSE dielectric properties calculated",
+             "BSE absorption spectrum with light polarization calculated",
+             "Single particle dielectric properties calculated",
+             "Single particle absorption spectrum with light polarization calculated", "End"]
+    pos = [text.index(x) for x in order]
+    assert pos == sorted(pos)
 
 
 def test_exciton_wavefunction_files(honeycomb_run):

@@ -17,6 +17,11 @@
 ! module_bse.F90:1206-2014): log_bse_kpath.dat, bands_bse.dat and
 ! KLABELS-BSE.dat.
 !
+! log.dat (CALC_DATA=, main.F90:61-76) echoes the cards and the finished
+! stages; PP_ONLY=T (main.F90:79) skips every calculation and recomputes the
+! spectra from the files already in OUTPUT.  BSE_WF=T writes exc_wf_<n>.dat
+! and, with BSE_BND=T, exc_wf_<n>_<iq>.dat per Q point.
+!
 ! Cards of this driver beyond the reference's: NEV= (eigenpairs, default 100
 ! as the reference's SLEPc call), PRECISION= (FP64 | MIXED | FP32), KERNEL=
 ! (REF | TEXTBOOK overlap convention), F32_COMPAT= (T: float32 phases as the
@@ -56,6 +61,15 @@ module wtb_cards
     double precision :: numdos = 6001.0d0, sme = 0.08d0
     character(len=70) :: orbw = "non declared file"
     character(len=1) :: dft = "V"
+    ! PP_ONLY=T (main.F90:79): no calculation, only the spectra from the files
+    ! already in OUTPUT.  CALC_DATA= is the folder of log.dat; NTHREADS= and the
+    ! cards of workflows outside this driver (BANDS=, BERRY=, BERRY_BZ=,
+    ! KPATH_FILE=, PCE= and its parameters) are only echoed to log.dat.
+    logical :: pponly = .false., bandscalc = .false., berryk = .false., berrybz = .false., pce = .false.
+    integer :: nthreads = 1
+    character(len=70) :: calcparms = "./", kpaths = "non declared file"
+    character(len=6) :: ses = "AM15G"
+    double precision :: ctemp = 298.15d0, tmax = 5.0d-6, eg = 0.0d0, egd = 0.0d0, egs = 0.0d0, ebgs = 0.0d0
 contains
     ! "CARD= value" lines from standard input, defaults of module_input.F90:53-120
     subroutine read_cards()
@@ -120,6 +134,21 @@ contains
             case ("KERNEL=");       kernel = trim(adjustl(b))
             case ("F32_COMPAT=");   read(b, *) compat
             case ("DEVICE=");       read(b, *) device
+            case ("PP_ONLY=");      read(b, *) pponly
+            case ("CALC_DATA=");    calcparms = b
+            case ("NTHREADS=");     read(b, *) nthreads
+            case ("BANDS=");        read(b, *) bandscalc
+            case ("BERRY=");        read(b, *) berryk
+            case ("BERRY_BZ=");     read(b, *) berrybz
+            case ("KPATH_FILE=");   kpaths = b
+            case ("PCE=");          read(b, *) pce
+            case ("SES=");          ses = trim(adjustl(b))
+            case ("CTEMP=");        read(b, *) ctemp
+            case ("THMAX=");        read(b, *) tmax
+            case ("EG=");           read(b, *) eg
+            case ("EGD=");          read(b, *) egd
+            case ("EGS=");          read(b, *) egs
+            case ("EBGS=");         read(b, *) ebgs
             case default
                 continue
             end select
@@ -542,11 +571,12 @@ program wtb_bse
     complex(c_double_complex), allocatable, target :: hev(:, :)
     character(len=2), parameter :: tens(6) = ["xx", "yy", "zz", "xy", "xz", "yz"]
     character(len=2), parameter :: pols(5) = ["x ", "y ", "z ", "sp", "sm"]
+    integer :: ulog
 
     call read_cards()
     if (bse .and. bset) stop "ERROR: Choose BSE=T or BSET=T not both"
     if (bsekpath .and. bsetbnd) stop "ERROR: Choose BSE_BND=T or BSET_BND=T not both"
-    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd .or. doscalc .or. spdiel .or. sppolbz)) &
+    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd .or. doscalc .or. spdiel .or. sppolbz .or. pponly)) &
         stop "wtb_bse: only the DOS=T, DIEL=T, BSE=T, BSET=T, BSE_BND=T and BSET_BND=T paths are provided"
     ! gap correction of the temperature paths (bsesolvertemp, module_bse.F90:2857-2875;
     ! bsebndstemp :1740-1756), applied with the scissor to the conduction bands
@@ -575,9 +605,15 @@ program wtb_bse
 
     flags = WTB_FLAG_GAUGE_LAPACK
     if (compat) flags = flags + WTB_FLAG_COMPAT_F32
-    if (doscalc) call dos_file()
-    if (sppolbz) call bz_activity_files()
-    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd .or. spdiel)) stop
+    call log_open()
+    if (pponly) go to 131
+    if (bandscalc .or. berryk .or. berrybz .or. pce) &
+        write(ulog, *) "BANDS=, BERRY=, BERRY_BZ= and PCE= are outside this driver: not run"
+    if (doscalc) then
+        call dos_file()
+        call log_line("DOS finished")
+    end if
+    if (.not. (bse .or. bset .or. bsekpath .or. bsetbnd .or. spdiel .or. sppolbz)) go to 131
     select case (trim(precision))
     case ("FP64");  prec = WTB_FP64
     case ("MIXED"); prec = WTB_MIXED
@@ -601,8 +637,7 @@ program wtb_bse
         stop
     end select
     coul = wtb_coulomb(kind, r0, ediel, ez, w, lc, ktol)
-    if (bsekpath .or. bsetbnd) call exciton_bands()
-    if (.not. (bse .or. bset .or. spdiel)) stop
+    if (.not. (bse .or. bset .or. spdiel)) go to 121
 
     ! ---- GPU path: eigsys loop, optsp, matrizelbse, slepc_solver, dielbse*
     call wtb_check(ctx, wtb_create(ctx, int(device, c_int)), "wtb_create")
@@ -755,10 +790,27 @@ program wtb_bse
     close(302)
     if (bsewf) call exciton_wf_files()
     end if
+    if (bse) call log_line("BSE and Single particle dielectric calculation finished")
+    if (bset) call log_line("BSE and Single particle dielectric calculation, with temperature, finished")
+
+121 continue
+    if (bsekpath) then
+        call exciton_bands()
+        call log_line("BSE exciton band structure finished")
+    else if (bsetbnd) then
+        call exciton_bands()
+        call log_line("BSE exciton band structure, with temperature, finished")
+    end if
+    if (spdiel) call log_line("Single particle dielectric calculation finished")
+    if (sppolbz) then
+        call bz_activity_files()
+        call log_line("Single particle optical activity in BZ finished")
+    end if
 
     ! ---- spectra from the written (4-decimal) file, optical constants, lifetimes
     ! (main.F90:183-223: with BSE the single-particle chains run too, and the
     ! lifetimes only without a temperature)
+131 continue
     if (spec) then
         spinf = 1.0
         if (systype == "NP") spinf = 2.0
@@ -767,24 +819,147 @@ program wtb_bse
         call diel_set("bse_opt_diel.dat", "bse_diel_", tens, spinf)
         call optprop_set("bse_diel_", tens, "bse_", "")
         if (.not. bset) call lifetime_file()
+        call log_line("BSE dielectric properties calculated")
         if (cpol) then
             call diel_set("bse_opt_diel-pol.dat", "bse_diel-pol_", pols, spinf)
             call optprop_set("bse_diel-pol_", pols, "bse_", "-pol")
             if (.not. bset) call lifetime_pol_file()
+            call log_line("BSE absorption spectrum with light polarization calculated")
         end if
     end if
-    if (spec) then
+    if (spec .and. (spdiel .or. bse .or. bset)) then
         call diel_set("sp_optics.dat", "sp_diel_", tens, spinf)
         call optprop_set("sp_diel_", tens, "sp_", "")
+        call log_line("Single particle dielectric properties calculated")
     end if
     if (spec .and. (cpol .or. bse .or. bset)) then
         call diel_set("sp_optics-pol.dat", "sp_diel-pol_", pols, spinf)
         call optprop_set("sp_diel-pol_", pols, "sp_", "-pol")
+        call log_line("Single particle absorption spectrum with light polarization calculated")
     end if
-    if (bse .or. bset) write(*, '(A,I0,A,I0,A,F12.6,A,ES10.3)') 'wtb_bse: N=', dimbse, ' nev=', nevs, ' E1=', &
-        ws(1), ' eV, max rel residual ', rep%max_rel_residual
+    if ((bse .or. bset) .and. .not. pponly) write(*, '(A,I0,A,I0,A,F12.6,A,ES10.3)') 'wtb_bse: N=', dimbse, &
+        ' nev=', nevs, ' E1=', ws(1), ' eV, max rel residual ', rep%max_rel_residual
+    call log_close()
 
 contains
+
+    ! log.dat in CALC_DATA (main.F90:61-76, 237-245): the cards as param_out
+    ! writes them (module_input.F90:422-556), a Begin line, one line per
+    ! finished stage and an End line.
+    subroutine log_open()
+        integer :: erro, values(8)
+        open(newunit=ulog, file=trim(calcparms)//"/log.dat", status='unknown', iostat=erro)
+        if (erro /= 0) stop "Error opening log output file "
+        write(ulog, "(A10,I0)") "NTHREADS= ", nthreads
+        write(ulog, "(A8,A2)") "SYSDIM= ", sysdim
+        write(ulog, "(A5,A1)") "DFT= ", dft
+        write(ulog, *)
+        write(ulog, *) "INPUT/OUTPUT FILES"
+        write(ulog, *)
+        write(ulog, "(A8,A70)") "OUTPUT= ", outputfolder
+        write(ulog, "(A11,A70)") "CALC_DATA= ", calcparms
+        write(ulog, "(A13,A70)") "PARAMS_FILE= ", params
+        write(ulog, "(A12,A70)") "KPATH_FILE= ", kpaths
+        write(ulog, "(A11,A70)") "KPATH_BSE= ", kpathsbse
+        write(ulog, "(A7,A70)") "ORB_W= ", orbw
+        write(ulog, *)
+        write(ulog, *) "CONTROL"
+        write(ulog, *)
+        write(ulog, "(A7,L1)") "BANDS= ", bandscalc
+        write(ulog, "(A5,L1)") "DOS= ", doscalc
+        write(ulog, "(A5,L1)") "BSE= ", bse
+        write(ulog, "(A6,L1)") "BSET= ", bset
+        write(ulog, "(A9,L1)") "BSE_BND= ", bsekpath
+        write(ulog, "(A10,L1)") "BSET_BND= ", bsetbnd
+        write(ulog, "(A6,L1)") "DIEL= ", spdiel
+        write(ulog, "(A8,L1)") "OPT_BZ= ", sppolbz
+        write(ulog, "(A6,L1)") "SPEC= ", spec
+        write(ulog, "(A10,L1)") "BERRY_BZ= ", berrybz
+        write(ulog, "(A7,L1)") "BERRY= ", berryk
+        write(ulog, "(A9,L1)") "PP_ONLY= ", pponly
+        write(ulog, "(A8,L1)") "BSE_WF= ", bsewf
+        write(ulog, "(A8,L1)") "TMCOEF= ", tmcoef
+        write(ulog, "(A8,L1)") "DTDIAG= ", dtfull
+        write(ulog, "(A6,L1)") "CPOL= ", cpol
+        write(ulog, "(A5,L1)") "PCE= ", pce
+        write(ulog, *)
+        write(ulog, *) "K-MESH"
+        write(ulog, *)
+        write(ulog, "(A5,I0)") "NGX= ", ngrid(1)
+        write(ulog, "(A5,I0)") "NGY= ", ngrid(2)
+        write(ulog, "(A5,I0)") "NGZ= ", ngrid(3)
+        write(ulog, "(A9,F8.4)") "SHIFT_1= ", mshift(1)
+        write(ulog, "(A9,F8.4)") "SHIFT_2= ", mshift(2)
+        write(ulog, "(A9,F8.4)") "SHIFT_3= ", mshift(3)
+        write(ulog, *)
+        write(ulog, *) "DOS"
+        write(ulog, *)
+        write(ulog, "(A7,I0)") "NEDOS= ", int(numdos)
+        write(ulog, "(A7,F8.4)") "SIGMA= ", sme
+        write(ulog, *)
+        write(ulog, *) "BSE/OPTICAL PROPERTIES"
+        write(ulog, *)
+        write(ulog, "(A9,I0)") "NBANDSC= ", nc
+        write(ulog, "(A9,I0)") "NBANDSV= ", nv
+        write(ulog, "(A9,F8.4)") "ENSPECI= ", ebse0
+        write(ulog, "(A9,F8.4)") "ENSPECF= ", ebsef
+        write(ulog, "(A8,I0)") "NESPEC= ", int(numbse)
+        write(ulog, "(A6,F8.4)") "KTOL= ", ktol
+        write(ulog, "(A10,I0)") "EXC_WF_I= ", excwf0
+        write(ulog, "(A10,I0)") "EXC_WF_F= ", excwff
+        write(ulog, "(A13,A5)") "COULOMB_POT= ", coultype
+        write(ulog, "(A8,F8.4)") "CSHIFT= ", cshift
+        write(ulog, "(A6,L1)") "RNMD= ", renorm
+        write(ulog, *)
+        write(ulog, *) "PARAMETERS FOR COULOMB POTENTIALS"
+        write(ulog, *)
+        write(ulog, "(A9,F8.4)") "EDIEL_Z= ", ez
+        write(ulog, "(A8,F8.4)") "W_COUL= ", w
+        write(ulog, "(A4,F8.4)") "LC= ", lc
+        write(ulog, "(A5,F8.4)") "R_0= ", r0
+        write(ulog, "(A9,F8.4)") "EDIEL_T= ", ediel(1)
+        write(ulog, "(A9,F8.4)") "EDIEL_B= ", ediel(3)
+        write(ulog, "(A7,F8.4)") "EDIEL= ", ediel(2)
+        write(ulog, *)
+        write(ulog, *) "PARAMETERS FOR BSE with Temperature Effects"
+        write(ulog, *)
+        write(ulog, "(A4,A2)") "TA= ", ta
+        write(ulog, "(A4,F8.4)") "ST= ", st
+        write(ulog, "(A7,F8.4)") "PHAVG= ", phavg
+        write(ulog, "(A6,F8.4)") "TEMP= ", temp
+        write(ulog, *)
+        write(ulog, *) "PARAMETERS FOR PCE"
+        write(ulog, *)
+        write(ulog, "(A5,A6)") "SES= ", ses
+        write(ulog, "(A7,F8.4)") "CTEMP= ", ctemp
+        write(ulog, "(A7,E15.4)") "THMAX= ", tmax
+        write(ulog, "(A4,F8.4)") "EG= ", eg
+        write(ulog, "(A5,F8.4)") "EGD= ", egd
+        write(ulog, "(A5,F8.4)") "EGS= ", egs
+        write(ulog, "(A6,F8.4)") "EBGS= ", ebgs
+        write(ulog, *)
+        write(ulog, *)
+        call date_and_time(values=values)
+        write(ulog, *)
+        write(ulog, *) 'Begin', '  ', 'day', values(3), '', values(5), 'hours', values(6), 'min', values(7), 'seg'
+        write(ulog, *)
+        flush(ulog)
+    end subroutine log_open
+
+    subroutine log_line(text)
+        character(len=*), intent(in) :: text
+        write(ulog, *) text
+        flush(ulog)
+    end subroutine log_line
+
+    subroutine log_close()
+        integer :: values(8)
+        call date_and_time(values=values)
+        write(ulog, *)
+        write(ulog, *) 'End', '   ', 'day', values(3), '', values(5), 'hours', values(6), 'min', values(7), 'seg'
+        write(ulog, *)
+        close(ulog)
+    end subroutine log_close
 
     ! BSE_BND=T / BSET_BND=T (bsebnds, bsebndstemp, module_bse.F90:1206-2014):
     ! for every Q of the path the bands on the k+Q mesh, the kernel of
@@ -798,6 +973,7 @@ contains
         integer :: u, ul, ub, erro, nks, npts, nq, iq, i, dimb, nevb
         double precision, allocatable :: ks(:, :), qv(:, :), exk(:, :), wq(:)
         double precision, allocatable, target :: kq(:, :), qc(:)
+        complex(c_double_complex), allocatable, target :: hq(:, :)
         open(newunit=u, file=trim(kpathsbse), status='old', iostat=erro)
         if (erro /= 0) stop "Error opening bse-kpath input file"
         read(u, *) nks
@@ -829,7 +1005,8 @@ contains
         call wtb_check(bctx, wtb_set_tb_model(bctx, norb, nvec, rvec, ffactor, hop, ihop, rlat, efermi, scs + tcor, &
                        flags), "wtb_set_tb_model")
         call wtb_check(bctx, wtb_set_kpoints(bctx, nk, kpt, ngrid), "wtb_set_kpoints")
-        call wtb_check(bctx, wtb_hk_eig(bctx, nv, nc, c_null_ptr, c_null_ptr, c_null_ptr), "wtb_hk_eig")
+        call wtb_check(bctx, wtb_hk_eig(bctx, nv, nc, c_null_ptr, c_null_ptr, c_loc(nocpk)), "wtb_hk_eig")
+        if (bsewf) allocate(hq(dimb, nevb))
         write(ul, *) "quantum numbers for exciton basis set finished"
         if (bsetbnd) call wtb_check(bctx, wtb_set_temperature(bctx, temp), "wtb_set_temperature")
         do iq = 1, nq
@@ -838,8 +1015,14 @@ contains
             call wtb_check(bctx, wtb_hk_eig_q(bctx, c_loc(qc), c_loc(kq), nv, nc, c_null_ptr, c_null_ptr, &
                            c_null_ptr), "wtb_hk_eig_q")
             call wtb_check(bctx, wtb_bse_build(bctx, coul, prec, conv), "wtb_bse_build")
-            call wtb_check(bctx, wtb_bse_solve(bctx, nevb, 1.0d-8, 200, wq, c_null_ptr, c_null_ptr), &
-                           "wtb_bse_solve")
+            if (bsewf) then
+                call wtb_check(bctx, wtb_bse_solve(bctx, nevb, 1.0d-8, 200, wq, c_loc(hq), c_null_ptr), &
+                               "wtb_bse_solve")
+                call exciton_wf_q_files(iq, qv(iq, :), wq, hq, nevb)
+            else
+                call wtb_check(bctx, wtb_bse_solve(bctx, nevb, 1.0d-8, 200, wq, c_null_ptr, c_null_ptr), &
+                               "wtb_bse_solve")
+            end if
             exk(:, iq) = wq(1:nevb)
             write(ul, *) 'progress:', iq, '/', nq
             flush(ul)
@@ -888,6 +1071,44 @@ contains
             close(u)
         end do
     end subroutine exciton_wf_files
+
+    ! BSE_WF=T with BSE_BND=T / BSET_BND=T (excwfi2, module_bse.F90:412-447;
+    ! called at :1515-1519, :1957-1961): OUTPUT//exc_wf_<n>_<iq>.dat per
+    ! exciton n = EXC_WF_I .. EXC_WF_F of Q point iq, headed by the Cartesian
+    ! Q in default real; the rows as those of exc_wf_<n>.dat (k-mesh band
+    ! indices and nocp), the amplitudes those of the finite-Q eigenvector.
+    subroutine exciton_wf_q_files(iq, q, wq, hq, nevb)
+        integer, intent(in) :: iq, nevb
+        double precision, intent(in) :: q(4), wq(:)
+        complex(c_double_complex), intent(in) :: hq(:, :)
+        character(len=200) :: file1
+        integer :: n, u, erro, kk, cc, vv, j
+        do n = excwf0, min(excwff, nevb)
+            write(file1, '(a7,I0,a1,I0,a4)') 'exc_wf_', n, "_", iq, '.dat'
+            open(newunit=u, file=trim(outputfolder)//trim(file1), status='unknown', iostat=erro)
+            if (erro /= 0) stop "Error opening wf BSE output file"
+            write(u, *) "#", " ", "excitonic momentum", real(q(2)), real(q(3)), real(q(4))
+            write(u, *) "#", " ", "excitonic state", n
+            write(u, *) "#", " ", "exciton energy:", wq(n)
+            write(u, *) "#", " ", "number occupied states", nocpk(1), "(not valid for metallic systems)"
+            write(u, *) "#", " ", "Number of conduction states", "", nc
+            write(u, *) "#", " ", "Number of valence states", "", nv
+            write(u, *) "#", " ", "Number of kpoints", "", nk
+            write(u, *) "#", " ", "nocpk", " ", "kx", " ", "ky", " ", "kz", " ", "nc", " ", "nv", " ", "re_wf", " ", &
+                "imag_wf"
+            j = 0
+            do kk = 1, nk
+                do cc = 1, nc
+                    do vv = 1, nv
+                        j = j + 1
+                        write(u, "(I12,3F15.4,2I12,2E15.4)") nocpk(kk), kpt(kk, 1), kpt(kk, 2), kpt(kk, 3), &
+                            nocpk(kk) + cc, nocpk(kk) - nv + vv, real(hq(j, n)), aimag(hq(j, n))
+                    end do
+                end do
+            end do
+            close(u)
+        end do
+    end subroutine exciton_wf_q_files
 
     ! OPT_BZ=T (spoptpolbz, module_bse.F90:4532-4797): per k the activity
     ! sums of x, y, z, sp and sm in bz_act_*.dat and (sp - sm)/(sp + sm) in
