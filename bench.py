@@ -202,6 +202,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-mesh", type=int, default=60,
                     help="k-mesh of the complete CPU solve of the cpu_baseline sample (90: the full workload)")
+    ap.add_argument("--mixed-only", action="store_true",
+                    help="time the mixed path only and print its report (for profiler runs)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run a 1 GiB device copy first (reference kernel for rocprofv3 --pmc scaling)")
     args = ap.parse_args()
@@ -236,6 +238,11 @@ def main():
     # mixed at the 0.1 meV tolerance (the headline), warm, K steps
     t_mixed, E_mixed, reps, stages = timed(ctx, hip.MIXED, nev, MIXED_TOL, args.steps, args.warmup, "mixed")
     rep = reps[-1]
+    if args.mixed_only:
+        print(json.dumps({"mixed_s_per_solve": t_mixed, "stages_mixed_s": stages[-1], "report_mixed": rep}),
+              flush=True)
+        ctx.close()
+        return
     f_mixed, t_opt_ms, t_osc_ms = oscillators(ctx, nev)
     # the same with the FP64-arithmetic refinement product instead of int8 digits
     os.environ["WTB_REFINE"] = "fp64"

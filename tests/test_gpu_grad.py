@@ -75,6 +75,21 @@ def base():
     return p, w, g
 
 
+def test_grad_is_reproducible():
+    """The per-workgroup partials are summed in a fixed order (no atomics):
+    two calls on the same solve give the same gradient bit for bit."""
+    say("case grad reproducible")
+    p = Pipe()
+    p.energies()
+    w = np.random.default_rng(7).uniform(0.5, 1.5, NEV)
+    g1 = p.ctx.grad_energies(w)
+    g2 = p.ctx.grad_energies(w)
+    assert list(g1["coulomb"].values()) == list(g2["coulomb"].values())
+    assert g1["scissor"] == g2["scissor"] and g1["efermi"] == g2["efermi"]
+    assert np.array_equal(g1["hop"], g2["hop"])
+    p.ctx.close()
+
+
 def test_grad_coulomb_parameters_fd(base):
     p, w, g = base
     say("case grad coulomb")

@@ -295,11 +295,32 @@ grad_pair_kernel(int nk, int norb, int nst, int textbook, int accumulate, Coulom
         if (lane == 0) red[p][wv] = v;
     }
     __syncthreads();
+    // this workgroup's partial sums in its own slot (a later state chunk
+    // adds to it): the fixed-order sum over k' follows in fixed_sum_kernel,
+    // so the result does not depend on the order the workgroups run in
     if (tid < kNP) {
         double v = 0.0;
         for (int i = 0; i < kPT / 64; ++i) v += red[tid][i];
-        atomicAdd(dcoul + tid, v);
+        double *slot = dcoul + (size_t)kp * kNP + tid;
+        *slot = accumulate ? *slot + v : v;
     }
+}
+
+// out[q] = sum_i part[i nq + q], i = 0..nparts-1, one workgroup per q: each
+// thread sums a fixed stride of the partials in order, then a fixed tree.
+__global__ void __launch_bounds__(256)
+fixed_sum_kernel(int nparts, int nq, const double *__restrict__ part, double *__restrict__ out) {
+    __shared__ double red[256];
+    const int q = blockIdx.x, tid = threadIdx.x;
+    double v = 0.0;
+    for (int i = tid; i < nparts; i += 256) v += part[(size_t)i * nq + q];
+    red[tid] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) red[tid] += red[tid + h];
+        __syncthreads();
+    }
+    if (tid == 0) out[q] = red[0];
 }
 
 __device__ inline void atomic_max_nonneg(double *addr, double v) {
@@ -308,7 +329,8 @@ __device__ inline void atomic_max_nonneg(double *addr, double v) {
 }
 
 // TB eigenvector adjoint, one wavefront per k.  W: [(j n + o) nk + k] (Ubar of
-// selected band j).  acc: [0] dL/dscissor, [1] dL/dE_F, [2] max |Im u_m^H ubar_m|.
+// selected band j).  acc: [2] max |Im u_m^H ubar_m|; [3 + 2k] and [4 + 2k]
+// this k's parts of dL/dscissor and dL/dE_F (summed by fixed_sum_kernel).
 // ebx, ubx (optional): further dL/dE_j(k) [k nb + j] and Ubar contributions in
 // the layout of W (from the dipole adjoint).
 __global__ void __launch_bounds__(64)
@@ -400,9 +422,9 @@ tb_adjoint_kernel(int nk, int n, int nv, int nc, int nst, double scissor, double
         double ds = 0.0, gi = 0.0;
         for (int j = nv; j < nb; ++j) ds += lbar[j];
         for (int j = 0; j < nb; ++j) gi = fmax(gi, fabs(Cm[(nocp - nv + j) * nb + j].y));
-        atomicAdd(acc + 0, ds);
-        atomicAdd(acc + 1, -tr);
-        atomic_max_nonneg(acc + 2, gi);
+        acc[3 + 2 * (size_t)k] = ds;          // per-k partials, summed in a fixed order
+        acc[4 + 2 * (size_t)k] = -tr;
+        atomic_max_nonneg(acc + 2, gi);        // a maximum: independent of the order
     }
 }
 
@@ -490,11 +512,17 @@ void grad_core(Ctx &c, const cd *X, int nx, const double *w, const GradExtra *ex
     std::memset(out, 0, sizeof(*out));
     const int npq = nc * nc + nv * nv;
     DBuf<cd> G, Wb, Hbar;
-    DBuf<double> scal, gdev;
+    DBuf<double> scal, gdev, dpart, apart;
     DBuf<int> didx;
     DBuf<double> dw;
     scal.alloc(kNP + 3);
     WTB_HIP_CHECK(hipMemsetAsync(scal.p, 0, sizeof(double) * (kNP + 3), c.stream));
+    // per-workgroup partials of the Coulomb derivatives (one row per k') and
+    // of the scissor / E_F derivatives (two per k) plus the gauge maximum
+    dpart.alloc((size_t)nk * kNP);
+    apart.alloc(3 + 2 * (size_t)nk);
+    WTB_HIP_CHECK(hipMemsetAsync(dpart.p, 0, sizeof(double) * nk * kNP, c.stream));
+    WTB_HIP_CHECK(hipMemsetAsync(apart.p, 0, sizeof(double) * (3 + 2 * (size_t)nk), c.stream));
     Hbar.alloc((size_t)nk * n * n);
     const bool extra = ex && (ex->ebar || ex->ubar);
     if (idx.empty() && !extra) {
@@ -510,7 +538,7 @@ void grad_core(Ctx &c, const cd *X, int nx, const double *w, const GradExtra *ex
             G.alloc((size_t)npq * nk * nk);
             for (size_t s0 = 0; s0 < idx.size(); s0 += kMaxStates) {
                 const int ns = (int)std::min<size_t>(kMaxStates, idx.size() - s0);
-                dispatch_pair(c, cd_, X, ns, s0 > 0, didx.p + s0, dw.p + s0, G.p, scal.p);
+                dispatch_pair(c, cd_, X, ns, s0 > 0, didx.p + s0, dw.p + s0, G.p, dpart.p);
             }
             // Ubar_j(k) = sum_{j'} A_{jj'} U_{j'}  (textbook valence: sum_{j'} A_{j'j}^T U_{j'})
             Wb.alloc((size_t)nb * n * nk);
@@ -543,9 +571,13 @@ void grad_core(Ctx &c, const cd *X, int nx, const double *w, const GradExtra *ex
         hipLaunchKernelGGL(tb_adjoint_kernel, dim3(nk), dim3(64), 0, c.stream, nk, n, nv, nc, (int)idx.size(),
                            c.model.scissor, 1e-8, b.eall.p, b.uall.p, b.nocp.p, idx.empty() ? nullptr : Wb.p, X,
                            c.bse.N, didx.p, dw.p, ex ? ex->ebar : nullptr, ex ? ex->ubar : nullptr, Hbar.p,
-                           scal.p + kNP);
+                           apart.p);
         WTB_LAUNCH_CHECK();
     }
+    hipLaunchKernelGGL(fixed_sum_kernel, dim3(kNP), dim3(256), 0, c.stream, nk, kNP, dpart.p, scal.p);
+    hipLaunchKernelGGL(fixed_sum_kernel, dim3(2), dim3(256), 0, c.stream, nk, 2, apart.p + 3, scal.p + kNP);
+    WTB_HIP_CHECK(hipMemcpyAsync(scal.p + kNP + 2, apart.p + 2, sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    WTB_LAUNCH_CHECK();
     if (g_re || g_im) {
         const int nvec = c.model.nvec;
         const size_t tot = (size_t)nvec * n * n;
