@@ -11,7 +11,10 @@
 // k-points is computed: one 256-thread workgroup per tile (bk, bk'), bk' <= bk.
 //   phase 1: U(k') of the tile is staged in LDS; every (k,k') pair gets V
 //            (FP64) and its two overlap blocks (FP64 or FP32), which then
-//            replace the staged U(k') in LDS.
+//            replace the staged U(k') in LDS.  On the FP32 store the overlap
+//            blocks of the whole tile are two complex products
+//            U_c(k)^H U_c(k') and U_v(k)^H U_v(k') on FP32 MFMA (U(k) staged
+//            too); the FP64 store forms them with FMAs per pair.
 //   phase 2: the TK*nc*nv square tile is written column by column (each store
 //            covering consecutive rows of one column) and, off the diagonal,
 //            its conjugate transpose into the mirrored tile: the stored
@@ -173,6 +176,72 @@ __device__ __forceinline__ void sub_exchange(C &val, const cd *__restrict__ xa, 
     val.y -= (decltype(val.y))(vx * im);
 }
 
+typedef float mf4 __attribute__((ext_vector_type(4)));
+
+// Phase 1 of the FP32 tile on FP32 MFMA (v_mfma_f32_16x16x4f32): the overlap
+// blocks of all pairs of the tile are the complex products
+//   Mc = U_c(k)^H U_c(k')   ((TK nc) x (TK nc), depth norb)
+//   Mv = U_v(k)^H U_v(k')   ((TK nv) x (TK nv))
+// with columns (k, c) of U_c(k) at kl nc + c.  Each 16 x 16 output tile keeps
+// four real accumulators, Ar^T Br, Ai^T Bi, Ar^T Bi and Ai^T Br, combined at
+// the end (Re = rr + ii, Im = ri - ir): on a diagonal tile (U(k') = U(k))
+// element (j, i) then sums the same products in the same order as (i, j), so
+// the two are exact conjugates and the store stays exactly Hermitian.
+// Operand lane l holds row (column) l % 16 at depth l / 16; result register r
+// of lane l holds element (4 (l / 16) + r, l % 16).  The four wavefronts take
+// every fourth output tile; the results are written into the pair planes.
+template <int NC, int NV, int TK>
+__device__ __forceinline__ void pair_products_mfma(int norb, const float2 *uk, const float2 *ukp, int tid,
+                                                   float2 *pmc, float2 *pmv, int npp) {
+    constexpr int NB = NC + NV;
+    constexpr int TC = TK * NC / 16, TV = TK * NV / 16;     // 16-wide tiles per side
+    constexpr int NT = TC * TC + TV * TV;
+    constexpr int W = kThreads / 64, MAXT = (NT + W - 1) / W;
+    const int lane = tid & 63, wave = tid >> 6, li = lane & 15, lk = lane >> 4;
+    mf4 re[MAXT], im[MAXT];
+#pragma unroll
+    for (int u = 0; u < MAXT; ++u) {
+        const int t = wave + W * u;
+        if (t >= NT) break;
+        const bool isc = t < TC * TC;
+        const int tt = isc ? t : t - TC * TC, side = isc ? TC : TV, nb = isc ? NC : NV, j0 = isc ? NV : 0;
+        const int ti = tt / side, tj = tt % side;
+        // row i of A^H (column i of U(k)) and column j of B (column j of U(k'))
+        const int i = 16 * ti + li, j = 16 * tj + li;
+        const float2 *pa = uk + (size_t)(i / nb) * NB * norb + (size_t)(j0 + i % nb) * norb;
+        const float2 *pb = ukp + (size_t)(j / nb) * NB * norb + (size_t)(j0 + j % nb) * norb;
+        mf4 rr = {0.f, 0.f, 0.f, 0.f}, ii = rr, ri = rr, ir = rr;
+        for (int o0 = 0; o0 < norb; o0 += 4) {
+            const int o = o0 + lk;
+            const float2 a = o < norb ? pa[o] : make_float2(0.f, 0.f);
+            const float2 b = o < norb ? pb[o] : make_float2(0.f, 0.f);
+            rr = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.x, rr, 0, 0, 0);
+            ii = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.y, ii, 0, 0, 0);
+            ri = __builtin_amdgcn_mfma_f32_16x16x4f32(a.x, b.y, ri, 0, 0, 0);
+            ir = __builtin_amdgcn_mfma_f32_16x16x4f32(a.y, b.x, ir, 0, 0, 0);
+        }
+        re[u] = rr + ii;
+        im[u] = ri - ir;
+    }
+    __syncthreads();    // the pair planes overwrite the staged U
+#pragma unroll
+    for (int u = 0; u < MAXT; ++u) {
+        const int t = wave + W * u;
+        if (t >= NT) break;
+        const bool isc = t < TC * TC;
+        const int tt = isc ? t : t - TC * TC, side = isc ? TC : TV, nb = isc ? NC : NV;
+        const int ti = tt / side, tj = tt % side;
+        const int j = 16 * tj + li, kpl = j / nb, c2 = j % nb;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = 16 * ti + 4 * lk + r, kl = i / nb, c = i % nb;
+            const int p = kpl * (TK + 1) + kl;
+            float2 *plane = isc ? pmc + (size_t)(c * NC + c2) * npp : pmv + (size_t)(c * NV + c2) * npp;
+            plane[p] = make_float2(re[u][r], im[u][r]);
+        }
+    }
+}
+
 // Pair slots of one plane of pair data in LDS: pair (kl, kpl) sits at
 // kpl (TK + 1) + kl, so that lanes running over kl (row writes) and lanes
 // running over kpl (mirror writes) both hit distinct banks.
@@ -220,31 +289,54 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
     C *pmc = reinterpret_cast<C *>(smem);
     C *pmv = pmc + (size_t)NC2 * NPP;
     C *uks = pmc;
-    R *pv = reinterpret_cast<R *>(pmc + (pairs > stage ? pairs : stage));
+    constexpr bool MF = sizeof(R) == sizeof(float);   // phase 1 on FP32 MFMA
+    const size_t stage_all = (MF ? 2 : 1) * stage;
+    R *pv = reinterpret_cast<R *>(pmc + (pairs > stage_all ? pairs : stage_all));
+    C *ukk = uks + stage;                               // U(k) of the tile's rows (MFMA path)
     const int tid = threadIdx.x;
 
-    // stage U(k') for the tile's k' columns, eight loads in flight per thread
+    // stage U(k') for the tile's k' columns (and U(k) for its k rows on the
+    // MFMA path), eight loads in flight per thread
     {
         constexpr int kL = 8;
-        const int tot = TK * NB * norb;
-        for (int base = 0; base < tot; base += kL * kThreads) {
+        const int tot = TK * NB * norb, tot2 = (MF ? 2 : 1) * tot;
+        for (int base = 0; base < tot2; base += kL * kThreads) {
             C v[kL];
 #pragma unroll
             for (int u = 0; u < kL; ++u) {
-                const int idx = base + u * kThreads + tid;
+                const int idx0 = base + u * kThreads + tid;
+                const bool second = idx0 >= tot;
+                const int idx = second ? idx0 - tot : idx0;
                 const int kpl = idx / (NB * norb), rem = idx % (NB * norb), j = rem / norb, o = rem % norb;
-                const int kp = kp0 + kpl;
-                v[u] = (idx < tot && kp < nk) ? usel[((size_t)o * NB + j) * nk + kp] : mk<C>(0.0, 0.0);
+                const int kp = (second ? k0 : kp0) + kpl;
+                v[u] = (idx0 < tot2 && kp < nk) ? usel[((size_t)o * NB + j) * nk + kp] : mk<C>(0.0, 0.0);
             }
 #pragma unroll
             for (int u = 0; u < kL; ++u) {
                 const int idx = base + u * kThreads + tid;
-                if (idx < tot) uks[idx] = v[u];
+                if (idx < tot2) uks[idx] = v[u];
             }
         }
     }
     __syncthreads();
 
+    if constexpr (MF) {
+        pair_products_mfma<NC, NV, TK>(norb, reinterpret_cast<const float2 *>(ukk),
+                                       reinterpret_cast<const float2 *>(uks), tid, reinterpret_cast<float2 *>(pmc),
+                                       reinterpret_cast<float2 *>(pmv), NPP);
+        // Coulomb potential of every pair
+        const int kl = tid % TK, k = k0 + kl;
+#pragma unroll
+        for (int s = 0; s < S; ++s) {
+            const int kpl = tid / TK + (kThreads / TK) * s;
+            const int kp = kp0 + kpl;
+            double vq = 0.0;
+            if (k < nk && kp < nk)
+                vq = coulomb_v(coul, kpt[3 * k] - kpt[3 * kp], kpt[3 * k + 1] - kpt[3 * kp + 1],
+                               kpt[3 * k + 2] - kpt[3 * kp + 2]);
+            pv[kpl * (TK + 1) + kl] = (R)vq;
+        }
+    } else {
     // phase 1: pair products.  Thread -> kl = tid % TK, kpl = tid / TK + (kThreads / TK) s
     {
         const int kl = tid % TK;
@@ -277,6 +369,7 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
 #pragma unroll
             for (int e = 0; e < NV2; ++e) pmv[(size_t)e * NPP + p] = mv[s][e];
         }
+    }
     }
     __syncthreads();
 
@@ -395,7 +488,8 @@ void launch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const double *esel
     constexpr int TK = (size_t)pair_slots<32>() * per * sizeof(C) <= 72 * 1024 ? 32 : 16;
     const int nk = c.bands.nk, norb = c.model.norb;
     const long N = (long)nk * NC * NV, ld = c.bse.ld;
-    const size_t pairs = (size_t)pair_slots<TK>() * per, stage = (size_t)TK * (NC + NV) * norb;
+    const size_t pairs = (size_t)pair_slots<TK>() * per,
+                 stage = (sizeof(R) == sizeof(float) ? 2 : 1) * (size_t)TK * (NC + NV) * norb;
     const size_t shmem = std::max(pairs, stage) * sizeof(C) + (size_t)pair_slots<TK>() * sizeof(R) + 16;
     if (shmem > 160 * 1024) throw Error{WTB_ERR_UNSUPPORTED, "assembly tile does not fit in LDS"};
     const long nblk = (nk + TK - 1) / TK;
