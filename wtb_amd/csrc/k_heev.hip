@@ -1,10 +1,11 @@
 // Dense Hermitian eigensolver for the Rayleigh-Ritz step (FP64, n <= 1024).
 //
 //  1. Householder reduction to real symmetric tridiagonal form (lower, ZHETD2
-//     semantics with ZLARFG reflectors) in one 1024-thread workgroup; the
-//     reflectors stay in the lower triangle of A.  The whole matrix is staged
-//     in LDS for n <= kTridLds; above, the reduction is blocked (ZLATRD
-//     scheme): trailing updates are deferred over a panel of steps.
+//     semantics with ZLARFG reflectors) in one workgroup; the reflectors are
+//     written to the lower triangle of A.  Up to n = 224 the lower triangle
+//     is held in registers and LDS for the whole reduction; above, the
+//     reduction streams the matrix from global memory in the blocked ZLATRD
+//     scheme (trailing updates deferred over a panel of steps).
 //  2. Eigenvalues of the tridiagonal matrix by Sturm-count multisection over
 //     several workgroups; eigenvectors by inverse iteration with partial
 //     pivoting, one thread per vector with its factors in LDS, then modified
@@ -22,10 +23,8 @@ namespace {
 
 constexpr int kT = 1024;
 // largest n whose Hermitian matrix (n^2 complex) is staged in LDS by the
-// tridiagonalisation and the Cholesky factorisation (with their work arrays
-// it stays under the 160 KB of a CU)
-constexpr int kTridLds = 90;
-constexpr int kTridT = 256;     // threads of the LDS-resident tridiagonalisation
+// Cholesky factorisation (with its work arrays it stays under the 160 KB of
+// a CU)
 constexpr int kCholLds = 96;
 // Eigenvalues closer than this (relative to |T|) are treated as one cluster
 // and their inverse-iteration vectors are re-orthogonalised.  Outside a
@@ -48,11 +47,10 @@ __device__ __forceinline__ double block_reduce(double v, double *red) {
 
 // ---------------------------------------------------------------------------
 // 1. tridiagonalisation
-// A (n x n, column-major, lda): full Hermitian matrix, destroyed.  On exit
-// A[r + i lda] (r >= i+2) holds v_i(r) of reflector i (v_i(i+1) = 1), tau[i]
-// its scalar, d/e the real tridiagonal matrix T = Q^H A Q.
-// The whole matrix is staged in LDS (n <= kTridLds); NT threads (256 keeps
-// the nine barriers of a step cheap at these sizes).
+// A (n x n, column-major, lda): full Hermitian matrix (the lower triangle is
+// read), destroyed.  On exit A[r + i lda] (r >= i+2) holds v_i(r) of
+// reflector i (v_i(i+1) = 1), tau[i] its scalar, d/e the real tridiagonal
+// matrix T = Q^H A Q.
 // Block sum with one barrier: alternate calls use the two halves of red, and
 // every thread has read a half before any thread passes the next call's
 // barrier, so the half is free again two calls later.
@@ -70,124 +68,7 @@ __device__ __forceinline__ double block_reduce_db(double v, double *red, int &ha
     return s;
 }
 
-template <int NT>
-__global__ void __launch_bounds__(NT)
-tridiag_kernel(int n, cd *__restrict__ Ag, int ldg, cd *__restrict__ tau, double *__restrict__ d,
-               double *__restrict__ e) {
-    extern __shared__ __align__(16) unsigned char smem[];
-    cd *v = reinterpret_cast<cd *>(smem);     // n
-    cd *y = v + n;                            // n
-    cd *part = y + n;                         // NT partial sums
-    cd *M = part + NT;                        // n x n
-    __shared__ double red[2 * (NT / 64)];
-    __shared__ cd s_tau;
-    int half = 0;
-    const int tid = threadIdx.x;
-    cd *A = M;
-    const int lda = n;
-    for (int idx = tid; idx < n * n; idx += NT) M[idx] = Ag[(idx % n) + (size_t)(idx / n) * ldg];
-    __syncthreads();
-    for (int i = 0; i < n - 1; ++i) {
-        const int s = n - i - 1;                               // trailing size
-        // column norm below the subdiagonal
-        double xn2 = 0.0;
-        for (int r = i + 2 + tid; r < n; r += NT) {
-            cd a = A[r + (size_t)i * lda];
-            xn2 += a.x * a.x + a.y * a.y;
-        }
-        xn2 = block_reduce_db<NT>(xn2, red, half);
-        const cd alpha = A[(i + 1) + (size_t)i * lda];
-        d[i] = A[i + (size_t)i * lda].x;
-        if (xn2 == 0.0 && alpha.y == 0.0) {
-            // H(i) = I
-            if (tid == 0) {
-                e[i] = alpha.x;
-                tau[i] = make_double2(0.0, 0.0);
-            }
-            __syncthreads();
-            continue;
-        }
-        const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
-        const double beta = alpha.x >= 0.0 ? -nrm : nrm;
-        const cd t = make_double2((beta - alpha.x) / beta, -alpha.y / beta);
-        const double dd = (alpha.x - beta) * (alpha.x - beta) + alpha.y * alpha.y;
-        const cd scal = make_double2((alpha.x - beta) / dd, -alpha.y / dd);   // 1/(alpha - beta)
-        for (int r = i + 1 + tid; r < n; r += NT) {
-            cd vr;
-            if (r == i + 1) {
-                vr = make_double2(1.0, 0.0);
-            } else {
-                cd a = A[r + (size_t)i * lda];
-                vr = make_double2(a.x * scal.x - a.y * scal.y, a.x * scal.y + a.y * scal.x);
-                A[r + (size_t)i * lda] = vr;      // keep the reflector
-            }
-            v[r] = vr;
-        }
-        if (tid == 0) {
-            e[i] = beta;
-            tau[i] = t;
-            s_tau = t;
-        }
-        __syncthreads();
-        // y = A_trail v  (rows i+1..n-1), threads split the columns in NT/64 groups
-        {
-            constexpr int G = NT / 64;
-            const int g = tid / 64, lr = tid % 64;
-            for (int r0 = 0; r0 < s; r0 += 64) {
-                const int r = i + 1 + r0 + lr;
-                cd acc = make_double2(0.0, 0.0);
-                if (r < n) {
-                    for (int c = i + 1 + g; c < n; c += G) {
-                        cd a = A[r + (size_t)c * lda], vc = v[c];
-                        acc.x += a.x * vc.x - a.y * vc.y;
-                        acc.y += a.x * vc.y + a.y * vc.x;
-                    }
-                }
-                part[tid] = acc;
-                __syncthreads();
-                if (g == 0 && r < n) {
-                    cd sum = part[lr];
-                    for (int q = 1; q < G; ++q) {
-                        sum.x += part[q * 64 + lr].x;
-                        sum.y += part[q * 64 + lr].y;
-                    }
-                    y[r] = sum;
-                }
-                __syncthreads();
-            }
-        }
-        // mu = Re(v^H y), w = tau y - |tau|^2 mu / 2 v
-        double mu = 0.0;
-        for (int r = i + 1 + tid; r < n; r += NT) mu += v[r].x * y[r].x + v[r].y * y[r].y;
-        mu = block_reduce_db<NT>(mu, red, half);
-        const double t2 = t.x * t.x + t.y * t.y;
-        for (int r = i + 1 + tid; r < n; r += NT) {
-            cd yr = y[r], vr = v[r];
-            y[r] = make_double2(t.x * yr.x - t.y * yr.y - 0.5 * t2 * mu * vr.x,
-                                t.x * yr.y + t.y * yr.x - 0.5 * t2 * mu * vr.y);
-        }
-        __syncthreads();
-        // A_trail -= v w^H + w v^H
-        for (int idx = tid; idx < s * s; idx += NT) {
-            const int r = i + 1 + idx % s, c = i + 1 + idx / s;
-            cd vr = v[r], wr = y[r], vc = v[c], wc = y[c];
-            cd a = A[r + (size_t)c * lda];
-            // v_r conj(w_c) + w_r conj(v_c)
-            a.x -= (vr.x * wc.x + vr.y * wc.y) + (wr.x * vc.x + wr.y * vc.y);
-            a.y -= (vr.y * wc.x - vr.x * wc.y) + (wr.y * vc.x - wr.x * vc.y);
-            A[r + (size_t)c * lda] = a;
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        d[n - 1] = A[(n - 1) + (size_t)(n - 1) * lda].x;
-        if (n >= 1) tau[n > 1 ? n - 1 : 0] = make_double2(0.0, 0.0);
-    }
-    // the reflectors (strict lower part) are read by the back-transformation
-    for (int idx = tid; idx < n * n; idx += NT) Ag[(idx % n) + (size_t)(idx / n) * ldg] = M[idx];
-}
-
-// Blocked form of the same reduction for n > kTridLds (the LATRD scheme).  The
+// Blocked form of the same reduction for n > 32 kRegNA (the LATRD scheme).  The
 // trailing matrix in global memory is brought up to date only once per panel
 // of nb steps; inside a panel the current matrix is A - V W^H - W V^H, with
 // the panel's reflectors V and their companions W held in LDS.  Each step then
@@ -371,6 +252,289 @@ tridiag_blocked_kernel(int n, int nb, cd *__restrict__ A, int lda, cd *__restric
     if (tid == 0) {
         d[n - 1] = A[(n - 1) + (size_t)(n - 1) * lda].x;
         tau[n > 1 ? n - 1 : 0] = make_double2(0.0, 0.0);
+    }
+}
+
+// Register-resident form of the same reduction for n <= 32 kRegNA: the lower
+// triangle lives in the registers of 512 threads and in LDS, so a step reads
+// no matrix data from memory and the pending rank-2 update shares one pass
+// with the next product.
+// Thread (p, q), p the row class mod 32 and q the column class mod 16 (a
+// wavefront holds four p and all sixteen q, one p per 16-lane row), owns the
+// elements (p + 32a, q + 16b):
+//  - b < 2a, strictly below the diagonal 32 x 32 blocks: NA (NA - 1) values
+//    in registers;
+//  - b = 2a, 2a + 1, inside diagonal block a, lower half only: packed in LDS
+//    (528 values a block).
+// Step i starts from column i, fully updated and staged in LDS.  Every
+// wavefront forms reflector i from it (a private copy, no barrier); every
+// thread then applies the pending update (reflector i - 1) to its live
+// elements and, in the same pass, adds their share of y = A v (A_rc v_c to row
+// r and, off the diagonal, conj(A_rc) v_r to row c) and of v^H A v, while the
+// owners of column i + 1 stage it.  Row sums reduce over the sixteen q lanes
+// with DPP moves, column sums over the four p rows with the permlane swaps
+// and then over the eight wavefronts in LDS (barrier); each thread forms w =
+// tau y - |tau|^2 (v^H A v) / 2 v for its row and applies reflector i to the
+// staged column i + 1 (barrier).  Elements whose 16-column block lies left of
+// the trailing matrix are skipped.
+constexpr int kRT = 512;
+constexpr int kRegNA = 7;          // n <= 224
+
+// The helpers below are chains of FMAs (one rounding per product): the pass
+// over the live elements is bound by FP64 issue, 16 FMAs an element a step.
+__device__ __forceinline__ void rank2_sub(cd &A, cd vr, cd wr, cd vc, cd wc) {
+    // A -= v_r conj(w_c) + w_r conj(v_c)
+    A.x = fma(-vr.x, wc.x, A.x);
+    A.y = fma(-vr.y, wc.x, A.y);
+    A.x = fma(-vr.y, wc.y, A.x);
+    A.y = fma(vr.x, wc.y, A.y);
+    A.x = fma(-wr.x, vc.x, A.x);
+    A.y = fma(-wr.y, vc.x, A.y);
+    A.x = fma(-wr.y, vc.y, A.x);
+    A.y = fma(wr.x, vc.y, A.y);
+}
+
+__device__ __forceinline__ void cmac(cd &acc, cd a, cd b) {     // acc += a b
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.x = fma(-a.y, b.y, acc.x);
+    acc.y = fma(a.y, b.x, acc.y);
+}
+
+__device__ __forceinline__ void cmac_conj(cd &acc, cd a, cd b) {   // acc += conj(a) b
+    acc.x = fma(a.x, b.x, acc.x);
+    acc.y = fma(a.x, b.y, acc.y);
+    acc.x = fma(a.y, b.y, acc.x);
+    acc.y = fma(-a.y, b.x, acc.y);
+}
+
+// v of the lane given by the DPP control (both halves of the double moved)
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const long long u = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_mov_dpp((int)u, CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xF, 0xF, true);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+
+// sum over the 16 lanes of a row, the same in every lane of it: lane pairs
+// l ^ 1 and l ^ 2 (quad permutations), then l <-> 7 - l and l <-> 15 - l
+// within the row (half mirror, mirror)
+__device__ __forceinline__ double row_sum(double v) {
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    v += dpp_mov<0x141>(v);
+    v += dpp_mov<0x140>(v);
+    return v;
+}
+
+// v[l] + v[l ^ 16] (permlane16 swap) or v[l] + v[l ^ 32] (permlane32 swap),
+// added in the same order in both lanes of a pair
+template <bool S32>
+__device__ __forceinline__ double swap_sum(double v) {
+    const long long u = __double_as_longlong(v);
+    const unsigned lo = (unsigned)u, hi = (unsigned)(u >> 32);
+    auto L = S32 ? __builtin_amdgcn_permlane32_swap(lo, lo, false, false)
+                 : __builtin_amdgcn_permlane16_swap(lo, lo, false, false);
+    auto H = S32 ? __builtin_amdgcn_permlane32_swap(hi, hi, false, false)
+                 : __builtin_amdgcn_permlane16_swap(hi, hi, false, false);
+    const double a = __longlong_as_double(((long long)H[0] << 32) | L[0]);
+    const double b = __longlong_as_double(((long long)H[1] << 32) | L[1]);
+    return a + b;
+}
+
+__device__ __forceinline__ double rows_sum(double v) { return swap_sum<true>(swap_sum<false>(v)); }
+__device__ __forceinline__ double wave_sum_d(double v) { return rows_sum(row_sum(v)); }
+
+template <int NA>
+__global__ void __launch_bounds__(kRT)
+tridiag_reg_kernel(int n, cd *__restrict__ Ag, int lda, cd *__restrict__ tau, double *__restrict__ d,
+                   double *__restrict__ e) {
+    constexpr int NR = 32 * NA, NDB = 528, NW = kRT / 64, NK = NR / 64 + (NR % 64 != 0);
+    constexpr int NREG = NA > 1 ? NA * (NA - 1) : 1;
+    __shared__ cd D[NA * NDB];     // packed lower halves of the diagonal blocks
+    __shared__ cd colbuf[NR];      // column i, up to date
+    __shared__ cd colpre[NR];      // column i + 1 before reflector i
+    __shared__ cd vsh[2][NR];      // reflectors (pending, current), written by wavefront 0
+    __shared__ cd wsh[2][NR];      // their companions w
+    __shared__ cd vw[NW][NR];      // the current reflector, one copy per wavefront
+    __shared__ cd rowpart[NR];     // row sums of y
+    __shared__ cd colpart[NW][NR]; // column sums of y, one row per wavefront
+    __shared__ double mured[NW];
+    const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
+    const int q = l & 15, p = 4 * wv + (l >> 4);
+    const int prow = p * (p + 1) / 2;
+    cd *vme = vw[wv];
+
+    cd R[NREG];
+#pragma unroll
+    for (int a = 1; a < NA; ++a)
+#pragma unroll
+        for (int b = 0; b < 2 * a; ++b) {
+            const int r = p + 32 * a, c = q + 16 * b;
+            R[a * (a - 1) + b] = (r < n && c < n) ? Ag[r + (size_t)c * lda] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+    for (int a = 0; a < NA; ++a)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int qq = q + 16 * h, r = 32 * a + p, c = 32 * a + qq;
+            if (qq <= p) {
+                cd x = (r < n && c < n) ? Ag[r + (size_t)c * lda] : make_double2(0.0, 0.0);
+                if (qq == p) x.y = 0.0;
+                D[a * NDB + prow + qq] = x;
+            }
+        }
+    for (int r = tid; r < 2 * NR; r += kRT) {
+        vsh[r / NR][r % NR] = make_double2(0.0, 0.0);
+        wsh[r / NR][r % NR] = make_double2(0.0, 0.0);
+    }
+    for (int r = tid; r < NR; r += kRT) {
+        cd x = r < n ? Ag[r] : make_double2(0.0, 0.0);
+        if (r == 0) x.y = 0.0;
+        colbuf[r] = x;
+    }
+    __syncthreads();
+    for (int i = 0; i < n - 1; ++i) {
+        const int pn = i & 1;
+        const cd *vo = vsh[pn ^ 1], *wo = wsh[pn ^ 1];
+        // reflector i from the staged column, formed by every wavefront
+        cd t = make_double2(0.0, 0.0);
+        {
+            cd x[NK];
+            double xn2 = 0.0;
+#pragma unroll
+            for (int k = 0; k < NK; ++k) {
+                const int r = l + 64 * k;
+                x[k] = r < NR ? colbuf[r] : make_double2(0.0, 0.0);
+                if (r >= i + 2 && r < n) xn2 = fma(x[k].x, x[k].x, fma(x[k].y, x[k].y, xn2));
+            }
+            xn2 = wave_sum_d(xn2);
+            const cd alpha = colbuf[i + 1];
+            const bool ident = xn2 == 0.0 && alpha.y == 0.0;    // H(i) = I
+            double beta = alpha.x;
+            cd scal = make_double2(0.0, 0.0);
+            if (!ident) {
+                const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+                beta = alpha.x >= 0.0 ? -nrm : nrm;
+                const double ib = 1.0 / beta;
+                t = make_double2((beta - alpha.x) * ib, -alpha.y * ib);
+                const double idd = 1.0 / ((alpha.x - beta) * (alpha.x - beta) + alpha.y * alpha.y);
+                scal = make_double2((alpha.x - beta) * idd, -alpha.y * idd);   // 1/(alpha - beta)
+            }
+#pragma unroll
+            for (int k = 0; k < NK; ++k) {
+                const int r = l + 64 * k;
+                if (r < NR) {
+                    cd v = make_double2(0.0, 0.0);
+                    if (!ident && r > i && r < n)
+                        v = r == i + 1 ? make_double2(1.0, 0.0)
+                                       : make_double2(x[k].x * scal.x - x[k].y * scal.y,
+                                                      x[k].x * scal.y + x[k].y * scal.x);
+                    vme[r] = v;
+                    if (wv == 0) {
+                        vsh[pn][r] = v;
+                        if (r >= i + 2 && r < n) Ag[r + (size_t)i * lda] = v;   // kept for the back-transformation
+                    }
+                }
+            }
+            if (wv == 0 && l == 0) {
+                d[i] = colbuf[i].x;
+                e[i] = beta;
+                tau[i] = t;
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        // pending update and y = A v in one pass over the live elements; the
+        // owners of column i + 1 stage it
+        const int bnx = (i + 1) >> 4, qnx = (i + 1) & 15;
+        cd racc[NA];
+#pragma unroll
+        for (int a = 0; a < NA; ++a) racc[a] = make_double2(0.0, 0.0);
+        double mup = 0.0;
+#pragma unroll
+        for (int b = 0; b < 2 * NA; ++b) {
+            if (16 * b + 15 <= i) continue;      // columns <= i are final
+            const int c = q + 16 * b;
+            const cd voc = vo[c], woc = wo[c], vnc = vme[c];
+            cd cacc = make_double2(0.0, 0.0), Ad = make_double2(0.0, 0.0);
+            const int qq = q + 16 * (b & 1), rd = 32 * (b >> 1) + p;
+            if (qq <= p) {
+                Ad = D[(b >> 1) * NDB + prow + qq];
+                rank2_sub(Ad, vo[rd], wo[rd], voc, woc);
+                D[(b >> 1) * NDB + prow + qq] = Ad;
+                cmac(racc[b >> 1], Ad, vnc);
+                if (qq < p) cmac_conj(cacc, Ad, vme[rd]);
+            }
+#pragma unroll
+            for (int a = b / 2 + 1; a < NA; ++a) {
+                const int r = p + 32 * a;
+                cd &A = R[a * (a - 1) + b];
+                rank2_sub(A, vo[r], wo[r], voc, woc);
+                cmac(racc[a], A, vnc);
+                cmac_conj(cacc, A, vme[r]);
+            }
+            if (b == bnx && q == qnx) {
+                if (qq <= p) colpre[rd] = Ad;
+#pragma unroll
+                for (int a = b / 2 + 1; a < NA; ++a) colpre[p + 32 * a] = R[a * (a - 1) + b];
+            }
+            mup = fma(vnc.x, cacc.x, fma(vnc.y, cacc.y, mup));
+            cacc.x = rows_sum(cacc.x);
+            cacc.y = rows_sum(cacc.y);
+            if (l < 16) colpart[wv][c] = cacc;
+        }
+#pragma unroll
+        for (int a = 0; a < NA; ++a) {
+            if (32 * a + 31 <= i) continue;      // rows <= i are not needed
+            const int r = p + 32 * a;
+            const cd vnr = vme[r];
+            cd s = racc[a];
+            mup = fma(vnr.x, s.x, fma(vnr.y, s.y, mup));
+            s.x = row_sum(s.x);
+            s.y = row_sum(s.y);
+            if (q == 0) rowpart[r] = s;
+        }
+        mup = wave_sum_d(mup);
+        if (l == 0) mured[wv] = mup;
+        __syncthreads();
+        // w = tau y - |tau|^2 mu / 2 v (mu = v^H A v) for each row; column
+        // i + 1 (rows > i) after reflector i, with w_(i+1) formed by every thread
+        {
+            double mu = 0.0;
+#pragma unroll
+            for (int k = 0; k < NW; ++k) mu += mured[k];
+            const double hm = 0.5 * (t.x * t.x + t.y * t.y) * mu;
+            auto w_of = [&](int r) {
+                cd y = rowpart[r];
+#pragma unroll
+                for (int k = 0; k < NW; ++k) {
+                    const cd cp = colpart[k][r];
+                    y.x += cp.x;
+                    y.y += cp.y;
+                }
+                const cd v = vme[r];
+                return make_double2(t.x * y.x - t.y * y.y - hm * v.x, t.x * y.y + t.y * y.x - hm * v.y);
+            };
+            const int j = i + 1;
+            const cd wj = w_of(j), vj = vme[j];
+            for (int r = tid; r < NR; r += kRT) {
+                cd w = make_double2(0.0, 0.0), cb = make_double2(0.0, 0.0);
+                if (r > i && r < n) {
+                    w = w_of(r);
+                    cb = colpre[r];
+                    rank2_sub(cb, vme[r], w, vj, wj);
+                    if (r == j) cb.y = 0.0;
+                }
+                wsh[pn][r] = w;
+                colbuf[r] = cb;
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        d[n - 1] = colbuf[n - 1].x;
+        tau[n - 1] = make_double2(0.0, 0.0);
     }
 }
 
@@ -947,10 +1111,17 @@ void heev_lowest(Ctx &c, int n, int nvec, cd *A, int lda, double *W, cd *Z, int 
     c.ws_heev_d.alloc((size_t)2 * n + (size_t)n * nvec);
     cd *tau = c.ws_heev_c.p;
     double *d = c.ws_heev_d.p, *e = d + n, *Y = e + n;
-    if (n <= kTridLds)
-        hipLaunchKernelGGL(tridiag_kernel<kTridT>, dim3(1), dim3(kTridT),
-                           (2 * (size_t)n + kTridT + (size_t)n * n) * sizeof(cd), c.stream, n, A, lda, tau, d, e);
-    else {
+    if (n <= 32 * kRegNA) {
+        switch ((n + 31) / 32) {
+#define WTB_TRID_CASE(K)                                                                                  \
+    case K:                                                                                               \
+        hipLaunchKernelGGL(tridiag_reg_kernel<K>, dim3(1), dim3(kRT), 0, c.stream, n, A, lda, tau, d, e); \
+        break;
+            WTB_TRID_CASE(1) WTB_TRID_CASE(2) WTB_TRID_CASE(3) WTB_TRID_CASE(4)
+            WTB_TRID_CASE(5) WTB_TRID_CASE(6) WTB_TRID_CASE(7)
+#undef WTB_TRID_CASE
+        }
+    } else {
         // panel width: V and W (2 nb n), y (n) and the partial sums (kT) in 156 KB
         const size_t fixed = ((size_t)n + kT) * sizeof(cd);
         const int nb = (int)std::min<size_t>(kMaxNb, (156 * 1024 - fixed) / (2 * (size_t)n * sizeof(cd)));
