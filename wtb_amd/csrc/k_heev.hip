@@ -308,23 +308,14 @@ __device__ __forceinline__ void cmac_conj(cd &acc, cd a, cd b) {   // acc += con
     acc.y = fma(-a.y, b.x, acc.y);
 }
 
-// v of the lane given by the DPP control (both halves of the double moved)
-template <int CTRL>
-__device__ __forceinline__ double dpp_mov(double v) {
-    const long long u = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_mov_dpp((int)u, CTRL, 0xF, 0xF, true);
-    const int hi = __builtin_amdgcn_mov_dpp((int)(u >> 32), CTRL, 0xF, 0xF, true);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
-}
-
 // sum over the 16 lanes of a row, the same in every lane of it: lane pairs
 // l ^ 1 and l ^ 2 (quad permutations), then l <-> 7 - l and l <-> 15 - l
 // within the row (half mirror, mirror)
 __device__ __forceinline__ double row_sum(double v) {
-    v += dpp_mov<0xB1>(v);
-    v += dpp_mov<0x4E>(v);
-    v += dpp_mov<0x141>(v);
-    v += dpp_mov<0x140>(v);
+    v += dpp_double<0xB1>(v);
+    v += dpp_double<0x4E>(v);
+    v += dpp_double<0x141>(v);
+    v += dpp_double<0x140>(v);
     return v;
 }
 

@@ -7,7 +7,8 @@
 // runs with only its real Z in LDS (four times the wavefronts per CU of the
 // complex A and Q of steps 1-2):
 //   1. Fourier sum: each lane owns up to 16 matrix entries and accumulates
-//      over R in the reference's order; phases are staged in LDS per chunk.
+//      over R in the reference's order; phases are staged in LDS per chunk
+//      of 64 R.
 //   2. Householder reduction to a real tridiagonal T = Q^H H(k) Q with the
 //      unblocked lower reflectors of LAPACK ZHETD2 (ZLARFG), Q accumulated in
 //      LDS.
@@ -31,10 +32,7 @@ __device__ __forceinline__ cd csub(cd a, cd b) { return make_double2(a.x - b.x, 
 __device__ __forceinline__ cd cscale(double s, cd a) { return make_double2(s * a.x, s * a.y); }
 __device__ __forceinline__ cd cconj(cd a) { return make_double2(a.x, -a.y); }
 
-__device__ __forceinline__ double wave_sum(double v) {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
-}
+__device__ __forceinline__ double wave_sum(double v) { return wave_sum_dpp(v); }
 
 constexpr int kLanes = 64;
 
@@ -63,13 +61,16 @@ hk_house_kernel(int n, int nvec, int flags, double efermi, const double *__restr
     cd *ph = V + nn;                                 // kLanes phases
     cd *vb = ph + kLanes;                            // 32: Householder vector
     cd *yb = vb + kMaxOrb;                           // 32: A v
+    cd *wb = yb + kMaxOrb;                           // 32: w = tau y - |tau|^2 (v^H y) / 2 v
     const bool compat = flags & WTB_FLAG_COMPAT_F32;
     const double kx = kpt[3 * k], ky = kpt[3 * k + 1], kz = kpt[3 * k + 2];
 
     // ---- 1. Fourier sum ----
-    cd acc[kMaxOrb * kMaxOrb / kLanes];
-#pragma unroll
-    for (int s = 0; s < kMaxOrb * kMaxOrb / kLanes; ++s) acc[s] = make_double2(0.0, 0.0);
+    // Entry idx = tid + 64 s accumulates over R in the reference's order; the
+    // partial sums wait in A between chunks of 64 R, and within a chunk the
+    // loop over R is unrolled so that several H(R) loads are in flight.
+    const int ns = (nn + kLanes - 1) / kLanes;
+    for (int idx = tid; idx < nn; idx += kLanes) A[idx] = make_double2(0.0, 0.0);
     for (int r0 = 0; r0 < nvec; r0 += kLanes) {
         int r = r0 + tid;
         if (r < nvec) {
@@ -80,35 +81,30 @@ hk_house_kernel(int n, int nvec, int flags, double efermi, const double *__restr
             ph[tid] = make_double2(cs * invdeg[r], sn * invdeg[r]);
         }
         __syncthreads();
-        int rend = min(kLanes, nvec - r0);
-        for (int rr = 0; rr < rend; ++rr) {
-            cd e = ph[rr];
-            const cd *hr = hop + (size_t)(r0 + rr) * nn;
-#pragma unroll
-            for (int s = 0; s < kMaxOrb * kMaxOrb / kLanes; ++s) {
-                int idx = tid + s * kLanes;
-                if (idx < nn) acc[s] = cadd(acc[s], cmul(e, hr[idx]));
-            }
+        const int rend = min(kLanes, nvec - r0);
+        for (int s = 0; s < ns; ++s) {
+            const int idx = tid + s * kLanes, ic = min(idx, nn - 1);
+            const cd *hb = hop + (size_t)r0 * nn + ic;
+            cd acc = A[ic];
+#pragma unroll 8
+            for (int rr = 0; rr < rend; ++rr) acc = cadd(acc, cmul(ph[rr], hb[(size_t)rr * nn]));
+            if (idx < nn) A[idx] = acc;
         }
         __syncthreads();
     }
     const double ef = compat ? (double)(float)efermi : efermi;
-#pragma unroll
-    for (int s = 0; s < kMaxOrb * kMaxOrb / kLanes; ++s) {
-        int idx = tid + s * kLanes;
-        if (idx < nn) {
-            cd h = acc[s];
-            if (idx / n == idx % n) h.x -= ef;
-            A[idx] = h;
-        }
-    }
+    if (tid < n) A[tid * n + tid].x -= ef;
     __syncthreads();
 
     // ---- 2. Householder reduction T = Q^H H Q (ZHETD2, lower, ZLARFG reflectors) ----
     // T is real tridiagonal (diagonal d, sub-diagonal beta); Q accumulates in V.
+    // The products and updates touch only the trailing rows and columns and
+    // use all 64 lanes: lane (c2, h2) takes row or column c2 and every other
+    // element along it from h2, the two halves joined by one xor-32 shuffle.
     for (int idx = tid; idx < nn; idx += kLanes)
         V[idx] = make_double2((idx / n == idx % n) ? 1.0 : 0.0, 0.0);
     __syncthreads();
+    const int c2 = tid & 31, h2 = tid >> 5;
     for (int i = 0; i < n - 1; ++i) {
         double xn2 = 0.0;
         if (tid >= i + 2 && tid < n) {
@@ -134,31 +130,50 @@ hk_house_kernel(int n, int nvec, int flags, double efermi, const double *__restr
             vb[tid] = v;
         }
         __syncthreads();
-        if (tid < n) {
+        if (tid == 0) A[(i + 1) * n + i] = make_double2(beta, 0.0);    // T(i+1, i)
+        // y = A v on the trailing rows
+        {
             cd y = make_double2(0.0, 0.0);
-            for (int c = i + 1; c < n; ++c) y = cadd(y, cmul(A[tid * n + c], vb[c]));
-            yb[tid] = y;
+            if (c2 > i && c2 < n)
+#pragma unroll 4
+                for (int c = i + 1 + h2; c < n; c += 2) y = cadd(y, cmul(A[c2 * n + c], vb[c]));
+            y.x += __shfl_xor(y.x, 32, 64);
+            y.y += __shfl_xor(y.y, 32, 64);
+            if (h2 == 0 && c2 < n) yb[c2] = y;
         }
         __syncthreads();
+        // w = tau y - |tau|^2 mu / 2 v, mu = v^H y
         double mu = 0.0;
         if (tid > i && tid < n) mu = vb[tid].x * yb[tid].x + vb[tid].y * yb[tid].y;
         mu = wave_sum(mu);
-        const double t2 = tau.x * tau.x + tau.y * tau.y;
-        // A <- H(i)^H A H(i):  - tau y_r conj(v_c) - conj(tau) v_r conj(y_c) + |tau|^2 mu v_r conj(v_c)
-        for (int idx = tid; idx < nn; idx += kLanes) {
-            const int r = idx / n, c = idx % n;
-            const cd vr = vb[r], vc = vb[c], yr = yb[r], yc = yb[c];
-            const cd t1 = cmul(tau, cmul(yr, cconj(vc)));
-            const cd t2c = cmul(cconj(tau), cmul(vr, cconj(yc)));
-            const cd t3 = cscale(t2 * mu, cmul(vr, cconj(vc)));
-            A[idx] = cadd(csub(csub(A[idx], t1), t2c), t3);
-        }
-        // Q <- Q H(i) (row tid)
+        const double hm = 0.5 * (tau.x * tau.x + tau.y * tau.y) * mu;
         if (tid < n) {
+            const cd y = yb[tid], v = vb[tid];
+            wb[tid] = make_double2(tau.x * y.x - tau.y * y.y - hm * v.x, tau.x * y.y + tau.y * y.x - hm * v.y);
+        }
+        __syncthreads();
+        // A <- H(i)^H A H(i) = A - v w^H - w v^H on the trailing block;
+        // Q <- Q H(i) = Q - tau (Q v) v^H, row c2
+        if (c2 > i && c2 < n) {
+            const cd vc = vb[c2], wc = wb[c2];
+            for (int r = i + 1 + h2; r < n; r += 2) {
+                const cd vr = vb[r], wr = wb[r];
+                cd a = A[r * n + c2];
+                a.x -= (vr.x * wc.x + vr.y * wc.y) + (wr.x * vc.x + wr.y * vc.y);
+                a.y -= (vr.y * wc.x - vr.x * wc.y) + (wr.y * vc.x - wr.x * vc.y);
+                A[r * n + c2] = a;
+            }
+        }
+        {
             cd sacc = make_double2(0.0, 0.0);
-            for (int c = i + 1; c < n; ++c) sacc = cadd(sacc, cmul(V[tid * n + c], vb[c]));
+            if (c2 < n)
+#pragma unroll 4
+                for (int c = i + 1 + h2; c < n; c += 2) sacc = cadd(sacc, cmul(V[c2 * n + c], vb[c]));
+            sacc.x += __shfl_xor(sacc.x, 32, 64);
+            sacc.y += __shfl_xor(sacc.y, 32, 64);
             const cd ts = cmul(tau, sacc);
-            for (int c = i + 1; c < n; ++c) V[tid * n + c] = csub(V[tid * n + c], cmul(ts, cconj(vb[c])));
+            if (c2 < n)
+                for (int c = i + 1 + h2; c < n; c += 2) V[c2 * n + c] = csub(V[c2 * n + c], cmul(ts, cconj(vb[c])));
         }
         __syncthreads();
     }
@@ -506,7 +521,7 @@ void hk_eig_on(Ctx &c, const double *kpt, int nk, int nv, int nc, Bands &b) {
     b.rix.alloc((size_t)nk * cap);
     b.nrot.alloc(nk);
     b.zrot.alloc((size_t)nk * n * n);
-    const size_t sh_house = 2 * (size_t)n * n * sizeof(cd) + (kLanes + 2 * kMaxOrb) * sizeof(cd);
+    const size_t sh_house = 2 * (size_t)n * n * sizeof(cd) + (kLanes + 3 * kMaxOrb) * sizeof(cd);
     const size_t sh_vec = (size_t)n * n * (sizeof(cd) + sizeof(double)) + kMaxOrb * (sizeof(double) + sizeof(int));
     // lane_ql: the QL on one lane per k with a rotation log replayed by
     // hk_rot_kernel; otherwise one wavefront per k rotates Z in place

@@ -292,11 +292,6 @@ struct StageTimer {
     }
 };
 
-// k.R = (kx Rx + ky Ry) + kz Rz, evaluated without FMA contraction in the
-// reference's order.  On a Monkhorst-Pack mesh k.R can land exactly on a
-// float32 rounding midpoint, so the sign of the double rounding error decides
-// the float32 phase of the compat mode: every evaluation must use the same
-// plain IEEE operations.
 // value of lane i (uniform i) of a per-lane double
 __device__ __forceinline__ double wave_lane(double v, int i) {
     const long long b = __double_as_longlong(v);
@@ -327,6 +322,11 @@ __device__ __forceinline__ double wave_sum_dpp(double v) {
     return (wave_lane(v, 0) + wave_lane(v, 16)) + (wave_lane(v, 32) + wave_lane(v, 48));
 }
 
+// k.R = (kx Rx + ky Ry) + kz Rz, evaluated without FMA contraction in the
+// reference's order.  On a Monkhorst-Pack mesh k.R can land exactly on a
+// float32 rounding midpoint, so the sign of the double rounding error decides
+// the float32 phase of the compat mode: every evaluation must use the same
+// plain IEEE operations.
 __device__ __forceinline__ double kdotr(double kx, double ky, double kz, const double *R) {
 #pragma clang fp contract(off)
     const double a = kx * R[0];
