@@ -188,7 +188,14 @@ int wtb_set_temperature(wtb_ctx *ctx, double temp_k);
  * (module_slepc_solver.F90:12-97) with nev explicit.  tol: relative residual
  * (SLEPc default 1e-8); max_it: outer iterations (200).
  *   E : double EX(nev), ascending;  X : complex(8) HE(N,nev) or NULL;
- *   rep : may be NULL. */
+ *   rep : may be NULL.
+ * tol bounds every returned pair's relative residual |Hx - Ex| / |E| in every
+ * precision, measured with the FP64 product (FP64 store) or the exact
+ * int8-digit product recombined in FP64 (FP32 store).  On an FP32 store the
+ * reduced-precision iterations stop at max(tol, 5e-6) and the solver goes on
+ * with the accurate product while a residual is above tol.  A solve that
+ * does not reach tol within max_it returns WTB_ERR_NOCONV; rep then holds
+ * nconv and the achieved max_rel_residual. */
 int wtb_bse_solve(wtb_ctx *ctx, int nev, double tol, int max_it, double *E, double *X,
                   wtb_report *rep);
 
