@@ -450,12 +450,16 @@ tridiag_reg_kernel(int n, cd *__restrict__ Ag, int lda, cd *__restrict__ tau, do
             const cd voc = vo[c], woc = wo[c], vnc = vme[c];
             cd cacc = make_double2(0.0, 0.0), Ad = make_double2(0.0, 0.0);
             const int qq = q + 16 * (b & 1), rd = 32 * (b >> 1) + p;
+            // the diagonal-block operands are read by every lane (clamped to
+            // a valid element) ahead of the lanes' branch
+            const int di = (b >> 1) * NDB + prow + min(qq, p);
+            Ad = D[di];
+            const cd vod = vo[rd], wod = wo[rd], vmd = vme[rd];
             if (qq <= p) {
-                Ad = D[(b >> 1) * NDB + prow + qq];
-                rank2_sub(Ad, vo[rd], wo[rd], voc, woc);
-                D[(b >> 1) * NDB + prow + qq] = Ad;
+                rank2_sub(Ad, vod, wod, voc, woc);
+                D[di] = Ad;
                 cmac(racc[b >> 1], Ad, vnc);
-                if (qq < p) cmac_conj(cacc, Ad, vme[rd]);
+                if (qq < p) cmac_conj(cacc, Ad, vmd);
             }
 #pragma unroll
             for (int a = b / 2 + 1; a < NA; ++a) {
