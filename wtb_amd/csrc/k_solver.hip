@@ -1018,9 +1018,13 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             }
             double worst = 0;
             for (int j = 0; j < nev; ++j) worst = std::max(worst, rel[j]);
-            if (verbose)
+            if (verbose) {
                 std::fprintf(stderr, "[wtb lobpcg] prec=%d it=%d m_active=%zu nconv=%d/%d max_rel_res=%.3e lam0=%.9f\n",
                              mvp, it, active.size(), nconv, nev, worst, L.h_lam[0]);
+                std::fprintf(stderr, "[wtb lobpcg] active:");
+                for (int j : active) std::fprintf(stderr, " %d", j);
+                std::fprintf(stderr, "\n");
+            }
             if (nconv == nev) break;
             // stagnation guard for products with a residual floor: stop when
             // three iterations near it gain less than 10 %
