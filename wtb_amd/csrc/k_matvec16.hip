@@ -14,15 +14,17 @@
 // 64 contiguous bytes per load instruction), i.e. the A operand of output rows
 // i is conj(H[k + i ld]).  X arrives
 // pre-split in FP16 planes (re_hi, re_lo, im_hi, im_lo), column-major with k
-// contiguous.  Each iteration covers KT k: the next H slice is loaded into
-// registers and the next X slice into a second LDS buffer while the MFMAs of
-// the current slice run, with one barrier per iteration.  Loads carry no
-// bounds checks (the store's rows N..ld-1 are zero, indexes are clamped), so
-// the compiler keeps the whole prefetch in flight across the MFMA phase.
+// contiguous.  Each step covers 32 k: H slices are loaded into registers two
+// steps ahead of their use and X slices two steps ahead, staged into a second
+// LDS buffer at the end of the step, with one barrier per step.  Loads carry
+// no bounds checks (the store's rows N..ld-1 are zero, indexes are clamped),
+// so the compiler keeps the whole prefetch in flight across the MFMA phase.
 #include "wtb_internal.h"
 
 #include <cmath>
 #include <mutex>
+#include <type_traits>
+#include <utility>
 
 namespace wtb {
 namespace {
@@ -30,11 +32,15 @@ namespace {
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
-constexpr int kKT = 64;        // k per iteration of the split-FP16 kernel
+constexpr int kKT = 32;        // k per step of the split-FP16 kernel (one MFMA depth)
 // wavefronts per workgroup (16 rows each): 8 puts two wavefronts on each SIMD
 // (a 256-register budget per wavefront), which keeps twice the H stream in
 // flight of a 4-wavefront workgroup
 constexpr int kWPB = 8;
+// H register sets of the split-FP16 kernel: slices are loaded kHD - 1 steps
+// ahead of their use (three sets fit the 256-register budget of two
+// wavefronts per SIMD up to 128 columns)
+constexpr int kHD = 3;
 
 __device__ __forceinline__ f4 mfma16(h8 a, h8 b, f4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
@@ -110,45 +116,78 @@ col_scales_kernel(long N, const double *__restrict__ X, float hscale, float *__r
     }
 }
 
+// 8 complex FP32 values of a stored column -> their split FP16 halves, real
+// and imaginary parts in separate operands (element e of an operand is the
+// e-th value).  Per pair of floats (a0, a1) and power-of-two scale s:
+//     hi = (f16(s a0), f16(s a1)),  lo = (f16(s a0 - hi.x), f16(s a1 - hi.y))
+// by four v_fma_mix{lo,hi}_f16 (FP32 sources, the scale applied inside the
+// fused operation, hi read back as FP16 through op_sel), so the split costs
+// two VALU operations per float and no packing.
+typedef unsigned u4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void split_pair(float a0, float a1, float s, unsigned &hi, unsigned &lo) {
+    asm("v_fma_mixlo_f16 %0, %2, %4, 0\n\t"
+        "v_fma_mixhi_f16 %0, %3, %4, 0\n\t"
+        "v_fma_mixlo_f16 %1, %2, %4, -%0 op_sel_hi:[0,0,1]\n\t"
+        "v_fma_mixhi_f16 %1, %3, %4, -%0 op_sel:[0,0,1] op_sel_hi:[0,0,1]"
+        : "=&v"(hi), "=&v"(lo)
+        : "v"(a0), "v"(a1), "v"(s));
+}
+
 struct Split {
-    h8 rh, rl, ih, il, nih, nil;
+    h8 rh, rl, ih, il;
 };
 
-// 8 complex FP32 values (conjugated row of H) -> split FP16 operands
 __device__ __forceinline__ Split split8(const float4 *h, float hinv) {
-    Split s;
+    u4 rh, rl, ih, il;
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
-        const float v[4] = {h[q].x, h[q].y, h[q].z, h[q].w};
-#pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const int e = 2 * q + t;
-            const float re = v[2 * t] * hinv, im = -v[2 * t + 1] * hinv;   // conj
-            const _Float16 a = (_Float16)re, b = (_Float16)im;
-            s.rh[e] = a;
-            s.rl[e] = (_Float16)(re - (float)a);
-            s.ih[e] = b;
-            s.il[e] = (_Float16)(im - (float)b);
-        }
+        unsigned a, b;
+        split_pair(h[q].x, h[q].z, hinv, a, b);
+        rh[q] = a;
+        rl[q] = b;
+        split_pair(h[q].y, h[q].w, hinv, a, b);
+        ih[q] = a;
+        il[q] = b;
     }
-    s.nih = -s.ih;
-    s.nil = -s.il;
-    return s;
+    return Split{__builtin_bit_cast(h8, rh), __builtin_bit_cast(h8, rl), __builtin_bit_cast(h8, ih),
+                 __builtin_bit_cast(h8, il)};
+}
+
+// sign flip of the eight FP16 values of an operand
+__device__ __forceinline__ h8 neg8(h8 v) { return __builtin_bit_cast(h8, __builtin_bit_cast(u4, v) ^ 0x80008000u); }
+
+template <int V>
+using IC = std::integral_constant<int, V>;
+template <class F, int... J>
+__device__ __forceinline__ void run_steps(F &&f, long it, std::integer_sequence<int, J...>) {
+    (f(it + J, IC<J>{}), ...);
+}
+template <class F, int... J>
+__device__ __forceinline__ void run_tail(F &&f, long it, long niter, std::integer_sequence<int, J...>) {
+    ((it + J < niter ? (f(it + J, IC<J>{}), 0) : 0), ...);
 }
 
 // WPB wavefronts per workgroup, each owning 16 output rows; the workgroup's X
-// slice is staged once in LDS for all of them.
-template <int NCB, int KT, int WPB>
+// slice is staged once in LDS for all of them.  A step covers kKT = 32 k (one
+// MFMA depth).  The H slice of step it + HD - 1 and the X slice of step it + 2
+// are loaded into registers while step it computes (HD register sets for H,
+// two for X), and the X slice of step it + 1 goes to the second LDS buffer
+// at the end of the step, one barrier per step.
+// The row element of H is conj(s) for the stored s = sr + i si, so
+//     Re y = sr xr + si xi,   Im y = sr xi - si xr.
+template <int NCB, int WPB, int HD>
 __global__ void __launch_bounds__(64 * WPB)
 hx_f16split_kernel(long N, long ld, int ncol, float hinv, const float2 *__restrict__ H,
                    const _Float16 *__restrict__ P, const float *__restrict__ ysc, double *__restrict__ Y) {
+    constexpr int KT = kKT;
     constexpr int NC = NCB * 16;
-    constexpr int NCH = KT / 32;                 // MFMA depths per iteration
     constexpr int XS = KT + 8;                   // halfs per staged column (pad against bank conflicts)
     constexpr int QN = KT / 8;                   // 16-byte chunks per column and plane
     constexpr int NT = 64 * WPB;                 // threads per workgroup
-    constexpr int XCH = 4 * NC * QN / NT;        // 16-byte X chunks per thread per iteration
-    static_assert(4 * NC * QN % NT == 0, "X slice must split evenly over the workgroup");
+    constexpr int XTOT = 4 * NC * QN;            // 16-byte X chunks per slice
+    constexpr int XCH = (XTOT + NT - 1) / NT;    // per thread
+    constexpr int U = HD == 3 ? 6 : 2;           // steps per unrolled round (register and LDS sets cycle)
     __shared__ __align__(16) _Float16 xs[2][4][NC][XS];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int blk = lane >> 4, li = lane & 15;
@@ -166,25 +205,27 @@ hx_f16split_kernel(long N, long ld, int ncol, float hinv, const float2 *__restri
 #pragma unroll
         for (int t = 0; t < XCH; ++t) {
             const int c = tid + NT * t;
-            const int q = c % QN, col = (c / QN) % NC, p = c / (QN * NC);
-            r[t] = *reinterpret_cast<const h8 *>(P + p * plane + (size_t)min(col, ncol - 1) * ld + k0 + 8 * q);
+            if (XTOT % NT == 0 || c < XTOT) {
+                const int q = c % QN, col = (c / QN) % NC, p = c / (QN * NC);
+                r[t] = *reinterpret_cast<const h8 *>(P + p * plane + (size_t)min(col, ncol - 1) * ld + k0 + 8 * q);
+            }
         }
     };
     auto store_x = [&](int buf, const h8 (&r)[XCH]) {
 #pragma unroll
         for (int t = 0; t < XCH; ++t) {
             const int c = tid + NT * t;
-            const int q = c % QN, col = (c / QN) % NC, p = c / (QN * NC);
-            *reinterpret_cast<h8 *>(&xs[buf][p][col][8 * q]) = r[t];
+            if (XTOT % NT == 0 || c < XTOT) {
+                const int q = c % QN, col = (c / QN) % NC, p = c / (QN * NC);
+                *reinterpret_cast<h8 *>(&xs[buf][p][col][8 * q]) = r[t];
+            }
         }
     };
-    // lane's A slice of depth ch: k = k0 + 32 ch + 8 q + 2 blk + t (see kperm)
-    auto load_h = [&](long it, float4 (&h)[4 * NCH]) {
+    // lane's A slice: k = k0 + 8 q + 2 blk + t (see kperm)
+    auto load_h = [&](long it, float4 (&h)[4]) {
         const float4 *src = reinterpret_cast<const float4 *>(hcol + min(it, niter - 1) * KT);
 #pragma unroll
-        for (int ch = 0; ch < NCH; ++ch)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) h[4 * ch + q] = src[16 * ch + 4 * q];
+        for (int q = 0; q < 4; ++q) h[q] = src[4 * q];
     };
 
     f4 yr[NCB], yi[NCB];
@@ -193,64 +234,66 @@ hx_f16split_kernel(long N, long ld, int ncol, float hinv, const float2 *__restri
         yr[cb] = f4{0.f, 0.f, 0.f, 0.f};
         yi[cb] = f4{0.f, 0.f, 0.f, 0.f};
     }
-    auto compute = [&](int buf, const float4 (&hc)[4 * NCH]) {
+    auto compute = [&](int buf, const float4 (&hc)[4]) {
+        const int kk = 8 * blk;
+        // every B fragment of the step is requested before the first MFMA, so
+        // the LDS latency overlaps the earlier column blocks
+        h8 bf[NCB][4];
 #pragma unroll
-        for (int ch = 0; ch < NCH; ++ch) {
-            const Split a = split8(hc + 4 * ch, hinv);
-            const int kk = 32 * ch + 8 * blk;
-            // every B fragment of this depth is requested before the first
-            // MFMA, so the LDS latency overlaps the earlier column blocks
-            h8 bf[NCB][4];
+        for (int cb = 0; cb < NCB; ++cb)
 #pragma unroll
-            for (int cb = 0; cb < NCB; ++cb)
+            for (int p = 0; p < 4; ++p) bf[cb][p] = *reinterpret_cast<const h8 *>(&xs[buf][p][cb * 16 + li][kk]);
+        const Split a = split8(hc, hinv);
+        const h8 nih = neg8(a.ih), nil = neg8(a.il);
 #pragma unroll
-                for (int p = 0; p < 4; ++p) bf[cb][p] = *reinterpret_cast<const h8 *>(&xs[buf][p][cb * 16 + li][kk]);
-#pragma unroll
-            for (int cb = 0; cb < NCB; ++cb) {
-                const h8 xrh = bf[cb][0], xrl = bf[cb][1], xih = bf[cb][2], xil = bf[cb][3];
-                // Yr += Ar Xr - Ai Xi,  Yi += Ar Xi + Ai Xr  (independent chains interleaved)
-                yr[cb] = mfma16(a.rh, xrh, yr[cb]);
-                yi[cb] = mfma16(a.rh, xih, yi[cb]);
-                yr[cb] = mfma16(a.rh, xrl, yr[cb]);
-                yi[cb] = mfma16(a.rh, xil, yi[cb]);
-                yr[cb] = mfma16(a.rl, xrh, yr[cb]);
-                yi[cb] = mfma16(a.rl, xih, yi[cb]);
-                yr[cb] = mfma16(a.nih, xih, yr[cb]);
-                yi[cb] = mfma16(a.ih, xrh, yi[cb]);
-                yr[cb] = mfma16(a.nih, xil, yr[cb]);
-                yi[cb] = mfma16(a.ih, xrl, yi[cb]);
-                yr[cb] = mfma16(a.nil, xih, yr[cb]);
-                yi[cb] = mfma16(a.il, xrh, yi[cb]);
-            }
+        for (int cb = 0; cb < NCB; ++cb) {
+            const h8 xrh = bf[cb][0], xrl = bf[cb][1], xih = bf[cb][2], xil = bf[cb][3];
+            // three split terms of each real product, two independent chains interleaved
+            yr[cb] = mfma16(a.rh, xrh, yr[cb]);
+            yi[cb] = mfma16(a.rh, xih, yi[cb]);
+            yr[cb] = mfma16(a.rh, xrl, yr[cb]);
+            yi[cb] = mfma16(a.rh, xil, yi[cb]);
+            yr[cb] = mfma16(a.rl, xrh, yr[cb]);
+            yi[cb] = mfma16(a.rl, xih, yi[cb]);
+            yr[cb] = mfma16(a.ih, xih, yr[cb]);
+            yi[cb] = mfma16(nih, xrh, yi[cb]);
+            yr[cb] = mfma16(a.ih, xil, yr[cb]);
+            yi[cb] = mfma16(nih, xrl, yi[cb]);
+            yr[cb] = mfma16(a.il, xih, yr[cb]);
+            yi[cb] = mfma16(nil, xrh, yi[cb]);
         }
     };
 
-    h8 xr[XCH];
-    float4 ha[4 * NCH], hb[4 * NCH];
-    load_x(0, xr);
-    load_h(0, ha);
-    store_x(0, xr);
+    h8 xr[2][XCH];
+    float4 hb[HD][4];
+    {
+        h8 x0[XCH];
+        load_x(0, x0);
+        load_x(1, xr[1]);
+#pragma unroll
+        for (int d = 0; d < HD - 1; ++d) load_h(d, hb[d]);
+        store_x(0, x0);
+    }
     __syncthreads();
-    // iteration it computes from LDS buffer it & 1 and H registers ha / hb
-    // alternately while the next slice is in flight (the prefetch past the
-    // last slice re-reads it and is never used)
+    // step it, j = it mod U: loads X(it + 2) into xr[j % 2] and H(it + HD - 1)
+    // into hb[(j + HD - 1) % HD], computes from LDS buffer j % 2 and hb[j % HD],
+    // then stages X(it + 1) (xr[(j + 1) % 2]) into LDS buffer (j + 1) % 2.
+    // Prefetches past the last slice re-read it and are never used.
     // (the scheduling fences keep the compiler from sinking the prefetch
     // loads down to their first use, which would expose their latency)
-    auto step = [&](long it, int buf, const float4 (&hc)[4 * NCH], float4 (&hn)[4 * NCH]) {
-        load_x(it + 1, xr);
-        load_h(it + 1, hn);
+    auto step = [&](long it, auto J) {
+        constexpr int j = decltype(J)::value;
+        load_x(it + 2, xr[j % 2]);
+        load_h(it + HD - 1, hb[(j + HD - 1) % HD]);
         __builtin_amdgcn_sched_barrier(0);
-        compute(buf, hc);
+        compute(j % 2, hb[j % HD]);
         __builtin_amdgcn_sched_barrier(0);
-        store_x(buf ^ 1, xr);
+        store_x((j + 1) % 2, xr[(j + 1) % 2]);
         __syncthreads();
     };
     long it = 0;
-    for (; it + 2 <= niter; it += 2) {
-        step(it, 0, ha, hb);
-        step(it + 1, 1, hb, ha);
-    }
-    if (it < niter) step(it, 0, ha, hb);
+    for (; it + U <= niter; it += U) run_steps(step, it, std::make_integer_sequence<int, U>{});
+    run_tail(step, it, niter, std::make_integer_sequence<int, U - 1>{});
 
     // D layout (16x16 f32): register i of lane l -> row 4*(l/16)+i, column l%16
     const long r0 = (long)blockIdx.x * (16 * WPB) + wave * 16 + 4 * blk;
@@ -270,15 +313,15 @@ hx_f16split_kernel(long N, long ld, int ncol, float hinv, const float2 *__restri
     }
 }
 
-template <int KT, int WPB>
+template <int WPB>
 void launch_f16split(hipStream_t st, int ncb, long N, long ld, int nc, float hinv, const cf *H,
                      const _Float16 *P, const float *ysc, double *y) {
     const dim3 grid((unsigned)((N + 16 * WPB - 1) / (16 * WPB)));
     switch (ncb) {
-#define WTB_F16_CASE(K)                                                                                        \
-    case K:                                                                                                    \
-        hipLaunchKernelGGL((hx_f16split_kernel<K, KT, WPB>), grid, dim3(64 * WPB), 0, st, N, ld, nc, hinv, H, P, \
-                           ysc, y);                                                                            \
+#define WTB_F16_CASE(K)                                                                                         \
+    case K:                                                                                                     \
+        hipLaunchKernelGGL((hx_f16split_kernel<K, WPB, kHD>), grid, dim3(64 * WPB), 0, st, N, ld, nc, hinv, H, \
+                           P, ysc, y);                                                                          \
         break;
         WTB_F16_CASE(1) WTB_F16_CASE(2) WTB_F16_CASE(3) WTB_F16_CASE(4)
         WTB_F16_CASE(5) WTB_F16_CASE(6) WTB_F16_CASE(7) WTB_F16_CASE(8)
@@ -308,7 +351,7 @@ void f16split_blocks(Ctx &c, int ncol, const cd *X, cd *Y, int blk, _Float16 *pl
         WTB_LAUNCH_CHECK();
         const int ncb = (nc + 15) / 16;
         c.kclock.begin(KernelClock::F16SPLIT, c.stream);
-        launch_f16split<kKT, kWPB>(c.stream, ncb, N, ld, nc, 1.0f / s.hscale, s.h32.p, planes, scales + nc, y);
+        launch_f16split<kWPB>(c.stream, ncb, N, ld, nc, 1.0f / s.hscale, s.h32.p, planes, scales + nc, y);
         c.kclock.end(KernelClock::F16SPLIT, c.stream, 8.0 * (double)ld * N, 8.0 * (double)N * N * nc);
         WTB_LAUNCH_CHECK();
     }
