@@ -146,7 +146,7 @@ __global__ void split_x8_kernel(long N, long ld, int ncol, const double *__restr
 
 // RT row tiles of 16 per workgroup; WPR wavefronts share a row tile, each
 // owning CPW blocks of 16 columns.  gridDim.y: chunks of kc_len values of k.
-// One wavefront per SIMD (the 80 KB LDS tile allows one workgroup per CU):
+// One wavefront per SIMD (the 96 KB LDS tile allows one workgroup per CU):
 // the MFMA accumulators take the AGPRs, the rest of the 512 registers hold
 // the digit operands of two steps and the H prefetch.
 // Yp: [chunk][col][N] complex FP64 partial products.
@@ -157,7 +157,10 @@ hx_int8_kernel(long N, long ld, int ncol, long kc_len, float hmul, const float2 
     constexpr int NCB = CPW * WPR;
     constexpr int NC = 16 * NCB;
     constexpr int NT = 64 * WPR * RT;
-    constexpr int XS = 64 + 16;                     // bytes per staged (plane, column), padded
+    // bytes per staged (plane, column): the 96-byte row stride keeps the
+    // B-fragment reads (16 lanes x 16 bytes at rows li, chunk blk) on
+    // distinct LDS banks
+    constexpr int XS = 64 + 32;
     constexpr int XCH = kPl8 * NC * 4 / NT;         // 16-byte X chunks per thread per step
     static_assert(kPl8 * NC * 4 % NT == 0, "X slice must split evenly over the workgroup");
     __shared__ __align__(16) int8_t xs[2][kPl8][NC][XS];
