@@ -317,7 +317,6 @@ __global__ void svqb_t(int m, int r, int off, const cd *__restrict__ U, int ldu,
 }
 
 const rocblas_double_complex kOne = {1.0, 0.0};
-const rocblas_double_complex kZero = {0.0, 0.0};
 
 inline const rocblas_double_complex *rb(const cd *p) { return reinterpret_cast<const rocblas_double_complex *>(p); }
 inline rocblas_double_complex *rb(cd *p) { return reinterpret_cast<rocblas_double_complex *>(p); }
@@ -658,17 +657,148 @@ void trsm_right_upper(Ctx &c, long N, int nb, const cd *R, int ldr, cd *B, cd *B
 }
 
 // C (M x n) = A (M x k) B (k x n)
-void gemm_nn(Ctx &c, long M, int n, int k, const cd *A, long lda, const cd *B, int ldb, cd *C, long ldc) {
-    WTB_BLAS_CHECK(rocblas_zgemm(c.blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, n, k,
-                                 &kOne, rb(A), (rocblas_int)lda, rb(B), ldb, &kZero, rb(C), (rocblas_int)ldc));
+// ---------------------------------------------------------------------------
+// Tall products C_s = A_s B (or C_s -= A_s B) for s = 0, 1 with one small
+// coefficient block B (k x n, n <= 80) shared by both: the Ritz and
+// projection updates of the block solver, where the pair (X, H X) takes the
+// same combination.  FP64 MFMA (v_mfma_f64_16x16x4f64, layout as in the Gram
+// kernel above).  A workgroup of four wavefronts owns 64 rows; each wavefront
+// forms its 16 rows for all 16-column tiles of C.  B is staged in LDS 16 rows
+// of k at a time (the next 16 rows and the next A fragments are in flight in
+// registers during the MFMAs); A fragments come straight from global memory.
+// The MFMAs of one k-step are issued phase by phase over the tiles (Re Re,
+// Re Im, -Im Im, Im Re), so consecutive MFMAs never share an accumulator.
+// gridDim.y = number of products.  A and C must not overlap.
+constexpr int kTgKB = 16;
+
+template <int NCB>
+__global__ void __launch_bounds__(256, 2)
+tall_gemm_kernel(long M, int k, int n, const cd *__restrict__ A0, long lda, cd *C0, const cd *__restrict__ A1,
+                 cd *C1, long ldc, const cd *__restrict__ B, int ldb, int sub) {
+    constexpr int KB = kTgKB, NC = 16 * NCB, BE = (KB * NC + 255) / 256;
+    __shared__ cd sb[2][KB][NC + 1];
+    const cd *A = blockIdx.y ? A1 : A0;
+    cd *C = blockIdx.y ? C1 : C0;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const int li = lane & 15, lk = lane >> 4;
+    const long r0 = (long)blockIdx.x * 64 + 16 * w;
+    const long ra = min(r0 + li, M - 1);      // rows past M read the last one (never written)
+    d4 cre[NCB], cim[NCB];
+#pragma unroll
+    for (int t = 0; t < NCB; ++t) {
+        cre[t] = d4{0.0, 0.0, 0.0, 0.0};
+        cim[t] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+    const int nch = (k + KB - 1) / KB;
+    cd af[KB / 4], an[KB / 4], bq[BE];
+    auto load_a = [&](int ch, cd (&f)[KB / 4]) {
+#pragma unroll
+        for (int s = 0; s < KB / 4; ++s) {
+            const int kk = ch * KB + 4 * s + lk;
+            f[s] = kk < k ? A[ra + (size_t)kk * lda] : make_double2(0.0, 0.0);
+        }
+    };
+    auto load_b = [&](int ch) {
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            const int idx = tid + 256 * e, kq = idx % KB, j = idx / KB, kk = ch * KB + kq;
+            bq[e] = (idx < KB * NC && kk < k && j < n) ? B[kk + (size_t)j * ldb] : make_double2(0.0, 0.0);
+        }
+    };
+    auto store_b = [&](int buf) {
+#pragma unroll
+        for (int e = 0; e < BE; ++e) {
+            const int idx = tid + 256 * e;
+            if (idx < KB * NC) sb[buf][idx % KB][idx / KB] = bq[e];
+        }
+    };
+    load_a(0, af);
+    load_b(0);
+    store_b(0);
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+        const int buf = ch & 1;
+        if (ch + 1 < nch) {
+            load_a(ch + 1, an);
+            load_b(ch + 1);
+        }
+#pragma unroll
+        for (int s = 0; s < KB / 4; ++s) {
+            cd b[NCB];
+#pragma unroll
+            for (int t = 0; t < NCB; ++t) b[t] = sb[buf][4 * s + lk][16 * t + li];
+            const double ar = af[s].x, ai = af[s].y;
+#pragma unroll
+            for (int t = 0; t < NCB; ++t) cre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b[t].x, cre[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NCB; ++t) cim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b[t].y, cim[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NCB; ++t) cre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, b[t].y, cre[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NCB; ++t) cim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b[t].x, cim[t], 0, 0, 0);
+        }
+        if (ch + 1 < nch) {
+            store_b(buf ^ 1);
+#pragma unroll
+            for (int s = 0; s < KB / 4; ++s) af[s] = an[s];
+        }
+        __syncthreads();
+    }
+    // result register q of lane l: row 4 q + l / 16, column l % 16 of the tile
+#pragma unroll
+    for (int t = 0; t < NCB; ++t) {
+        const int j = 16 * t + li;
+        if (j >= n) continue;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const long r = r0 + 4 * q + lk;
+            if (r >= M) continue;
+            cd *cp = C + r + (size_t)j * ldc;
+            if (sub) {
+                const cd v = *cp;
+                *cp = make_double2(v.x - cre[t][q], v.y - cim[t][q]);
+            } else {
+                *cp = make_double2(cre[t][q], cim[t][q]);
+            }
+        }
+    }
 }
 
-const rocblas_double_complex kMinusOne = {-1.0, 0.0};
+// C0 (=|-=) A0 B and, when A1 is given, C1 (=|-=) A1 B: M x n outputs, A_s
+// M x k (lda), C_s (ldc), B k x n (ldb).  Tall products with n <= 80 run on
+// tall_gemm_kernel; wider or short ones on rocBLAS ZGEMM.
+void gemm_pair(Ctx &c, long M, int n, int k, const cd *A0, const cd *A1, long lda, const cd *B, int ldb, cd *C0,
+               cd *C1, long ldc, bool sub) {
+    if (M <= 0 || n <= 0) return;
+    if (n <= 80 && M >= 1024) {
+        const dim3 g((unsigned)((M + 63) / 64), A1 ? 2 : 1);
+        switch ((n + 15) / 16) {
+#define WTB_TG_CASE(K)                                                                                          \
+    case K:                                                                                                     \
+        hipLaunchKernelGGL(tall_gemm_kernel<K>, g, dim3(256), 0, c.stream, M, k, n, A0, lda, C0, A1, C1, ldc, B, \
+                           ldb, sub ? 1 : 0);                                                                   \
+        break;
+            WTB_TG_CASE(1) WTB_TG_CASE(2) WTB_TG_CASE(3) WTB_TG_CASE(4) WTB_TG_CASE(5)
+#undef WTB_TG_CASE
+        }
+        WTB_LAUNCH_CHECK();
+        return;
+    }
+    const rocblas_double_complex one = {1.0, 0.0}, mone = {-1.0, 0.0}, zero = {0.0, 0.0};
+    for (int s = 0; s < (A1 ? 2 : 1); ++s)
+        WTB_BLAS_CHECK(rocblas_zgemm(c.blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, n, k,
+                                     sub ? &mone : &one, rb(s ? A1 : A0), (rocblas_int)lda, rb(B), ldb,
+                                     sub ? &one : &zero, rb(s ? C1 : C0), (rocblas_int)ldc));
+}
+
+// C (M x n) = A (M x k) B (k x n)
+void gemm_nn(Ctx &c, long M, int n, int k, const cd *A, long lda, const cd *B, int ldb, cd *C, long ldc) {
+    gemm_pair(c, M, n, k, A, nullptr, lda, B, ldb, C, nullptr, ldc, false);
+}
 
 // C (M x n) -= A (M x k) B (k x n)
 void gemm_nn_sub(Ctx &c, long M, int n, int k, const cd *A, long lda, const cd *B, int ldb, cd *C, long ldc) {
-    WTB_BLAS_CHECK(rocblas_zgemm(c.blas, rocblas_operation_none, rocblas_operation_none, (rocblas_int)M, n, k,
-                                 &kMinusOne, rb(A), (rocblas_int)lda, rb(B), ldb, &kOne, rb(C), (rocblas_int)ldc));
+    gemm_pair(c, M, n, k, A, nullptr, lda, B, ldb, C, nullptr, ldc, true);
 }
 
 }  // namespace
@@ -716,7 +846,7 @@ struct Lobpcg {
     Ctx &c;
     long N;
     int b, nev, m3;
-    DBuf<cd> &S, &AS, &T1, &P, &AP;      // N x cols
+    DBuf<cd> &S, &AS, &T1, &T2, &P, &AP;   // N x cols
     DBuf<cd> &G, &A, &U, &T, &Tmp, &Cm;  // small m3 x m3
     DBuf<double> &D, &lam, &lamr, &norm2, &sc;
     DBuf<int> &idx, &act;
@@ -724,7 +854,7 @@ struct Lobpcg {
     std::vector<double> h_lam;
 
     Lobpcg(Ctx &cx, int nev_, int b_)
-        : c(cx), N(cx.bse.N), b(b_), nev(nev_), m3(3 * b_), S(cx.lws.S), AS(cx.lws.AS), T1(cx.lws.T1),
+        : c(cx), N(cx.bse.N), b(b_), nev(nev_), m3(3 * b_), S(cx.lws.S), AS(cx.lws.AS), T1(cx.lws.T1), T2(cx.lws.T2),
           P(cx.lws.P), AP(cx.lws.AP), G(cx.lws.G), A(cx.lws.A), U(cx.lws.U), T(cx.lws.T), Tmp(cx.lws.Tmp),
           Cm(cx.lws.Cm), D(cx.lws.D), lam(cx.lws.lam), lamr(cx.lws.lamr), norm2(cx.lws.norm2), sc(cx.lws.sc),
           idx(cx.lws.idx), act(cx.lws.act) {
@@ -734,7 +864,8 @@ struct Lobpcg {
         trr.st = c.stream;
         S.alloc((size_t)N * m3);
         AS.alloc((size_t)N * m3);
-        T1.alloc((size_t)N * m3);       // trades places with S and AS
+        T1.alloc((size_t)N * m3);       // trade places with S and AS
+        T2.alloc((size_t)N * m3);
         P.alloc((size_t)N * b);
         AP.alloc((size_t)N * b);
         G.alloc((size_t)m3 * m3);
@@ -781,8 +912,7 @@ struct Lobpcg {
     void project_out(int q, const cd *Q, const cd *Q2, int nb, cd *B, cd *B2) {
         trr.begin();
         gram(c, q, nb, N, Q, (int)N, B, (int)N, Tmp.p, m3);
-        gemm_nn_sub(c, N, nb, q, Q, N, Tmp.p, m3, B, N);
-        if (Q2) gemm_nn_sub(c, N, nb, q, Q2, N, Tmp.p, m3, B2, N);
+        gemm_pair(c, N, nb, q, Q, Q2, N, Tmp.p, m3, B, B2, N, true);
         trr.end();
     }
 
@@ -859,19 +989,20 @@ struct Lobpcg {
         WTB_HIP_CHECK(hipMemcpyAsync(lam.p, lamr.p, sizeof(double) * nb, hipMemcpyDeviceToDevice, c.stream));
         // new X, AX (and P, AP)
         // P and H P come straight out of the products (they read only S and AS);
-        // the new X (H X) is formed in T1, which then trades places with S
-        // (AS): the rest of S is rebuilt every iteration.  A partial update
+        // the new X and H X are formed in T1 and T2, which then trade places
+        // with S and AS: the rest of S is rebuilt every iteration.  A partial update
         // (nb < b) copies its columns back instead.
-        if (withP && m > b) {
-            gemm_nn(c, N, b, m - b, S.p + (size_t)b * N, N, Cm.p + b, m3, P.p, N);
-            gemm_nn(c, N, b, m - b, AS.p + (size_t)b * N, N, Cm.p + b, m3, AP.p, N);
+        if (withP && m > b)
+            gemm_pair(c, N, b, m - b, S.p + (size_t)b * N, AS.p + (size_t)b * N, N, Cm.p + b, m3, P.p, AP.p, N,
+                      false);
+        gemm_pair(c, N, nb, m, S.p, AS.p, N, Cm.p, m3, T1.p, T2.p, N, false);
+        if (nb == b) {
+            swap_bufs(S, T1);
+            swap_bufs(AS, T2);
+        } else {
+            WTB_HIP_CHECK(hipMemcpyAsync(S.p, T1.p, sizeof(cd) * N * nb, hipMemcpyDeviceToDevice, c.stream));
+            WTB_HIP_CHECK(hipMemcpyAsync(AS.p, T2.p, sizeof(cd) * N * nb, hipMemcpyDeviceToDevice, c.stream));
         }
-        gemm_nn(c, N, nb, m, S.p, N, Cm.p, m3, T1.p, N);
-        if (nb == b) swap_bufs(S, T1);
-        else WTB_HIP_CHECK(hipMemcpyAsync(S.p, T1.p, sizeof(cd) * N * nb, hipMemcpyDeviceToDevice, c.stream));
-        gemm_nn(c, N, nb, m, AS.p, N, Cm.p, m3, T1.p, N);
-        if (nb == b) swap_bufs(AS, T1);
-        else WTB_HIP_CHECK(hipMemcpyAsync(AS.p, T1.p, sizeof(cd) * N * nb, hipMemcpyDeviceToDevice, c.stream));
         trr.end();
     }
 };

@@ -103,7 +103,7 @@ inline void swap_bufs(DBuf<T> &a, DBuf<T> &b) {
 }
 
 struct SolverWs {
-    DBuf<cd> S, AS, T1, P, AP;      // N x cols
+    DBuf<cd> S, AS, T1, T2, P, AP;  // N x cols
     DBuf<cd> G, A, U, T, Tmp, Cm;   // small (3b) x (3b)
     DBuf<double> D, lam, lamr, norm2, sc;
     DBuf<int> idx, act;
