@@ -334,12 +334,12 @@ inline rocblas_double_complex *rb(cd *p) { return reinterpret_cast<rocblas_doubl
 typedef double d4 __attribute__((ext_vector_type(4)));
 
 // HERM (m == n, square blocks): only the upper triangle of C is wanted.
-// Blocks below the diagonal return at once and a diagonal block forms only
+// Only the upper blocks are launched, and a diagonal block forms only
 // its tiles with ti <= tj (15 of 25 at 80 x 80), dealt to the wavefronts in
 // that order; reduce_chunks then mirrors the upper triangle.
 template <int MT, int NT, bool HERM>
 __global__ void __launch_bounds__(256)
-gram_block_kernel(int m, int n, long K, long kc, const cd *__restrict__ A, int lda, const cd *__restrict__ B,
+gram_block_kernel(int m, int n, int bx, long K, long kc, const cd *__restrict__ A, int lda, const cd *__restrict__ B,
                   int ldb, cd *__restrict__ part) {
     constexpr int KS = 16, TI = MT / 16, TJ = NT / 16, NTILE = TI * TJ, TPW = (NTILE + 3) / 4;
     constexpr int AE = KS * MT / 256, BE = KS * NT / 256;       // staged elements per thread
@@ -347,9 +347,21 @@ gram_block_kernel(int m, int n, long K, long kc, const cd *__restrict__ A, int l
     static_assert(!HERM || MT == NT, "Hermitian blocks are square");
     __shared__ cd sa[KS][MT + 1], sb[KS][NT + 1];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    if (HERM && blockIdx.x > blockIdx.y) return;
-    const int i0 = blockIdx.x * MT, j0 = blockIdx.y * NT;
-    const bool tri = HERM && blockIdx.x == blockIdx.y;
+    // output block (bi, bj) of blockIdx.x: column-major over the bx x by
+    // blocks, or over the upper ones (bi <= bj) of a Hermitian product
+    int bi, bj;
+    if (HERM) {
+        const int t = blockIdx.x;
+        bj = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
+        while ((bj + 1) * (bj + 2) / 2 <= t) ++bj;
+        while (bj * (bj + 1) / 2 > t) --bj;
+        bi = t - bj * (bj + 1) / 2;
+    } else {
+        bi = blockIdx.x % bx;
+        bj = blockIdx.x / bx;
+    }
+    const int i0 = bi * MT, j0 = bj * NT;
+    const bool tri = HERM && bi == bj;
     const int ntile = tri ? TI * (TI + 1) / 2 : NTILE;
     // tile t of the wavefront -> (ti, tj): column-major over all tiles, or
     // over the upper ones (ti <= tj) of a diagonal block
@@ -367,7 +379,7 @@ gram_block_kernel(int m, int n, long K, long kc, const cd *__restrict__ A, int l
             tjs[t] = tile / TI;
         }
     }
-    const long k0 = (long)blockIdx.z * kc, k1 = min(K, k0 + kc);
+    const long k0 = (long)blockIdx.y * kc, k1 = min(K, k0 + kc);
     d4 cre[TPW], cim[TPW];
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
@@ -423,7 +435,7 @@ gram_block_kernel(int m, int n, long K, long kc, const cd *__restrict__ A, int l
             }
         }
     }
-    cd *p = part + (size_t)blockIdx.z * m * n;
+    cd *p = part + (size_t)blockIdx.y * m * n;
 #pragma unroll
     for (int t = 0; t < TPW; ++t) {
         const int tile = w + 4 * t;
@@ -500,21 +512,24 @@ void gram(Ctx &c, int m, int n, long K, const cd *A, int lda, const cd *B, int l
     if (herm && m != n) throw Error{WTB_ERR_STATE, "Hermitian Gram product of a non-square block"};
     const int bs = std::min(80, std::max((m + 15) / 16, (n + 15) / 16) * 16);
     const int bx = (m + bs - 1) / bs, by = (n + bs - 1) / bs;
-    int nsp = std::max(1, 256 / (bx * by));
+    // output blocks (the upper ones only when Hermitian) times K chunks: about
+    // two workgroups per CU
+    const int nblk = herm ? bx * (bx + 1) / 2 : bx * by;
+    int nsp = std::max(1, 512 / nblk);
     long kc = (K + nsp - 1) / nsp;
     kc = std::max<long>(16, (kc + 15) / 16 * 16);
     nsp = (int)((K + kc - 1) / kc);
     c.ws_gram.alloc((size_t)nsp * m * n);
-    const dim3 g(bx, by, nsp);
+    const dim3 g(nblk, nsp);
     switch (bs) {
 #define WTB_GRAM_CASE(S)                                                                                       \
     case S:                                                                                                    \
         if (herm)                                                                                              \
-            hipLaunchKernelGGL((gram_block_kernel<S, S, true>), g, dim3(256), 0, c.stream, m, n, K, kc, A, lda, B, \
-                               ldb, c.ws_gram.p);                                                              \
+            hipLaunchKernelGGL((gram_block_kernel<S, S, true>), g, dim3(256), 0, c.stream, m, n, bx, K, kc, A, \
+                               lda, B, ldb, c.ws_gram.p);                                                      \
         else                                                                                                   \
-            hipLaunchKernelGGL((gram_block_kernel<S, S, false>), g, dim3(256), 0, c.stream, m, n, K, kc, A, lda,   \
-                               B, ldb, c.ws_gram.p);                                                           \
+            hipLaunchKernelGGL((gram_block_kernel<S, S, false>), g, dim3(256), 0, c.stream, m, n, bx, K, kc, A, \
+                               lda, B, ldb, c.ws_gram.p);                                                      \
         break;
         WTB_GRAM_CASE(16) WTB_GRAM_CASE(32) WTB_GRAM_CASE(48) WTB_GRAM_CASE(64) WTB_GRAM_CASE(80)
 #undef WTB_GRAM_CASE
