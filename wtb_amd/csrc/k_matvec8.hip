@@ -325,7 +325,7 @@ __global__ void sum_chunks8(size_t n, int nch, const double *__restrict__ Yp, do
     Y[i] = s;
 }
 
-constexpr int kRT8 = 4, kWPR8 = 1, kCPW8 = 4;   // 4 wavefronts of 16 rows x 64 columns per workgroup
+constexpr int kRT8 = 4, kWPR8 = 1, kCPW8 = 5;   // 4 wavefronts of 16 rows x up to 80 columns per workgroup
 constexpr int kNL8 = 5;                          // significance levels kept (13 digit pairs)
 
 }  // namespace
@@ -363,9 +363,14 @@ void matvec_int8(Ctx &c, int ncol, const cd *X, cd *Y) {
         }
         const dim3 grid((unsigned)((N + 16 * kRT8 - 1) / (16 * kRT8)), (unsigned)nch);
         c.kclock.begin(KernelClock::INT8X, c.stream);
-        hipLaunchKernelGGL((hx_int8_kernel<kCPW8, kWPR8, kRT8, kNL8>), grid, dim3(64 * kWPR8 * kRT8), 0, c.stream, N, ld,
-                           nc, kc, (float)std::ldexp(1.0 / s.hscale, 17), (const float2 *)s.h32.p, P,
-                           c.ws_scale8.p + nc, yp);
+        // 16-column blocks per wavefront: four up to 64 columns, five beyond
+        const float hmul = (float)std::ldexp(1.0 / s.hscale, 17);
+        if (nc <= 64)
+            hipLaunchKernelGGL((hx_int8_kernel<4, kWPR8, kRT8, kNL8>), grid, dim3(64 * kWPR8 * kRT8), 0, c.stream, N,
+                               ld, nc, kc, hmul, (const float2 *)s.h32.p, P, c.ws_scale8.p + nc, yp);
+        else
+            hipLaunchKernelGGL((hx_int8_kernel<kCPW8, kWPR8, kRT8, kNL8>), grid, dim3(64 * kWPR8 * kRT8), 0, c.stream,
+                               N, ld, nc, kc, hmul, (const float2 *)s.h32.p, P, c.ws_scale8.p + nc, yp);
         c.kclock.end(KernelClock::INT8X, c.stream, 8.0 * (double)ld * N, 8.0 * (double)N * N * nc);
         WTB_LAUNCH_CHECK();
         if (nch > 1) {

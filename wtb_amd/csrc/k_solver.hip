@@ -1128,12 +1128,12 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     // FP32 products cannot resolve residuals much below ~1e-6 relative.  The
     // reduced-precision phase stops at 5e-6 (or the caller's tol if larger):
     // a Ritz value with relative residual r is within r |l| of an eigenvalue
-    // (within (r |l|)^2 / gap once it is isolated).  The refinement below then
-    // applies an accurate product.  When the caller's tol is below what that
-    // one step reaches, the iteration continues on the nev wanted columns with
-    // the accurate product (int8 digits, FP64 recombination: ~3e-11 relative)
-    // until tol is met; every pair counts as converged only at the caller's
-    // tol, in every precision.
+    // (within (r |l|)^2 / gap once it is isolated).  Its last step, or the
+    // refinement below, then applies an accurate product to H X.  When the
+    // caller's tol is below what that reaches, the iteration continues on the
+    // nev wanted columns with the accurate product (int8 digits, FP64
+    // recombination: ~3e-11 relative) until tol is met; every pair counts as
+    // converged only at the caller's tol, in every precision.
     // The preconditioner (diag - l)^-1 is clamped at |diag - l| >= prec_floor
     // (eV).  Ritz values inside the band of transition energies (resonant
     // states) meet diagonal entries within meV; 0.1 eV keeps the correction
@@ -1146,10 +1146,22 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     bool haveP = false;
     int it = 0, nconv = 0;
     std::vector<double> rel(b);
+    // The accurate product of the refinement (int8 digits recombined in FP64,
+    // or FP64 MFMA arithmetic with WTB_REFINE=fp64).
+    const char *renv = std::getenv("WTB_REFINE");
+    const int acc = (renv && std::strcmp(renv, "fp64") == 0) ? WTB_FP64 : WTB_INT8X;
     // one LOBPCG phase with products of precision mvp, until the first nev
     // pairs have relative residual < tol_ph, a reduced-precision stall, or
-    // the iteration budget; returns nconv of the last residual check
-    auto run_phase = [&](int mvp, double tol_ph, bool stall_guard) {
+    // the iteration budget.
+    // With merge, the reduced-precision phase ends in an accurate step once at
+    // most 80 - b columns are still active and every wanted residual is below
+    // 1e-4: [X W] (b + active columns) takes one accurate product, which
+    // replaces both the reduced-precision H W of that step and the separate
+    // refinement of H X.  H P stays the carried combination: its product
+    // errors enter the new H X only through P's coefficients, which are of the
+    // size of the remaining residuals.  merged reports whether this happened.
+    bool merged = false;
+    auto run_phase = [&](int mvp, double tol_ph, bool stall_guard, bool merge) {
         double prev_best = 1e300;
         int stall = 0;
         const int bb = L.b;
@@ -1202,8 +1214,14 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             // orthonormal W (CholQR2); a rank-deficient W falls back to SVQB
             // (H W is formed afterwards from the orthonormal W: no error amplification)
             bool orth = L.cholqr(na, W, nullptr, 1e-12);
-            L.apply(mvp, na, W, AW);
-            mvcols += na;
+            const bool last = merge && bb + na <= 80 && worst < 1e-4;
+            if (last) {
+                L.apply(acc, bb + na, L.S.p, L.AS.p);   // X and W are contiguous in S
+                mvcols += bb + na;
+            } else {
+                L.apply(mvp, na, W, AW);
+                mvcols += na;
+            }
             int m = bb + na;
             if (haveP) {
                 // P orthogonal to X and W; H P follows the same combinations
@@ -1248,9 +1266,15 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             }
             L.rayleigh_ritz(m, true, orth);
             haveP = true;
+            if (last) {
+                merged = true;
+                ++it;
+                break;
+            }
         }
     };
-    run_phase(mv_prec, precision == WTB_FP64 ? tol : std::max(tol, 5e-6), precision != WTB_FP64);
+    run_phase(mv_prec, precision == WTB_FP64 ? tol : std::max(tol, 5e-6), precision != WTB_FP64,
+              precision != WTB_FP64);
     WTB_HIP_CHECK(hipEventRecord(tr, c.stream));
     double max_rel = 0;
     auto final_check = [&]() {
@@ -1264,14 +1288,14 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
         }
     };
     if (precision != WTB_FP64) {
-        // residual refinement: H X on the nev wanted columns with exact int8
-        // digit products recombined in FP64 (or FP64 MFMA arithmetic with
-        // WTB_REFINE=fp64), then an FP64 Rayleigh-Ritz step on them
-        const char *renv = std::getenv("WTB_REFINE");
-        const int acc = (renv && std::strcmp(renv, "fp64") == 0) ? WTB_FP64 : WTB_INT8X;
-        L.apply(acc, nev, L.S.p, L.AS.p);
-        mvcols += nev;
-        L.rayleigh_ritz(nev, false, true, nev);
+        // residual refinement: unless the last step already took the accurate
+        // product, H X on the nev wanted columns with the accurate product,
+        // then an FP64 Rayleigh-Ritz step on them
+        if (!merged) {
+            L.apply(acc, nev, L.S.p, L.AS.p);
+            mvcols += nev;
+            L.rayleigh_ritz(nev, false, true, nev);
+        }
         final_check();
         if (nconv < nev && it < max_it) {
             // tol below what one refinement step reaches: iterate on the nev
@@ -1283,7 +1307,7 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             piv_p = 1e-10;
             p_keep = 1e-5;
             ++it;
-            run_phase(acc, tol, true);
+            run_phase(acc, tol, true, false);
             final_check();
         }
     } else {
