@@ -273,10 +273,14 @@ tridiag_blocked_kernel(int n, int nb, cd *__restrict__ A, int lda, cd *__restric
 // r and, off the diagonal, conj(A_rc) v_r to row c) and of v^H A v, while the
 // owners of column i + 1 stage it.  Row sums reduce over the sixteen q lanes
 // with DPP moves, column sums over the four p rows with the permlane swaps
-// and then over the eight wavefronts in LDS (barrier); each thread forms w =
-// tau y - |tau|^2 (v^H A v) / 2 v for its row and applies reflector i to the
-// staged column i + 1 (barrier).  Elements whose 16-column block lies left of
-// the trailing matrix are skipped.
+// (two 16-column blocks at a time: each swap stage exchanges halves of two
+// values, so one lane row ends with each of the four sums) and then over the
+// eight wavefronts in LDS (barrier); each thread forms w = tau y - |tau|^2
+// (v^H A v) / 2 v for its row and applies reflector i to the staged column
+// i + 1 (barrier).  Elements whose 16-column block lies left of the trailing
+// matrix, or wholly in the padding rows and columns n..32 NA - 1, are
+// skipped.  Up to NA = 5 the row operands of a step stay in registers and the
+// column operands of the next block are read while the current one is used.
 constexpr int kRT = 512;
 constexpr int kRegNA = 7;          // n <= 224
 
@@ -335,6 +339,29 @@ __device__ __forceinline__ double swap_sum(double v) {
 }
 
 __device__ __forceinline__ double rows_sum(double v) { return swap_sum<true>(swap_sum<false>(v)); }
+
+// a <- rows (a0, b0, a2, b2), b <- rows (a1, b1, a3, b3) of the 16-lane rows
+// (S32 false), or a <- (a0, a1, b0, b1), b <- (a2, a3, b2, b3) (S32 true)
+template <bool S32>
+__device__ __forceinline__ void swap_rows(double &a, double &b) {
+    const long long ua = __double_as_longlong(a), ub = __double_as_longlong(b);
+    auto L = S32 ? __builtin_amdgcn_permlane32_swap((unsigned)ua, (unsigned)ub, false, false)
+                 : __builtin_amdgcn_permlane16_swap((unsigned)ua, (unsigned)ub, false, false);
+    auto H = S32 ? __builtin_amdgcn_permlane32_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false)
+                 : __builtin_amdgcn_permlane16_swap((unsigned)(ua >> 32), (unsigned)(ub >> 32), false, false);
+    a = __longlong_as_double(((long long)H[0] << 32) | L[0]);
+    b = __longlong_as_double(((long long)H[1] << 32) | L[1]);
+}
+
+// sums over the four lane rows of four values at once: lane row j ends with
+// the sum of value j, added in the order of rows_sum ((r0 + r1) + (r2 + r3))
+__device__ __forceinline__ double rows_sum4(double v0, double v1, double v2, double v3) {
+    swap_rows<false>(v0, v1);
+    swap_rows<false>(v2, v3);
+    double s = v0 + v1, t = v2 + v3;
+    swap_rows<true>(s, t);
+    return s + t;
+}
 __device__ __forceinline__ double wave_sum_d(double v) { return rows_sum(row_sum(v)); }
 
 template <int NA>
@@ -350,7 +377,7 @@ tridiag_reg_kernel(int n, cd *__restrict__ Ag, int lda, cd *__restrict__ tau, do
     __shared__ cd wsh[2][NR];      // their companions w
     __shared__ cd vw[NW][NR];      // the current reflector, one copy per wavefront
     __shared__ cd rowpart[NR];     // row sums of y
-    __shared__ cd colpart[NW][NR]; // column sums of y, one row per wavefront
+    __shared__ double colx[NW][NR], coly[NW][NR];   // column sums of y, one row per wavefront
     __shared__ double mured[NW];
     const int tid = threadIdx.x, wv = tid >> 6, l = tid & 63;
     const int q = l & 15, p = 4 * wv + (l >> 4);
@@ -439,51 +466,97 @@ tridiag_reg_kernel(int n, cd *__restrict__ Ag, int lda, cd *__restrict__ tau, do
         // pending update and y = A v in one pass over the live elements; the
         // owners of column i + 1 stage it
         const int bnx = (i + 1) >> 4, qnx = (i + 1) & 15;
+        const int b0 = bnx;                       // first live 16-column block
         cd racc[NA];
 #pragma unroll
         for (int a = 0; a < NA; ++a) racc[a] = make_double2(0.0, 0.0);
         double mup = 0.0;
+        // row operands of the step: pending reflector, its w, current reflector
+        constexpr bool HOIST = NA <= 5, AHEAD = HOIST;
+        cd rvo[HOIST ? NA : 1], rwo[HOIST ? NA : 1], rvn[HOIST ? NA : 1];
+        if constexpr (HOIST) {
+#pragma unroll
+            for (int a = 0; a < NA; ++a) {
+                const int r = p + 32 * a;
+                rvo[a] = vo[r];
+                rwo[a] = wo[r];
+                rvn[a] = vme[r];
+            }
+        }
+        auto row_vo = [&](int a) { if constexpr (HOIST) return rvo[a]; else return vo[p + 32 * a]; };
+        auto row_wo = [&](int a) { if constexpr (HOIST) return rwo[a]; else return wo[p + 32 * a]; };
+        auto row_vn = [&](int a) { if constexpr (HOIST) return rvn[a]; else return vme[p + 32 * a]; };
+        // column operands and diagonal element of the next live block (read
+        // one block ahead when the registers allow)
+        cd nvo, nwo, nvn, nD;
+        if constexpr (AHEAD) {
+            const int c = q + 16 * b0, qq = q + 16 * (b0 & 1);
+            nvo = vo[c];
+            nwo = wo[c];
+            nvn = vme[c];
+            nD = D[(b0 >> 1) * NDB + prow + min(qq, p)];
+        }
+        cd cpair[2];     // column partials of blocks b - 1 and b
 #pragma unroll
         for (int b = 0; b < 2 * NA; ++b) {
-            if (16 * b + 15 <= i) continue;      // columns <= i are final
-            const int c = q + 16 * b;
-            const cd voc = vo[c], woc = wo[c], vnc = vme[c];
-            cd cacc = make_double2(0.0, 0.0), Ad = make_double2(0.0, 0.0);
-            const int qq = q + 16 * (b & 1), rd = 32 * (b >> 1) + p;
-            // the diagonal-block operands are read by every lane (clamped to
-            // a valid element) ahead of the lanes' branch
-            const int di = (b >> 1) * NDB + prow + min(qq, p);
-            Ad = D[di];
-            const cd vod = vo[rd], wod = wo[rd], vmd = vme[rd];
-            if (qq <= p) {
-                rank2_sub(Ad, vod, wod, voc, woc);
-                D[di] = Ad;
-                cmac(racc[b >> 1], Ad, vnc);
-                if (qq < p) cmac_conj(cacc, Ad, vmd);
-            }
+            cpair[b & 1] = make_double2(0.0, 0.0);
+            if (b >= b0 && 16 * b < n) {
+                const int c = q + 16 * b;
+                if constexpr (!AHEAD) {
+                    nvo = vo[c];
+                    nwo = wo[c];
+                    nvn = vme[c];
+                    nD = D[(b >> 1) * NDB + prow + min(q + 16 * (b & 1), p)];
+                }
+                const cd voc = nvo, woc = nwo, vnc = nvn;
+                // the diagonal-block element is read by every lane (clamped to
+                // a valid element) ahead of the lanes' branch
+                cd Ad = nD;
+                if (AHEAD && b + 1 < 2 * NA) {
+                    const int qn = q + 16 * ((b + 1) & 1);
+                    nvo = vo[c + 16];
+                    nwo = wo[c + 16];
+                    nvn = vme[c + 16];
+                    nD = D[((b + 1) >> 1) * NDB + prow + min(qn, p)];
+                }
+                cd cacc = make_double2(0.0, 0.0);
+                const int qq = q + 16 * (b & 1), rd = 32 * (b >> 1) + p;
+                if (qq <= p) {
+                    rank2_sub(Ad, row_vo(b >> 1), row_wo(b >> 1), voc, woc);
+                    D[(b >> 1) * NDB + prow + qq] = Ad;
+                    cmac(racc[b >> 1], Ad, vnc);
+                    if (qq < p) cmac_conj(cacc, Ad, row_vn(b >> 1));
+                }
 #pragma unroll
-            for (int a = b / 2 + 1; a < NA; ++a) {
-                const int r = p + 32 * a;
-                cd &A = R[a * (a - 1) + b];
-                rank2_sub(A, vo[r], wo[r], voc, woc);
-                cmac(racc[a], A, vnc);
-                cmac_conj(cacc, A, vme[r]);
-            }
-            if (b == bnx && q == qnx) {
-                if (qq <= p) colpre[rd] = Ad;
+                for (int a = b / 2 + 1; a < NA; ++a) {
+                    if (32 * a >= n) break;
+                    cd &A = R[a * (a - 1) + b];
+                    rank2_sub(A, row_vo(a), row_wo(a), voc, woc);
+                    cmac(racc[a], A, vnc);
+                    cmac_conj(cacc, A, row_vn(a));
+                }
+                if (b == bnx && q == qnx) {
+                    if (qq <= p) colpre[rd] = Ad;
 #pragma unroll
-                for (int a = b / 2 + 1; a < NA; ++a) colpre[p + 32 * a] = R[a * (a - 1) + b];
+                    for (int a = b / 2 + 1; a < NA; ++a) colpre[p + 32 * a] = R[a * (a - 1) + b];
+                }
+                mup = fma(vnc.x, cacc.x, fma(vnc.y, cacc.y, mup));
+                cpair[b & 1] = cacc;
             }
-            mup = fma(vnc.x, cacc.x, fma(vnc.y, cacc.y, mup));
-            cacc.x = rows_sum(cacc.x);
-            cacc.y = rows_sum(cacc.y);
-            if (l < 16) colpart[wv][c] = cacc;
+            if ((b & 1) && b >= b0 && 16 * (b - 1) < n) {
+                // lane row j: x (j = 0) and y (j = 1) of block b - 1, x and y
+                // (j = 2, 3) of block b
+                const double sum = rows_sum4(cpair[0].x, cpair[0].y, cpair[1].x, cpair[1].y);
+                const int j = l >> 4;
+                ((j & 1) ? coly : colx)[wv][q + 16 * (b - 1 + (j >> 1))] = sum;
+            }
         }
 #pragma unroll
         for (int a = 0; a < NA; ++a) {
             if (32 * a + 31 <= i) continue;      // rows <= i are not needed
+            if (32 * a >= n) break;
             const int r = p + 32 * a;
-            const cd vnr = vme[r];
+            const cd vnr = row_vn(a);
             cd s = racc[a];
             mup = fma(vnr.x, s.x, fma(vnr.y, s.y, mup));
             s.x = row_sum(s.x);
@@ -504,9 +577,8 @@ tridiag_reg_kernel(int n, cd *__restrict__ Ag, int lda, cd *__restrict__ tau, do
                 cd y = rowpart[r];
 #pragma unroll
                 for (int k = 0; k < NW; ++k) {
-                    const cd cp = colpart[k][r];
-                    y.x += cp.x;
-                    y.y += cp.y;
+                    y.x += colx[k][r];
+                    y.y += coly[k][r];
                 }
                 const cd v = vme[r];
                 return make_double2(t.x * y.x - t.y * y.y - hm * v.x, t.x * y.y + t.y * y.x - hm * v.y);
@@ -895,30 +967,44 @@ backtransform_reg_kernel(int n, int nvec, const cd *__restrict__ A, int lda, con
         const int r = lane + 64 * s;
         z[s] = make_double2((active && r < n) ? Y[(size_t)col * n + r] : 0.0, 0.0);
     }
+    // reflector q of a chunk (i = i1 - q), rows r = tid + kBT u (S rows per 64
+    // lanes: n <= 64 S, so u < U); the next chunk's loads are in flight while
+    // the current one is applied
+    constexpr int U = (S + 3) / 4;
+    cd vn[kBTChunk][U];
+    cd tn = make_double2(0.0, 0.0);
+    auto load_chunk = [&](int i1) {
+        const int cnt = min(kBTChunk, i1 + 1);
+        if (tid < cnt) tn = tau[i1 - tid];
+#pragma unroll
+        for (int q = 0; q < kBTChunk; ++q) {
+            const int i = i1 - q;
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int r = tid + kBT * u;
+                cd x = make_double2(0.0, 0.0);
+                if (q < cnt) {
+                    if (r == i + 1) x = make_double2(1.0, 0.0);
+                    else if (r > i + 1 && r < n) x = A[r + (size_t)i * lda];
+                }
+                vn[q][u] = x;
+            }
+        }
+    };
+    if (n >= 2) load_chunk(n - 2);
     for (int i1 = n - 2; i1 >= 0; i1 -= kBTChunk) {
         const int cnt = min(kBTChunk, i1 + 1);     // reflectors i1, i1 - 1, ..., i1 - cnt + 1
         __syncthreads();                           // the previous chunk is consumed
-        // reflector q of the chunk (i = i1 - q), rows r = tid + kBT u (S rows per
-        // 64 lanes: n <= 64 S, so u < S / 4 + 1)
-        for (int q = 0; q < cnt; ++q) {
-            const int i = i1 - q;
-            const cd *ai = A + (size_t)i * lda;
-            cd v[(S + 3) / 4];
 #pragma unroll
-            for (int u = 0; u < (S + 3) / 4; ++u) {
-                const int r = tid + kBT * u;
-                v[u] = make_double2(0.0, 0.0);
-                if (r == i + 1) v[u] = make_double2(1.0, 0.0);
-                else if (r > i + 1 && r < n) v[u] = ai[r];
-            }
+        for (int q = 0; q < kBTChunk; ++q)
 #pragma unroll
-            for (int u = 0; u < (S + 3) / 4; ++u) {
+            for (int u = 0; u < U; ++u) {
                 const int r = tid + kBT * u;
-                if (r < n) vs[(size_t)q * n + r] = v[u];
+                if (q < cnt && r < n) vs[(size_t)q * n + r] = vn[q][u];
             }
-        }
-        if (tid < cnt) ts[tid] = tau[i1 - tid];
+        if (tid < cnt) ts[tid] = tn;
         __syncthreads();
+        if (i1 - kBTChunk >= 0) load_chunk(i1 - kBTChunk);
         if (!active) continue;
         for (int q = 0; q < cnt; ++q) {
             const cd t = ts[q];

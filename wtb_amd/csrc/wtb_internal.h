@@ -300,12 +300,14 @@ __device__ __forceinline__ double wave_lane(double v, int i) {
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
-// v moved across lanes by the DPP pattern CTRL (both 32-bit halves)
+// v moved across lanes by the DPP pattern CTRL (both 32-bit halves); a lane
+// whose source lane is disabled reads zero (bound_ctrl), so no old value has
+// to be set up
 template <int CTRL>
 __device__ __forceinline__ double dpp_double(double v) {
     const long long b = __double_as_longlong(v);
-    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
-    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)b, CTRL, 0xF, 0xF, true);
+    const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(b >> 32), CTRL, 0xF, 0xF, true);
     return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
 }
 
