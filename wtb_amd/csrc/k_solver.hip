@@ -450,6 +450,99 @@ gram_block_kernel(int m, int n, int bx, long K, long kc, const cd *__restrict__ 
     }
 }
 
+// Single-block form (C of at most 80 x 80, the whole product one block; upper
+// tiles only when HERM): a wavefront owns NS tiles (w + 4 t, repeating tile 0
+// past the last one, which is then not stored), and per 4-deep k slice the
+// first product of every tile is issued, then the second, ..., so consecutive
+// MFMAs never share an accumulator.  The partial sums per tile are those of
+// gram_block_kernel, in the same order.
+template <int MT, bool HERM>
+__global__ void __launch_bounds__(256)
+gram_one_kernel(int m, int n, long K, long kc, const cd *__restrict__ A, int lda, const cd *__restrict__ B, int ldb,
+                cd *__restrict__ part) {
+    constexpr int KS = 16, TI = MT / 16, NTILE = HERM ? TI * (TI + 1) / 2 : TI * TI, NS = (NTILE + 3) / 4;
+    constexpr int AE = KS * MT / 256;
+    static_assert(KS * MT % 256 == 0, "block must split evenly over the workgroup");
+    __shared__ cd sa[KS][MT + 1], sb[KS][MT + 1];
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    int tis[NS], tjs[NS];
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        int tile = w + 4 * t;
+        if (tile >= NTILE) tile = 0;
+        if (HERM) {
+            int tj = 0;
+            while ((tj + 1) * (tj + 2) / 2 <= tile) ++tj;
+            tjs[t] = tj;
+            tis[t] = tile - tj * (tj + 1) / 2;
+        } else {
+            tis[t] = tile % TI;
+            tjs[t] = tile / TI;
+        }
+    }
+    const long k0 = (long)blockIdx.y * kc, k1 = min(K, k0 + kc);
+    // columns past m (n) read the last valid one; rows past the chunk read zero
+    cd ra[AE], rb[AE];
+    auto load = [&](long s) {
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+            const int idx = tid + 256 * e, k = idx % KS, i = idx / KS;
+            const long kk = s + k;
+            ra[e] = kk < k1 ? A[kk + (size_t)min(i, m - 1) * lda] : make_double2(0.0, 0.0);
+            rb[e] = kk < k1 ? B[kk + (size_t)min(i, n - 1) * ldb] : make_double2(0.0, 0.0);
+        }
+    };
+    d4 cre[NS], cim[NS];
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        cre[t] = d4{0.0, 0.0, 0.0, 0.0};
+        cim[t] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+    load(k0);
+    for (long s = k0; s < k1; s += KS) {
+        __syncthreads();
+#pragma unroll
+        for (int e = 0; e < AE; ++e) {
+            const int idx = tid + 256 * e;
+            sa[idx % KS][idx / KS] = ra[e];
+            sb[idx % KS][idx / KS] = rb[e];
+        }
+        __syncthreads();
+        if (s + KS < k1) load(s + KS);      // next K-step in flight during the MFMAs
+#pragma unroll
+        for (int q = 0; q < KS / 4; ++q) {
+            const int k = 4 * q + (lane >> 4);
+            cd a[NS], b[NS];
+#pragma unroll
+            for (int t = 0; t < NS; ++t) {
+                a[t] = sa[k][16 * tis[t] + (lane & 15)];
+                b[t] = sb[k][16 * tjs[t] + (lane & 15)];
+            }
+            // conj(a) b
+#pragma unroll
+            for (int t = 0; t < NS; ++t) cre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t].x, b[t].x, cre[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NS; ++t) cim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t].x, b[t].y, cim[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NS; ++t) cre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[t].y, b[t].y, cre[t], 0, 0, 0);
+#pragma unroll
+            for (int t = 0; t < NS; ++t)
+                cim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-a[t].y, b[t].x, cim[t], 0, 0, 0);
+        }
+    }
+    cd *p = part + (size_t)blockIdx.y * m * n;
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+        if (w + 4 * t >= NTILE) continue;
+        const int jj = 16 * tjs[t] + (lane & 15);
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int ii = 16 * tis[t] + 4 * r + (lane >> 4);
+            if (ii < m && jj < n) p[ii + (size_t)jj * m] = make_double2(cre[t][r], cim[t][r]);
+        }
+    }
+}
+
 // C[i + j ldc] = sum over chunks of part[s][i + j m]: lanes take 64 consecutive
 // elements, the kRedG wavefronts every kRedG-th chunk with eight loads in
 // flight, then an LDS sum in a fixed order.  Sixteen wavefronts per block keep
@@ -515,13 +608,29 @@ void gram(Ctx &c, int m, int n, long K, const cd *A, int lda, const cd *B, int l
     // output blocks (the upper ones only when Hermitian) times K chunks: about
     // two workgroups per CU
     const int nblk = herm ? bx * (bx + 1) / 2 : bx * by;
-    int nsp = std::max(1, 512 / nblk);
+    // one block: about one workgroup per CU (the single-block kernel holds
+    // more accumulators); several blocks: about two per CU
+    int nsp = std::max(1, (nblk == 1 ? 256 : 512) / nblk);
     long kc = (K + nsp - 1) / nsp;
     kc = std::max<long>(16, (kc + 15) / 16 * 16);
     nsp = (int)((K + kc - 1) / kc);
     c.ws_gram.alloc((size_t)nsp * m * n);
     const dim3 g(nblk, nsp);
-    switch (bs) {
+    if (nblk == 1) {
+        switch (bs) {
+#define WTB_GRAM1_CASE(S)                                                                                         \
+    case S:                                                                                                       \
+        if (herm)                                                                                                 \
+            hipLaunchKernelGGL((gram_one_kernel<S, true>), g, dim3(256), 0, c.stream, m, n, K, kc, A, lda, B, ldb, \
+                               c.ws_gram.p);                                                                      \
+        else                                                                                                      \
+            hipLaunchKernelGGL((gram_one_kernel<S, false>), g, dim3(256), 0, c.stream, m, n, K, kc, A, lda, B, ldb, \
+                               c.ws_gram.p);                                                                      \
+        break;
+            WTB_GRAM1_CASE(16) WTB_GRAM1_CASE(32) WTB_GRAM1_CASE(48) WTB_GRAM1_CASE(64) WTB_GRAM1_CASE(80)
+#undef WTB_GRAM1_CASE
+        }
+    } else switch (bs) {
 #define WTB_GRAM_CASE(S)                                                                                       \
     case S:                                                                                                    \
         if (herm)                                                                                              \
