@@ -790,8 +790,9 @@ void trsm_right_upper(Ctx &c, long N, int nb, const cd *R, int ldr, cd *B, cd *B
 // forms its 16 rows for all 16-column tiles of C.  B is staged in LDS 16 rows
 // of k at a time (the next 16 rows and the next A fragments are in flight in
 // registers during the MFMAs); A fragments come straight from global memory.
-// The MFMAs of one k-step are issued phase by phase over the tiles (Re Re,
-// Re Im, -Im Im, Im Re), so consecutive MFMAs never share an accumulator.
+// A complex tile step is three real MFMAs (Gauss's product: ar br, ai bi and
+// (ar + ai)(br + bi), combined at the end), issued phase by phase over the
+// tiles so consecutive MFMAs never share an accumulator.
 // gridDim.y = number of products.  A and C must not overlap.
 constexpr int kTgKB = 16;
 
@@ -807,11 +808,14 @@ tall_gemm_kernel(long M, int k, int n, const cd *__restrict__ A0, long lda, cd *
     const int li = lane & 15, lk = lane >> 4;
     const long r0 = (long)blockIdx.x * 64 + 16 * w;
     const long ra = min(r0 + li, M - 1);      // rows past M read the last one (never written)
-    d4 cre[NCB], cim[NCB];
+    // three real products per complex tile step (Gauss): t1 = ar br,
+    // t2 = ai bi, t3 = (ar + ai)(br + bi); re = t1 - t2, im = t3 - t1 - t2
+    d4 c1[NCB], c2[NCB], c3[NCB];
 #pragma unroll
     for (int t = 0; t < NCB; ++t) {
-        cre[t] = d4{0.0, 0.0, 0.0, 0.0};
-        cim[t] = d4{0.0, 0.0, 0.0, 0.0};
+        c1[t] = d4{0.0, 0.0, 0.0, 0.0};
+        c2[t] = d4{0.0, 0.0, 0.0, 0.0};
+        c3[t] = d4{0.0, 0.0, 0.0, 0.0};
     }
     const int nch = (k + KB - 1) / KB;
     cd af[KB / 4], an[KB / 4], bq[BE];
@@ -851,15 +855,14 @@ tall_gemm_kernel(long M, int k, int n, const cd *__restrict__ A0, long lda, cd *
             cd b[NCB];
 #pragma unroll
             for (int t = 0; t < NCB; ++t) b[t] = sb[buf][4 * s + lk][16 * t + li];
-            const double ar = af[s].x, ai = af[s].y;
+            const double ar = af[s].x, ai = af[s].y, as = ar + ai;
 #pragma unroll
-            for (int t = 0; t < NCB; ++t) cre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b[t].x, cre[t], 0, 0, 0);
+            for (int t = 0; t < NCB; ++t) c1[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b[t].x, c1[t], 0, 0, 0);
 #pragma unroll
-            for (int t = 0; t < NCB; ++t) cim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, b[t].y, cim[t], 0, 0, 0);
+            for (int t = 0; t < NCB; ++t) c2[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b[t].y, c2[t], 0, 0, 0);
 #pragma unroll
-            for (int t = 0; t < NCB; ++t) cre[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, b[t].y, cre[t], 0, 0, 0);
-#pragma unroll
-            for (int t = 0; t < NCB; ++t) cim[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, b[t].x, cim[t], 0, 0, 0);
+            for (int t = 0; t < NCB; ++t)
+                c3[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(as, b[t].x + b[t].y, c3[t], 0, 0, 0);
         }
         if (ch + 1 < nch) {
             store_b(buf ^ 1);
@@ -878,11 +881,12 @@ tall_gemm_kernel(long M, int k, int n, const cd *__restrict__ A0, long lda, cd *
             const long r = r0 + 4 * q + lk;
             if (r >= M) continue;
             cd *cp = C + r + (size_t)j * ldc;
+            const double re = c1[t][q] - c2[t][q], im = c3[t][q] - c1[t][q] - c2[t][q];
             if (sub) {
                 const cd v = *cp;
-                *cp = make_double2(v.x - cre[t][q], v.y - cim[t][q]);
+                *cp = make_double2(v.x - re, v.y - im);
             } else {
-                *cp = make_double2(cre[t][q], cim[t][q]);
+                *cp = make_double2(re, im);
             }
         }
     }
