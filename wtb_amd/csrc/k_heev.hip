@@ -1053,7 +1053,13 @@ backtransform_reg_kernel(int n, int nvec, const cd *__restrict__ A, int lda, con
 // workgroup with one barrier per column.  Step j updates the trailing upper
 // triangle with the unscaled pivot row, g_pq -= conj(g_jp) g_jq / g_jj, and
 // leaves row j unscaled; the rows are divided by sqrt(g_jj) at the end.
+// Each thread owns the same upper-triangle elements for the whole
+// factorisation: element e = tid + kCT u, the elements numbered row by row
+// from the last row up (row and column decoded once).  The trailing triangle
+// of step j (rows > j) is then the first (n - 1 - j)(n - j) / 2 elements, so
+// a step stops at the first slot past it.
 constexpr int kCT = 256;
+constexpr int kCholU = (kCholLds * (kCholLds + 1) / 2 + kCT - 1) / kCT;   // elements per thread
 __global__ void __launch_bounds__(kCT)
 cholesky_lds_kernel(int n, cd *__restrict__ Gg, int ldgg, double rel, int *__restrict__ flag) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -1073,6 +1079,18 @@ cholesky_lds_kernel(int n, cd *__restrict__ Gg, int ldgg, double rel, int *__res
         *flag = 0;
     }
     __syncthreads();
+    // owned elements: row | column << 8; element e lies in row n - 1 - r with
+    // r (r + 1) / 2 <= e < (r + 1) (r + 2) / 2
+    int own[kCholU];
+#pragma unroll
+    for (int u = 0; u < kCholU; ++u) {
+        const int e = tid + kCT * u;
+        int r = (int)((sqrtf(8.0f * (float)e + 1.0f) - 1.0f) * 0.5f);
+        while ((r + 1) * (r + 2) / 2 <= e) ++r;
+        while (r * (r + 1) / 2 > e) --r;
+        const int P = n - 1 - r;
+        own[u] = P | ((P + e - r * (r + 1) / 2) << 8);
+    }
     for (int j = 0; j < n; ++j) {
         const double piv = G[j + j * ln].x;
         if (!(piv > rel * d0[j])) {
@@ -1082,23 +1100,20 @@ cholesky_lds_kernel(int n, cd *__restrict__ Gg, int ldgg, double rel, int *__res
             }
             break;      // every thread reads the same pivot and leaves together
         }
-        const double ip = 1.0 / piv;
-        // upper triangle of the trailing block, p <= q, as (q, p) pairs
-        const int s = n - j - 1;
-        const int tri = s * (s + 1) / 2;
-        for (int t = tid; t < tri; t += kCT) {
-            // column q (0-based within the block) holds q+1 entries
-            int q = (int)((sqrtf(8.0f * (float)t + 1.0f) - 1.0f) * 0.5f);
-            while ((q + 1) * (q + 2) / 2 <= t) ++q;
-            while (q * (q + 1) / 2 > t) --q;
-            const int p = t - q * (q + 1) / 2;
-            const int P = j + 1 + p, Q = j + 1 + q;
-            const cd a = G[j + P * ln], b = G[j + Q * ln];
-            cd g = G[P + Q * ln];
-            // g -= conj(a) b / piv
-            g.x -= (a.x * b.x + a.y * b.y) * ip;
-            g.y -= (a.x * b.y - a.y * b.x) * ip;
-            G[P + Q * ln] = g;
+        const double ip = recip(piv);      // off the division's latency: the step's chain
+        const int live = (n - 1 - j) * (n - j) / 2;      // elements of rows > j
+#pragma unroll
+        for (int u = 0; u < kCholU; ++u) {
+            if (kCT * u >= live) break;
+            const int P = own[u] & 255, Q = own[u] >> 8;
+            if (tid + kCT * u < live) {
+                const cd a = G[j + P * ln], b = G[j + Q * ln];
+                cd g = G[P + Q * ln];
+                // g -= conj(a) b / piv
+                g.x -= (a.x * b.x + a.y * b.y) * ip;
+                g.y -= (a.x * b.y - a.y * b.x) * ip;
+                G[P + Q * ln] = g;
+            }
         }
         __syncthreads();
     }
