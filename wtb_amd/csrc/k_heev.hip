@@ -364,6 +364,23 @@ __device__ __forceinline__ double rows_sum4(double v0, double v1, double v2, dou
 }
 __device__ __forceinline__ double wave_sum_d(double v) { return rows_sum(row_sum(v)); }
 
+// 1/q from the hardware reciprocal refined by two Newton steps: within an ulp
+// of the divided form at a fraction of an IEEE division's latency, for the
+// sequential chains (reflectors, Sturm counts, pivots).
+__device__ __forceinline__ double recip(double q) {
+    double r = __builtin_amdgcn_rcp(q);
+    r = fma(r, fma(-q, r, 1.0), r);
+    return fma(r, fma(-q, r, 1.0), r);
+}
+
+// 1/sqrt(s) (s > 0) from the hardware estimate refined by two Newton steps
+__device__ __forceinline__ double rsqrt_nr(double s) {
+    double r = __builtin_amdgcn_rsq(s);
+    const double h = 0.5 * s;
+    r = fma(r, fma(-h * r, r, 0.5), r);
+    return fma(r, fma(-h * r, r, 0.5), r);
+}
+
 template <int NA>
 __global__ void __launch_bounds__(kRT)
 tridiag_reg_kernel(int n, cd *__restrict__ Ag, int lda, cd *__restrict__ tau, double *__restrict__ d,
@@ -433,11 +450,17 @@ tridiag_reg_kernel(int n, cd *__restrict__ Ag, int lda, cd *__restrict__ tau, do
             double beta = alpha.x;
             cd scal = make_double2(0.0, 0.0);
             if (!ident) {
-                const double nrm = sqrt(alpha.x * alpha.x + alpha.y * alpha.y + xn2);
+                // sqrt and the two reciprocals by Newton-refined hardware
+                // estimates: they sit on the step's critical path
+                // (the norm gets one residual correction: its rounding sets
+                // how far the reflector is from unitary)
+                const double s2 = alpha.x * alpha.x + alpha.y * alpha.y + xn2;
+                const double rs = rsqrt_nr(s2), g = s2 * rs;
+                const double nrm = fma(fma(-g, g, s2), 0.5 * rs, g);
                 beta = alpha.x >= 0.0 ? -nrm : nrm;
-                const double ib = 1.0 / beta;
+                const double ib = recip(beta);
                 t = make_double2((beta - alpha.x) * ib, -alpha.y * ib);
-                const double idd = 1.0 / ((alpha.x - beta) * (alpha.x - beta) + alpha.y * alpha.y);
+                const double idd = recip((alpha.x - beta) * (alpha.x - beta) + alpha.y * alpha.y);
                 scal = make_double2((alpha.x - beta) * idd, -alpha.y * idd);   // 1/(alpha - beta)
             }
 #pragma unroll
@@ -607,15 +630,8 @@ tridiag_reg_kernel(int n, cd *__restrict__ Ag, int lda, cd *__restrict__ tau, do
 
 // ---------------------------------------------------------------------------
 // 2. tridiagonal eigenpairs
-// Number of eigenvalues of T below x (Sturm sequence of the LDL^T pivots).
-// 1/q comes from the hardware reciprocal refined by two Newton steps, which
-// keeps the pivots to within an ulp of the divided form at a fraction of the
-// latency of an IEEE division on the sequential chain.
-__device__ __forceinline__ double recip(double q) {
-    double r = __builtin_amdgcn_rcp(q);
-    r = fma(r, fma(-q, r, 1.0), r);
-    return fma(r, fma(-q, r, 1.0), r);
-}
+// Number of eigenvalues of T below x (Sturm sequence of the LDL^T pivots),
+// with 1/q from recip().
 
 __device__ int sturm_count(int n, const double *d, const double *e2, double x, double pivmin) {
     int cnt = 0;
