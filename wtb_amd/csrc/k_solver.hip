@@ -1282,10 +1282,15 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             std::vector<double> n2 = L.norms(bb, L.S.p, L.AS.p, L.lam.p);
             nconv = 0;
             std::vector<int> active;
+            // the guard columns (j >= nev) get a search direction in the first
+            // iteration only: afterwards they follow the Rayleigh-Ritz rotations
+            // of the block, which keeps their part in the wanted pairs' convergence
+            // (the same iteration counts on the bench workload) with fewer columns
+            // in every product and a smaller Rayleigh-Ritz problem
             for (int j = 0; j < bb; ++j) {
                 rel[j] = std::sqrt(n2[j]) / std::max(std::fabs(L.h_lam[j]), 1e-300);
                 if (j < nev && rel[j] < tol_ph) ++nconv;
-                if (rel[j] >= tol_ph) active.push_back(j);
+                if (rel[j] >= tol_ph && (j < nev || it == 0)) active.push_back(j);
             }
             double worst = 0;
             for (int j = 0; j < nev; ++j) worst = std::max(worst, rel[j]);
