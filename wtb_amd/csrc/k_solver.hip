@@ -1206,8 +1206,11 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     // would amplify that noise beyond the spectral resolution, so they are
     // dropped from the basis.
     L.drop_rel = (precision == WTB_FP64) ? 1e-12 : 1e-4;
-    double piv_p = (precision == WTB_FP64) ? 1e-12 : 1e-2;
-    double p_keep = (precision == WTB_FP64) ? 1e-6 : 3e-2;
+    // P block thresholds of the FP64 phase (the accurate phase after a
+    // reduced-precision one sets its own; the reduced-precision phase has no P):
+    // CholQR pivot floor and the smallest projected norm a P column keeps
+    double piv_p = 1e-12;
+    double p_keep = 1e-6;
     // initial block: unit vectors on the b smallest diagonal entries
     const double *d = c.lws.hd.p;
     WTB_HIP_CHECK(hipMemcpyAsync(c.lws.hd.p, c.bse.diag.p, sizeof(double) * N, hipMemcpyDeviceToHost, c.stream));
@@ -1266,15 +1269,19 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     // one LOBPCG phase with products of precision mvp, until the first nev
     // pairs have relative residual < tol_ph, a reduced-precision stall, or
     // the iteration budget.
+    // useP: the previous step's direction P joins the basis.  With the
+    // reduced-precision products it does not: H P would be carried by
+    // combinations, and after its projection against [X W] every column
+    // fell below the threshold that bounds the amplified product error
+    // (on the bench workload none survived in any iteration), so the
+    // phase runs as a block Davidson iteration on [X W].
     // With merge, the reduced-precision phase ends in an accurate step once at
     // most 80 - b columns are still active and every wanted residual is below
     // 1e-4: [X W] (b + active columns) takes one accurate product, which
     // replaces both the reduced-precision H W of that step and the separate
-    // refinement of H X.  H P stays the carried combination: its product
-    // errors enter the new H X only through P's coefficients, which are of the
-    // size of the remaining residuals.  merged reports whether this happened.
+    // refinement of H X.  merged reports whether this happened.
     bool merged = false;
-    auto run_phase = [&](int mvp, double tol_ph, bool stall_guard, bool merge) {
+    auto run_phase = [&](int mvp, double tol_ph, bool stall_guard, bool merge, bool useP) {
         double prev_best = 1e300;
         int stall = 0;
         const int bb = L.b;
@@ -1341,7 +1348,7 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
                 mvcols += na;
             }
             int m = bb + na;
-            if (haveP) {
+            if (haveP && useP) {
                 // P orthogonal to X and W; H P follows the same combinations
                 cd *Pd = L.S.p + (size_t)(bb + na) * N;
                 cd *APd = L.AS.p + (size_t)(bb + na) * N;
@@ -1382,8 +1389,8 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
                     if (!orth || L.cholqr(np, Pd, APd, piv_p)) m += np;
                 }
             }
-            L.rayleigh_ritz(m, true, orth);
-            haveP = true;
+            L.rayleigh_ritz(m, useP, orth);
+            haveP = useP;
             if (last) {
                 merged = true;
                 ++it;
@@ -1392,7 +1399,7 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
         }
     };
     run_phase(mv_prec, precision == WTB_FP64 ? tol : std::max(tol, 5e-6), precision != WTB_FP64,
-              precision != WTB_FP64);
+              precision != WTB_FP64, precision == WTB_FP64);
     WTB_HIP_CHECK(hipEventRecord(tr, c.stream));
     double max_rel = 0;
     auto final_check = [&]() {
@@ -1425,7 +1432,7 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             piv_p = 1e-10;
             p_keep = 1e-5;
             ++it;
-            run_phase(acc, tol, true, false);
+            run_phase(acc, tol, true, false, true);
             final_check();
         }
     } else {
