@@ -260,8 +260,8 @@ void matvec(Ctx &c, int precision, int ncol, const cd *X, cd *Y) {
                                      KernelClock::F64_ON_F32);
         else if (precision == WTB_MIXED)
             matvec_f16split(c, ncol, X, Y);
-        else if (precision == WTB_INT8X)
-            matvec_int8(c, ncol, X, Y);
+        else if (precision == WTB_INT8X || precision == kInt8Short)
+            matvec_int8(c, ncol, X, Y, precision == kInt8Short ? 4 : 5);
         else
             launch_hx<float, float>(c.stream, N, s.ld, ncol, (const float *)s.h32.p, x, y, c.kclock,
                                     KernelClock::F32_ON_F32);

@@ -27,6 +27,12 @@
 // The product is exact with respect to this fixed-point H and X; it is not an
 // FP64 product, and it is compared with the FP64 MFMA product on the same
 // store in the tests.
+// A short form keeps four levels (the 10 pairs with s_H + s_X >= 3): the
+// dropped pairs then reach 2^-28 max|H| max|X(:,c)| per term, about 2^3
+// times the fixed-point rounding, for callers whose tolerance is far above
+// it; it issues 10 instead of 13 digit-pair MFMA groups.  (Three levels, 6
+// pairs, leave 2^-20 per term: a residual error near 5e-5 of |lambda| at
+// N = 32400, above the solver's loosest tolerance.)
 //
 // Overflow: per k, a level accumulates two components times at most four
 // digit pairs of |d_H d_X| <= 128 * 128 (64 * 128 with a top digit): 98304 in
@@ -327,12 +333,18 @@ __global__ void sum_chunks8(size_t n, int nch, const double *__restrict__ Yp, do
 
 constexpr int kRT8 = 4, kWPR8 = 1, kCPW8 = 5;   // 4 wavefronts of 16 rows x up to 80 columns per workgroup
 constexpr int kNL8 = 5;                          // significance levels kept (13 digit pairs)
+constexpr int kNL8s = 4;                         // short form (10 digit pairs)
 
 }  // namespace
 
 // Y = H_offdiag X on the FP32 store with exact int8 digit products; the
-// caller adds the FP64 diagonal.
-void matvec_int8(Ctx &c, int ncol, const cd *X, cd *Y) {
+// caller adds the FP64 diagonal.  levels: 5 (full) or 4 (short form);
+// WTB_INT8_LEVELS=4 selects the short form for every call (tests).
+void matvec_int8(Ctx &c, int ncol, const cd *X, cd *Y, int levels) {
+    if (const char *e = std::getenv("WTB_INT8_LEVELS")) {
+        if (std::atoi(e) == kNL8s) levels = kNL8s;
+    }
+    const bool shrt = levels == kNL8s;
     Bse &s = c.bse;
     const long N = s.N, ld = s.ld;
     constexpr int NCMAX = 16 * kCPW8 * kWPR8;
@@ -365,12 +377,17 @@ void matvec_int8(Ctx &c, int ncol, const cd *X, cd *Y) {
         c.kclock.begin(KernelClock::INT8X, c.stream);
         // 16-column blocks per wavefront: four up to 64 columns, five beyond
         const float hmul = (float)std::ldexp(1.0 / s.hscale, 17);
-        if (nc <= 64)
-            hipLaunchKernelGGL((hx_int8_kernel<4, kWPR8, kRT8, kNL8>), grid, dim3(64 * kWPR8 * kRT8), 0, c.stream, N,
-                               ld, nc, kc, hmul, (const float2 *)s.h32.p, P, c.ws_scale8.p + nc, yp);
-        else
-            hipLaunchKernelGGL((hx_int8_kernel<kCPW8, kWPR8, kRT8, kNL8>), grid, dim3(64 * kWPR8 * kRT8), 0, c.stream,
-                               N, ld, nc, kc, hmul, (const float2 *)s.h32.p, P, c.ws_scale8.p + nc, yp);
+#define WTB_INT8_LAUNCH(CPW, NL)                                                                              \
+    hipLaunchKernelGGL((hx_int8_kernel<CPW, kWPR8, kRT8, NL>), grid, dim3(64 * kWPR8 * kRT8), 0, c.stream, N, ld, \
+                       nc, kc, hmul, (const float2 *)s.h32.p, P, c.ws_scale8.p + nc, yp)
+        if (nc <= 64) {
+            if (shrt) WTB_INT8_LAUNCH(4, kNL8s);
+            else WTB_INT8_LAUNCH(4, kNL8);
+        } else {
+            if (shrt) WTB_INT8_LAUNCH(kCPW8, kNL8s);
+            else WTB_INT8_LAUNCH(kCPW8, kNL8);
+        }
+#undef WTB_INT8_LAUNCH
         c.kclock.end(KernelClock::INT8X, c.stream, 8.0 * (double)ld * N, 8.0 * (double)N * N * nc);
         WTB_LAUNCH_CHECK();
         if (nch > 1) {

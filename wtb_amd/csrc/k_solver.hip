@@ -1266,6 +1266,13 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     // or FP64 MFMA arithmetic with WTB_REFINE=fp64).
     const char *renv = std::getenv("WTB_REFINE");
     const int acc = (renv && std::strcmp(renv, "fp64") == 0) ? WTB_FP64 : WTB_INT8X;
+    // The accurate step that ends the reduced-precision phase: with a
+    // tolerance of 1e-6 or above the int8 product's short form (10 instead of
+    // 13 digit-pair groups; its dropped pairs add ~2^-28 max|H| max|X| per
+    // term) certifies the residuals and fixes the energies to far below the
+    // tolerance; tighter tolerances, and the accurate phase that may follow,
+    // take the full product.
+    const int acc_step = (acc == WTB_INT8X && tol >= 1e-6) ? kInt8Short : acc;
     // one LOBPCG phase with products of precision mvp, until the first nev
     // pairs have relative residual < tol_ph, a reduced-precision stall, or
     // the iteration budget.
@@ -1341,7 +1348,7 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             bool orth = L.cholqr(na, W, nullptr, 1e-12);
             const bool last = merge && bb + na <= 80 && worst < 1e-4;
             if (last) {
-                L.apply(acc, bb + na, L.S.p, L.AS.p);   // X and W are contiguous in S
+                L.apply(acc_step, bb + na, L.S.p, L.AS.p);   // X and W are contiguous in S
                 mvcols += bb + na;
             } else {
                 L.apply(mvp, na, W, AW);
@@ -1417,7 +1424,7 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
         // product, H X on the nev wanted columns with the accurate product,
         // then an FP64 Rayleigh-Ritz step on them
         if (!merged) {
-            L.apply(acc, nev, L.S.p, L.AS.p);
+            L.apply(acc_step, nev, L.S.p, L.AS.p);
             mvcols += nev;
             L.rayleigh_ritz(nev, false, true, nev);
         }

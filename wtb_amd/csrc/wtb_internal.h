@@ -366,7 +366,10 @@ void matvec_noalloc(Ctx &c, int precision, int ncol, const cd *X, cd *Y);
 
 // Y = H_offdiag X on the FP32 store with exact int8 digit products (the
 // refinement product of the mixed-precision solver).
-void matvec_int8(Ctx &c, int ncol, const cd *X, cd *Y);
+void matvec_int8(Ctx &c, int ncol, const cd *X, cd *Y, int levels = 5);
+// solver-internal product id (never accepted at the C-ABI): the int8-digit
+// product in its short form (four significance levels)
+constexpr int kInt8Short = 103;
 
 // Hermitian eigensolver for small dense matrices on the GPU (FP64 Jacobi).
 // A (m x m column-major, lda) is destroyed; W receives ascending eigenvalues,
