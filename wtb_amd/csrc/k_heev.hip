@@ -1135,6 +1135,13 @@ cholesky_lds_kernel(int n, cd *__restrict__ Gg, int ldgg, double rel, int *__res
     }
     __syncthreads();
     if (s_fail) return;
+    // smallest pivot relative to its original diagonal (flag[2..3] as a double)
+    if (tid < 64) {
+        double m = 1e300;
+        for (int j = tid; j < n; j += 64) m = fmin(m, G[j + j * ln].x / d0[j]);
+        for (int off = 32; off > 0; off >>= 1) m = fmin(m, __shfl_xor(m, off, 64));
+        if (tid == 0) *reinterpret_cast<double *>(flag + 2) = m;
+    }
     // R: row j = (unscaled row j) / sqrt(g_jj), diagonal sqrt(g_jj)
     for (int idx = tid; idx < n * n; idx += kCT) {
         const int p = idx % n, q = idx / n;
@@ -1155,6 +1162,7 @@ cholesky_kernel(int n, cd *__restrict__ G, int ldg, double rel, int *__restrict_
     double *d0 = reinterpret_cast<double *>(smem);
     __shared__ int s_fail;
     const int tid = threadIdx.x;
+    double ratio = 1e300;      // smallest pivot relative to its diagonal (thread 0)
     for (int j = tid; j < n; j += kT) d0[j] = G[j + (size_t)j * ldg].x;
     if (tid == 0) {
         s_fail = 0;
@@ -1178,6 +1186,7 @@ cholesky_kernel(int n, cd *__restrict__ G, int ldg, double rel, int *__restrict_
         }
         __syncthreads();
         if (tid == 0) G[j + (size_t)j * ldg] = make_double2(r, 0.0);
+        if (tid == 0) ratio = fmin(ratio, piv / d0[j]);
         const int s = n - j - 1;
         for (int idx = tid; idx < s * s; idx += kT) {
             const int p = j + 1 + idx % s, q = j + 1 + idx / s;
@@ -1191,15 +1200,18 @@ cholesky_kernel(int n, cd *__restrict__ G, int ldg, double rel, int *__restrict_
         }
         __syncthreads();
     }
+    if (tid == 0 && !s_fail) *reinterpret_cast<double *>(flag + 2) = ratio;
 }
 
 }  // namespace
 
 // Returns 0 on success, or 1 + the first failing column.  With check false the
 // pivot test is not read back (no host synchronisation) and 0 is returned.
-int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel, bool check) {
+// With check and min_ratio, the smallest pivot relative to its diagonal entry
+// of G comes back in the same round trip (on success).
+int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel, bool check, double *min_ratio) {
     if (n > kT * 8) throw Error{WTB_ERR_UNSUPPORTED, "Cholesky size"};
-    c.ws_flag.alloc(1);
+    c.ws_flag.alloc(4);     // [0] failing column + 1, [2..3] the pivot ratio (double)
     const size_t d0 = (size_t)((n + 1) & ~1) * sizeof(double);
     if (n <= kCholLds)
         hipLaunchKernelGGL(cholesky_lds_kernel, dim3(1), dim3(kCT), d0 + (size_t)(n | 1) * n * sizeof(cd), c.stream, n, G,
@@ -1208,10 +1220,11 @@ int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel, bool check) {
         hipLaunchKernelGGL(cholesky_kernel, dim3(1), dim3(kT), d0, c.stream, n, G, ldg, rel, c.ws_flag.p);
     WTB_LAUNCH_CHECK();
     if (!check) return 0;
-    int flag = 0;
-    WTB_HIP_CHECK(hipMemcpyAsync(&flag, c.ws_flag.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    alignas(8) int hf[4] = {0, 0, 0, 0};
+    WTB_HIP_CHECK(hipMemcpyAsync(hf, c.ws_flag.p, sizeof(hf), hipMemcpyDeviceToHost, c.stream));
     WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
-    return flag;
+    if (min_ratio) *min_ratio = hf[0] ? 0.0 : *reinterpret_cast<const double *>(hf + 2);
+    return hf[0];
 }
 
 // Lowest nvec eigenpairs of the Hermitian matrix A (n x n, lda; destroyed).

@@ -1059,20 +1059,54 @@ struct Lobpcg {
     // B (N x nb) <- B R^-1 with G = B^H B = R^H R (and B2 <- B2 R^-1), two
     // passes (CholQR2).  False when a pivot falls below piv_rel: B is then
     // numerically rank deficient and left half-processed.
+    // The first pass leaves B R^-1 orthonormal to about eps cond(G), and with
+    // unit columns cond(G) follows the smallest pivot ratio rho (the squared
+    // distance of a column from the span of the ones before it): about
+    // 0.2 eps / rho^2 was measured (5e-12 at rho = 1.5e-3, 2e-10 at 3.6e-4).
+    // With loose set (a tolerance of 1e-6 or above, or reduced-precision
+    // products whose errors are far larger) the second pass is skipped when
+    // rho >= one_pass_loose (orthonormal to ~2e-9 at worst), otherwise when
+    // rho >= one_pass_tight.  The Rayleigh-Ritz step then sees residual
+    // components of that size inside the basis, far below the tolerance.
     double piv_rel = 1e-12;
-    bool cholqr(int nb, cd *B, cd *B2, double piv) {
+    double one_pass_loose = 1e-4, one_pass_tight = 0.05;
+    bool verbose = false;
+    bool cholqr(int nb, cd *B, cd *B2, double piv, bool loose = false) {
+        const double one_pass = loose ? one_pass_loose : one_pass_tight;
         for (int pass = 0; pass < 2; ++pass) {
             trr.begin();
             gram(c, nb, nb, N, B, (int)N, B, (int)N, Tmp.p, m3, true);
             // the second pass factors I + O(eps cond^2) and cannot fail once the
             // first has passed: only the first pass's pivot test is read back
-            int fail = cholesky_upper(c, nb, Tmp.p, m3, piv, pass == 0);
+            double ratio = 0.0;
+            int fail = cholesky_upper(c, nb, Tmp.p, m3, piv, pass == 0, &ratio);
             if (fail) {
                 trr.end();
                 return false;
             }
             trsm_right_upper(c, N, nb, Tmp.p, m3, B, B2);
             trr.end();
+            if (pass == 0) {
+                if (verbose) std::fprintf(stderr, "[wtb lobpcg] CholQR of %d columns: smallest pivot ratio %.3e\n", nb, ratio);
+                if (ratio >= one_pass) {
+                    if (verbose) {
+                        // the orthonormality the single pass left
+                        gram(c, nb, nb, N, B, (int)N, B, (int)N, Tmp.p, m3, true);
+                        std::vector<cd> g((size_t)m3 * nb);
+                        WTB_HIP_CHECK(hipMemcpyAsync(g.data(), Tmp.p, sizeof(cd) * g.size(), hipMemcpyDeviceToHost,
+                                                     c.stream));
+                        WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+                        double e = 0;
+                        for (int q = 0; q < nb; ++q)
+                            for (int p = 0; p <= q; ++p) {
+                                const cd v = g[p + (size_t)q * m3];
+                                e = std::max(e, std::hypot(v.x - (p == q ? 1.0 : 0.0), v.y));
+                            }
+                        std::fprintf(stderr, "[wtb lobpcg] one CholQR pass: max |B^H B - I| = %.3e\n", e);
+                    }
+                    break;
+                }
+            }
         }
         return true;
     }
@@ -1201,6 +1235,10 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     WTB_HIP_CHECK(hipEventCreate(&tr));
     WTB_HIP_CHECK(hipEventRecord(t0, c.stream));
     Lobpcg L(c, nev, b);
+    L.verbose = verbose;
+    // WTB_CHOLQR_ONE_PASS overrides the pivot ratio above which CholQR stops
+    // after one pass (above 1: always two passes)
+    if (const char *e = std::getenv("WTB_CHOLQR_ONE_PASS")) L.one_pass_loose = L.one_pass_tight = std::atof(e);
     // With FP32 products the projected matrix carries absolute errors of
     // ~1e-7 |H|; directions whose Gram eigenvalue is below 1e-6 of the largest
     // would amplify that noise beyond the spectral resolution, so they are
@@ -1343,9 +1381,11 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             L.project_out(bb, L.S.p, nullptr, na, W, nullptr);
             L.project_out(bb, L.S.p, nullptr, na, W, nullptr);
             L.normalize(na, W, nullptr);
-            // orthonormal W (CholQR2); a rank-deficient W falls back to SVQB
-            // (H W is formed afterwards from the orthonormal W: no error amplification)
-            bool orth = L.cholqr(na, W, nullptr, 1e-12);
+            // orthonormal W (CholQR2, or one pass when well conditioned); a
+            // rank-deficient W falls back to SVQB (H W is formed afterwards from
+            // the orthonormal W: no error amplification)
+            const bool reduced = mvp != WTB_FP64 && mvp != WTB_INT8X && mvp != kInt8Short;
+            bool orth = L.cholqr(na, W, nullptr, 1e-12, reduced || tol_ph >= 1e-6);
             const bool last = merge && bb + na <= 80 && worst < 1e-4;
             if (last) {
                 L.apply(acc_step, bb + na, L.S.p, L.AS.p);   // X and W are contiguous in S

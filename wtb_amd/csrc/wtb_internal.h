@@ -383,7 +383,7 @@ void heev_lowest(Ctx &c, int n, int nvec, cd *A, int lda, double *W, cd *Z, int 
 
 // In-place upper Cholesky factor G = R^H R (n x n, ldg).  Returns 0, or
 // 1 + the first column whose pivot falls below rel * its original diagonal.
-int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel, bool check = true);
+int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel, bool check = true, double *min_ratio = nullptr);
 
 // Copy H(i0:i0+ni, j0:j0+nj) of the stored matrix into out (ni x nj, ldo) as FP64.
 // gradients of sum_n w_n E_n (k_grad.hip)
