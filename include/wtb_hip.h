@@ -85,7 +85,8 @@ typedef struct {
     double t_solve_ms;       /* eigensolver wall time on the stream           */
     double t_refine_ms;      /* FP64 refinement part of t_solve_ms            */
     double t_matvec_ms;      /* time inside H*X kernels                       */
-    double t_rr_ms;          /* time in Rayleigh-Ritz / orthogonalisation     */
+    double t_rr_ms;          /* the rest: orthogonalisation, Rayleigh-Ritz,
+                                residual norms (t_solve_ms - t_matvec_ms)      */
     /* The solve's dominant matvec kernel (split-FP16 MFMA for WTB_MIXED, FP64
      * MFMA for WTB_FP64, FP32 MFMA for WTB_FP32), timed with HIP events on the
      * library stream around each launch: */

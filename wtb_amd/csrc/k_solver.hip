@@ -978,7 +978,9 @@ struct Lobpcg {
     DBuf<cd> &G, &A, &U, &T, &Tmp, &Cm;  // small m3 x m3
     DBuf<double> &D, &lam, &lamr, &norm2, &sc;
     DBuf<int> &idx, &act;
-    Timer tmv, trr;
+    // matvec time (the only per-segment events on the stream: each event
+    // marker between two launches idles the queue for a few microseconds)
+    Timer tmv;
     std::vector<double> h_lam;
 
     Lobpcg(Ctx &cx, int nev_, int b_)
@@ -989,7 +991,6 @@ struct Lobpcg {
         c.lws.hd.alloc(std::max<size_t>((size_t)N, 2 * (size_t)m3));
         c.lws.hi.alloc((size_t)m3);
         tmv.st = c.stream;
-        trr.st = c.stream;
         S.alloc((size_t)N * m3);
         AS.alloc((size_t)N * m3);
         T1.alloc((size_t)N * m3);       // trade places with S and AS
@@ -1038,10 +1039,8 @@ struct Lobpcg {
     // same combination is applied to B2 -= Q2 (Q^H B) when Q2 is given (keeps
     // B2 = H B).  FP64 classical Gram-Schmidt.
     void project_out(int q, const cd *Q, const cd *Q2, int nb, cd *B, cd *B2) {
-        trr.begin();
         gram(c, q, nb, N, Q, (int)N, B, (int)N, Tmp.p, m3);
         gemm_pair(c, N, nb, q, Q, Q2, N, Tmp.p, m3, B, B2, N, true);
-        trr.end();
     }
 
     // scale columns of B (and B2 with the same factors) to unit norm, on the device
@@ -1074,18 +1073,15 @@ struct Lobpcg {
     bool cholqr(int nb, cd *B, cd *B2, double piv, bool loose = false) {
         const double one_pass = loose ? one_pass_loose : one_pass_tight;
         for (int pass = 0; pass < 2; ++pass) {
-            trr.begin();
             gram(c, nb, nb, N, B, (int)N, B, (int)N, Tmp.p, m3, true);
             // the second pass factors I + O(eps cond^2) and cannot fail once the
             // first has passed: only the first pass's pivot test is read back
             double ratio = 0.0;
             int fail = cholesky_upper(c, nb, Tmp.p, m3, piv, pass == 0, &ratio);
             if (fail) {
-                trr.end();
                 return false;
             }
             trsm_right_upper(c, N, nb, Tmp.p, m3, B, B2);
-            trr.end();
             if (pass == 0) {
                 if (verbose) std::fprintf(stderr, "[wtb lobpcg] CholQR of %d columns: smallest pivot ratio %.3e\n", nb, ratio);
                 if (ratio >= one_pass) {
@@ -1121,7 +1117,6 @@ struct Lobpcg {
     void rayleigh_ritz(int m, bool withP, bool orth = true, int nout = 0) {
         const int nb = nout > 0 ? nout : b;
         if (withP && nb != b) throw Error{WTB_ERR_STATE, "partial Rayleigh-Ritz with P"};
-        trr.begin();
         gram(c, m, m, N, S.p, (int)N, AS.p, (int)N, A.p, m3, true);
         if (orth) {
             heev_lowest(c, m, nb, A.p, m3, lamr.p, Cm.p, m3);
@@ -1165,7 +1160,6 @@ struct Lobpcg {
             WTB_HIP_CHECK(hipMemcpyAsync(S.p, T1.p, sizeof(cd) * N * nb, hipMemcpyDeviceToDevice, c.stream));
             WTB_HIP_CHECK(hipMemcpyAsync(AS.p, T2.p, sizeof(cd) * N * nb, hipMemcpyDeviceToDevice, c.stream));
         }
-        trr.end();
     }
 };
 
@@ -1503,7 +1497,7 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
         rep->t_solve_ms = elapsed_ms(t0, t1);
         rep->t_refine_ms = precision != WTB_FP64 ? elapsed_ms(tr, t1) : 0.0;
         rep->t_matvec_ms = L.tmv.total();
-        rep->t_rr_ms = L.trr.total();
+        rep->t_rr_ms = std::max(0.0, rep->t_solve_ms - rep->t_matvec_ms);
         const KernelClock &k = c.kclock;
         const KernelClock::Kind main = c.bse.precision == WTB_FP64 ? KernelClock::F64_ON_F64
                                        : precision == WTB_MIXED    ? KernelClock::F16SPLIT
