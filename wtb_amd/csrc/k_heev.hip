@@ -1227,6 +1227,17 @@ int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel, bool check, double
     return hf[0];
 }
 
+// cholesky_upper without a host round trip: the status words (failing column
+// + 1, then the smallest pivot ratio as a double) are copied to the pinned
+// host words hflag[0..3] and ev is recorded after the copy.  Returns the
+// device status word, which a conditional triangular solve reads.
+const int *cholesky_upper_async(Ctx &c, int n, cd *G, int ldg, double rel, int *hflag, hipEvent_t ev) {
+    cholesky_upper(c, n, G, ldg, rel, false);
+    WTB_HIP_CHECK(hipMemcpyAsync(hflag, c.ws_flag.p, 4 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    WTB_HIP_CHECK(hipEventRecord(ev, c.stream));
+    return c.ws_flag.p;
+}
+
 // Lowest nvec eigenpairs of the Hermitian matrix A (n x n, lda; destroyed).
 void heev_lowest(Ctx &c, int n, int nvec, cd *A, int lda, double *W, cd *Z, int ldz) {
     if (n <= 0 || nvec <= 0) return;

@@ -663,8 +663,9 @@ void gram(Ctx &c, int m, int n, long K, const cd *A, int lda, const cd *B, int l
 constexpr int kTr4 = 256;
 __global__ void __launch_bounds__(kTr4)
 trsm_ru4_kernel(long N, int nb, const cd *__restrict__ R, int ldr, cd *__restrict__ B, cd *__restrict__ B2,
-                long ldb) {
+                long ldb, const int *__restrict__ skip) {
     constexpr int JB = 16;
+    if (skip && *skip) return;      // the factorisation failed: B stays as it is
     extern __shared__ __align__(16) unsigned char smem[];
     cd *rp = reinterpret_cast<cd *>(smem);                       // packed upper R
     double *rinv = reinterpret_cast<double *>(rp + (size_t)nb * (nb + 1) / 2);
@@ -765,9 +766,11 @@ trsm_ru4_kernel(long N, int nb, const cd *__restrict__ R, int ldr, cd *__restric
     }
 }
 
-void trsm_right_upper(Ctx &c, long N, int nb, const cd *R, int ldr, cd *B, cd *B2) {
+// skip (device, may be null): a nonzero value leaves B and B2 unchanged.
+void trsm_right_upper(Ctx &c, long N, int nb, const cd *R, int ldr, cd *B, cd *B2, const int *skip = nullptr) {
     const size_t sh = (size_t)nb * (nb + 1) / 2 * sizeof(cd) + (size_t)nb * sizeof(double);
     if (sh > 150 * 1024) {      // R too large to stage: library triangular solve
+        if (skip) throw Error{WTB_ERR_STATE, "conditional triangular solve of a large factor"};
         for (cd *M : {B, B2})
             if (M)
                 WTB_BLAS_CHECK(rocblas_ztrsm(c.blas, rocblas_side_right, rocblas_fill_upper, rocblas_operation_none,
@@ -776,7 +779,7 @@ void trsm_right_upper(Ctx &c, long N, int nb, const cd *R, int ldr, cd *B, cd *B
         return;
     }
     hipLaunchKernelGGL(trsm_ru4_kernel, dim3((unsigned)((N + kTr4 / 4 - 1) / (kTr4 / 4)), B2 ? 2 : 1), dim3(kTr4),
-                       sh, c.stream, N, nb, R, ldr, B, B2, N);
+                       sh, c.stream, N, nb, R, ldr, B, B2, N, skip);
     WTB_LAUNCH_CHECK();
 }
 
@@ -981,6 +984,7 @@ struct Lobpcg {
     // matvec time (the only per-segment events on the stream: each event
     // marker between two launches idles the queue for a few microseconds)
     Timer tmv;
+    hipEvent_t ev_chol = nullptr;      // the first CholQR factor's status has reached the host
     std::vector<double> h_lam;
 
     Lobpcg(Ctx &cx, int nev_, int b_)
@@ -990,6 +994,8 @@ struct Lobpcg {
           idx(cx.lws.idx), act(cx.lws.act) {
         c.lws.hd.alloc(std::max<size_t>((size_t)N, 2 * (size_t)m3));
         c.lws.hi.alloc((size_t)m3);
+        c.lws.hflag.alloc(4);
+        WTB_HIP_CHECK(hipEventCreateWithFlags(&ev_chol, hipEventDisableTiming));
         tmv.st = c.stream;
         S.alloc((size_t)N * m3);
         AS.alloc((size_t)N * m3);
@@ -1011,6 +1017,12 @@ struct Lobpcg {
         idx.alloc(m3);
         act.alloc(m3);
     }
+
+    ~Lobpcg() {
+        if (ev_chol) (void)hipEventDestroy(ev_chol);
+    }
+    Lobpcg(const Lobpcg &) = delete;
+    Lobpcg &operator=(const Lobpcg &) = delete;
 
     dim3 gN(int cols) const { return dim3((unsigned)((N + 255) / 256), (unsigned)cols); }
 
@@ -1104,6 +1116,53 @@ struct Lobpcg {
                 }
             }
         }
+        return true;
+    }
+
+    // CholQR of W without a host round trip before its product: the first
+    // pass (Gram, factor, W R^-1 unless the factor failed, when W stays as it
+    // is) is queued here; its status reaches the pinned host words while the
+    // product that follows runs, and cholqr_finish reads it.  Needs a factor
+    // the triangular-solve kernel can stage (cholqr_async_ok).
+    bool cholqr_async_ok(int nb) const { return (size_t)nb * (nb + 1) / 2 * sizeof(cd) + nb * sizeof(double) <= 150 * 1024; }
+    void cholqr_begin(int nb, cd *B, double piv) {
+        gram(c, nb, nb, N, B, (int)N, B, (int)N, Tmp.p, m3, true);
+        const int *df = cholesky_upper_async(c, nb, Tmp.p, m3, piv, c.lws.hflag.p, ev_chol);
+        trsm_right_upper(c, N, nb, Tmp.p, m3, B, nullptr, df);
+    }
+    // False when the first factor failed (B numerically rank deficient, B and
+    // B2 untouched).  Otherwise the second pass, when the pivot ratio asks
+    // for it, is applied to B and to its product B2 = H B alike (R^-1 of the
+    // second pass is I + O(eps cond): the product's error is not amplified).
+    bool cholqr_finish(int nb, cd *B, cd *B2, bool loose) {
+        WTB_HIP_CHECK(hipEventSynchronize(ev_chol));
+        const int *hf = c.lws.hflag.p;
+        if (hf[0]) return false;
+        double ratio;
+        std::memcpy(&ratio, hf + 2, sizeof(double));
+        const double one_pass = loose ? one_pass_loose : one_pass_tight;
+        if (verbose) std::fprintf(stderr, "[wtb lobpcg] CholQR of %d columns: smallest pivot ratio %.3e\n", nb, ratio);
+        if (ratio >= one_pass) {
+            if (verbose) {
+                // the orthonormality the single pass left
+                gram(c, nb, nb, N, B, (int)N, B, (int)N, Tmp.p, m3, true);
+                std::vector<cd> g((size_t)m3 * nb);
+                WTB_HIP_CHECK(
+                    hipMemcpyAsync(g.data(), Tmp.p, sizeof(cd) * g.size(), hipMemcpyDeviceToHost, c.stream));
+                WTB_HIP_CHECK(hipStreamSynchronize(c.stream));
+                double e = 0;
+                for (int q = 0; q < nb; ++q)
+                    for (int p = 0; p <= q; ++p) {
+                        const cd v = g[p + (size_t)q * m3];
+                        e = std::max(e, std::hypot(v.x - (p == q ? 1.0 : 0.0), v.y));
+                    }
+                std::fprintf(stderr, "[wtb lobpcg] one CholQR pass: max |B^H B - I| = %.3e\n", e);
+            }
+            return true;
+        }
+        gram(c, nb, nb, N, B, (int)N, B, (int)N, Tmp.p, m3, true);
+        cholesky_upper(c, nb, Tmp.p, m3, piv_rel, false);
+        trsm_right_upper(c, N, nb, Tmp.p, m3, B, B2);
         return true;
     }
 
@@ -1376,10 +1435,16 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             L.project_out(bb, L.S.p, nullptr, na, W, nullptr);
             L.normalize(na, W, nullptr);
             // orthonormal W (CholQR2, or one pass when well conditioned); a
-            // rank-deficient W falls back to SVQB (H W is formed afterwards from
-            // the orthonormal W: no error amplification)
+            // rank-deficient W falls back to SVQB.  H W is formed from the
+            // once-orthonormalised W (no error amplification); the first pass's
+            // status is read while the product runs, and a second pass, when
+            // needed, transforms W and H W alike.
             const bool reduced = mvp != WTB_FP64 && mvp != WTB_INT8X && mvp != kInt8Short;
-            bool orth = L.cholqr(na, W, nullptr, 1e-12, reduced || tol_ph >= 1e-6);
+            const bool loose = reduced || tol_ph >= 1e-6;
+            const bool async_qr = L.cholqr_async_ok(na);
+            bool orth = true;
+            if (async_qr) L.cholqr_begin(na, W, 1e-12);
+            else orth = L.cholqr(na, W, nullptr, 1e-12, loose);
             const bool last = merge && bb + na <= 80 && worst < 1e-4;
             if (last) {
                 L.apply(acc_step, bb + na, L.S.p, L.AS.p);   // X and W are contiguous in S
@@ -1388,6 +1453,7 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
                 L.apply(mvp, na, W, AW);
                 mvcols += na;
             }
+            if (async_qr) orth = L.cholqr_finish(na, W, AW, loose);
             int m = bb + na;
             if (haveP && useP) {
                 // P orthogonal to X and W; H P follows the same combinations

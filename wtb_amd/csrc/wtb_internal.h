@@ -109,6 +109,7 @@ struct SolverWs {
     DBuf<int> idx, act;
     HBuf<double> hd;                // pinned: diagonal, residual norms, Ritz values
     HBuf<int> hi;                   // pinned: active column indices
+    HBuf<int> hflag;                // pinned: status words of the CholQR factor
 };
 
 struct Model {
@@ -384,6 +385,7 @@ void heev_lowest(Ctx &c, int n, int nvec, cd *A, int lda, double *W, cd *Z, int 
 // In-place upper Cholesky factor G = R^H R (n x n, ldg).  Returns 0, or
 // 1 + the first column whose pivot falls below rel * its original diagonal.
 int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel, bool check = true, double *min_ratio = nullptr);
+const int *cholesky_upper_async(Ctx &c, int n, cd *G, int ldg, double rel, int *hflag, hipEvent_t ev);
 
 // Copy H(i0:i0+ni, j0:j0+nj) of the stored matrix into out (ni x nj, ldo) as FP64.
 // gradients of sum_n w_n E_n (k_grad.hip)
