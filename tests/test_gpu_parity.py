@@ -298,6 +298,41 @@ def test_solve_lobpcg(precision):
     ctx.close()
 
 
+@pytest.mark.parametrize("precision", [hip.FP64, hip.MIXED])
+@pytest.mark.parametrize("mode", ["two-pass", "fallback"])
+def test_solve_lobpcg_cholqr_paths(precision, mode, monkeypatch):
+    """The block solver's CholQR of W off its usual path: "two-pass" forces
+    the second pass, which runs after the product and transforms W and H W
+    alike; "fallback" raises the pivot floor so that every first factor
+    fails, the triangular solve skips itself on the device and the step takes
+    the SVQB Rayleigh-Ritz on the untouched W.  Both must still give the
+    oracle's lowest eigenpairs."""
+    say("case lobpcg cholqr", mode, precision)
+    if mode == "two-pass":
+        monkeypatch.setenv("WTB_CHOLQR_ONE_PASS", "2")
+    else:
+        monkeypatch.setenv("WTB_CHOLQR_PIV", "0.999")
+    mesh = (20, 20, 1)
+    m = hex_model(mesh)
+    ctx, k, eig, vec, nocp, ref = setup(m, mesh, 2, 2)
+    ctx.bse_build(COUL, precision)
+    E, X, rep = ctx.bse_solve(32, tol=1e-8)
+    say("report", rep)
+    Hr = O.bse_matrix(k, ref["eig"], ref["vec"], O.coulomb(**COUL), mesh, m.rlat, 2, 2)
+    w = np.linalg.eigvalsh(Hr)[:32]
+    err = np.abs(E - w).max()
+    say("max |dE| eV", err)
+    assert rep["nconv"] == 32
+    if precision == hip.FP64:
+        assert err < 1e-9
+        r = np.linalg.norm(Hr @ X - X * E, axis=0) / np.abs(E)
+        assert r.max() < 1e-8
+    else:
+        assert err < 1e-4          # 0.1 meV
+    assert np.allclose(X.conj().T @ X, np.eye(32), atol=1e-8)
+    ctx.close()
+
+
 def test_optics_and_oscillator():
     say("case optics")
     mesh = (8, 8, 1)

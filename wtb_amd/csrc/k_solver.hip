@@ -1292,6 +1292,11 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     // WTB_CHOLQR_ONE_PASS overrides the pivot ratio above which CholQR stops
     // after one pass (above 1: always two passes)
     if (const char *e = std::getenv("WTB_CHOLQR_ONE_PASS")) L.one_pass_loose = L.one_pass_tight = std::atof(e);
+    // pivot floor of W's CholQR factor (relative to the diagonal); a factor
+    // below it sends the step to the SVQB Rayleigh-Ritz.  WTB_CHOLQR_PIV
+    // raises it (tests of that fallback)
+    double piv_w = 1e-12;
+    if (const char *e = std::getenv("WTB_CHOLQR_PIV")) piv_w = std::atof(e);
     // With FP32 products the projected matrix carries absolute errors of
     // ~1e-7 |H|; directions whose Gram eigenvalue is below 1e-6 of the largest
     // would amplify that noise beyond the spectral resolution, so they are
@@ -1443,8 +1448,8 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             const bool loose = reduced || tol_ph >= 1e-6;
             const bool async_qr = L.cholqr_async_ok(na);
             bool orth = true;
-            if (async_qr) L.cholqr_begin(na, W, 1e-12);
-            else orth = L.cholqr(na, W, nullptr, 1e-12, loose);
+            if (async_qr) L.cholqr_begin(na, W, piv_w);
+            else orth = L.cholqr(na, W, nullptr, piv_w, loose);
             const bool last = merge && bb + na <= 80 && worst < 1e-4;
             if (last) {
                 L.apply(acc_step, bb + na, L.S.p, L.AS.p);   // X and W are contiguous in S
