@@ -1224,6 +1224,10 @@ struct Lobpcg {
 
 }  // namespace
 
+// Squared norm a unit column must keep through one Gram-Schmidt projection
+// against X for that pass to be final (orthogonal to X to ~eps 1e4).
+constexpr double kReorth = 1e-8;
+
 void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *rep) {
     const long N = c.bse.N;
     if (nev < 1 || nev > N) throw Error{WTB_ERR_ARG, "nev out of range"};
@@ -1433,12 +1437,26 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             hipLaunchKernelGGL(precondition, L.gN(na), dim3(256), 0, c.stream, N, L.S.p, L.AS.p, L.lam.p,
                                c.bse.diag.p, L.act.p, L.sc.p, W, prec_floor);
             WTB_LAUNCH_CHECK();
-            // W orthogonal to X (two classical Gram-Schmidt passes in FP64), unit
-            // columns, then its image H W
+            // W orthogonal to X (classical Gram-Schmidt in FP64), unit columns,
+            // then its image H W.  One pass leaves X^H w at ~eps |w0| / |w1| for
+            // a unit column w0 and its projection w1: the second pass runs only
+            // when some column kept less than kReorth of its squared norm
+            // (then ~1e-12 or worse; CGS2 brings it back to eps).
             L.normalize(na, W, nullptr);
             L.project_out(bb, L.S.p, nullptr, na, W, nullptr);
-            L.project_out(bb, L.S.p, nullptr, na, W, nullptr);
-            L.normalize(na, W, nullptr);
+            {
+                const std::vector<double> pn = L.norms(na, nullptr, W, nullptr);
+                const double kept = *std::min_element(pn.begin(), pn.end());
+                if (verbose) std::fprintf(stderr, "[wtb lobpcg] projection kept a squared norm of %.3e at least\n", kept);
+                if (kept < kReorth) {
+                    L.project_out(bb, L.S.p, nullptr, na, W, nullptr);
+                    L.normalize(na, W, nullptr);
+                } else {
+                    // the norms just read are those of the projected W
+                    hipLaunchKernelGGL(unit_cols, L.gN(na), dim3(256), 0, c.stream, N, L.norm2.p, W);
+                    WTB_LAUNCH_CHECK();
+                }
+            }
             // orthonormal W (CholQR2, or one pass when well conditioned); a
             // rank-deficient W falls back to SVQB.  H W is formed from the
             // once-orthonormalised W (no error amplification); the first pass's
