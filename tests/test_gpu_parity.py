@@ -299,19 +299,23 @@ def test_solve_lobpcg(precision):
 
 
 @pytest.mark.parametrize("precision", [hip.FP64, hip.MIXED])
-@pytest.mark.parametrize("mode", ["two-pass", "fallback"])
+@pytest.mark.parametrize("mode", ["two-pass", "fallback", "reorth"])
 def test_solve_lobpcg_cholqr_paths(precision, mode, monkeypatch):
-    """The block solver's CholQR of W off its usual path: "two-pass" forces
-    the second pass, which runs after the product and transforms W and H W
-    alike; "fallback" raises the pivot floor so that every first factor
-    fails, the triangular solve skips itself on the device and the step takes
-    the SVQB Rayleigh-Ritz on the untouched W.  Both must still give the
+    """The block solver's orthogonalisation of W off its usual path:
+    "two-pass" forces the second CholQR pass, which runs after the product
+    and transforms W and H W alike; "fallback" raises the pivot floor so that
+    every first factor fails, the triangular solve skips itself on the device
+    and the step takes the SVQB Rayleigh-Ritz on the untouched W; "reorth"
+    forces the second Gram-Schmidt pass against X, applied after the product
+    to W and H W with H X, and CholQR again.  All must still give the
     oracle's lowest eigenpairs."""
     say("case lobpcg cholqr", mode, precision)
     if mode == "two-pass":
         monkeypatch.setenv("WTB_CHOLQR_ONE_PASS", "2")
-    else:
+    elif mode == "fallback":
         monkeypatch.setenv("WTB_CHOLQR_PIV", "0.999")
+    else:
+        monkeypatch.setenv("WTB_REORTH", "2")
     mesh = (20, 20, 1)
     m = hex_model(mesh)
     ctx, k, eig, vec, nocp, ref = setup(m, mesh, 2, 2)

@@ -985,6 +985,7 @@ struct Lobpcg {
     // marker between two launches idles the queue for a few microseconds)
     Timer tmv;
     hipEvent_t ev_chol = nullptr;      // the first CholQR factor's status has reached the host
+    hipEvent_t ev_kept = nullptr;      // the projected W's column norms have reached the host
     std::vector<double> h_lam;
 
     Lobpcg(Ctx &cx, int nev_, int b_)
@@ -996,6 +997,8 @@ struct Lobpcg {
         c.lws.hi.alloc((size_t)m3);
         c.lws.hflag.alloc(4);
         WTB_HIP_CHECK(hipEventCreateWithFlags(&ev_chol, hipEventDisableTiming));
+        WTB_HIP_CHECK(hipEventCreateWithFlags(&ev_kept, hipEventDisableTiming));
+        c.lws.hkept.alloc((size_t)m3);
         tmv.st = c.stream;
         S.alloc((size_t)N * m3);
         AS.alloc((size_t)N * m3);
@@ -1020,6 +1023,7 @@ struct Lobpcg {
 
     ~Lobpcg() {
         if (ev_chol) (void)hipEventDestroy(ev_chol);
+        if (ev_kept) (void)hipEventDestroy(ev_kept);
     }
     Lobpcg(const Lobpcg &) = delete;
     Lobpcg &operator=(const Lobpcg &) = delete;
@@ -1301,6 +1305,10 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
     // raises it (tests of that fallback)
     double piv_w = 1e-12;
     if (const char *e = std::getenv("WTB_CHOLQR_PIV")) piv_w = std::atof(e);
+    // squared norm below which a projected column takes the second
+    // Gram-Schmidt pass (WTB_REORTH overrides it; above 1: always)
+    double reorth = kReorth;
+    if (const char *e = std::getenv("WTB_REORTH")) reorth = std::atof(e);
     // With FP32 products the projected matrix carries absolute errors of
     // ~1e-7 |H|; directions whose Gram eigenvalue is below 1e-6 of the largest
     // would amplify that noise beyond the spectral resolution, so they are
@@ -1442,21 +1450,18 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
             // a unit column w0 and its projection w1: the second pass runs only
             // when some column kept less than kReorth of its squared norm
             // (then ~1e-12 or worse; CGS2 brings it back to eps).
+            // The projected W's column norms travel to the host behind the
+            // projection and are read after the product: the rare second pass
+            // then transforms W and H W alike (H X known, H W linear in W).
             L.normalize(na, W, nullptr);
             L.project_out(bb, L.S.p, nullptr, na, W, nullptr);
-            {
-                const std::vector<double> pn = L.norms(na, nullptr, W, nullptr);
-                const double kept = *std::min_element(pn.begin(), pn.end());
-                if (verbose) std::fprintf(stderr, "[wtb lobpcg] projection kept a squared norm of %.3e at least\n", kept);
-                if (kept < kReorth) {
-                    L.project_out(bb, L.S.p, nullptr, na, W, nullptr);
-                    L.normalize(na, W, nullptr);
-                } else {
-                    // the norms just read are those of the projected W
-                    hipLaunchKernelGGL(unit_cols, L.gN(na), dim3(256), 0, c.stream, N, L.norm2.p, W);
-                    WTB_LAUNCH_CHECK();
-                }
-            }
+            hipLaunchKernelGGL(resid_norms, dim3(na), dim3(kNT), 0, c.stream, N, nullptr, W, nullptr, L.norm2.p);
+            WTB_LAUNCH_CHECK();
+            WTB_HIP_CHECK(hipMemcpyAsync(c.lws.hkept.p, L.norm2.p, sizeof(double) * na, hipMemcpyDeviceToHost,
+                                         c.stream));
+            WTB_HIP_CHECK(hipEventRecord(L.ev_kept, c.stream));
+            hipLaunchKernelGGL(unit_cols, L.gN(na), dim3(256), 0, c.stream, N, L.norm2.p, W);
+            WTB_LAUNCH_CHECK();
             // orthonormal W (CholQR2, or one pass when well conditioned); a
             // rank-deficient W falls back to SVQB.  H W is formed from the
             // once-orthonormalised W (no error amplification); the first pass's
@@ -1477,6 +1482,19 @@ void lobpcg(Ctx &c, int nev, double tol, int max_it, int precision, wtb_report *
                 mvcols += na;
             }
             if (async_qr) orth = L.cholqr_finish(na, W, AW, loose);
+            {
+                WTB_HIP_CHECK(hipEventSynchronize(L.ev_kept));
+                const double *pn = c.lws.hkept.p;
+                const double kept = *std::min_element(pn, pn + na);
+                if (verbose) std::fprintf(stderr, "[wtb lobpcg] projection kept a squared norm of %.3e at least\n", kept);
+                if (kept < reorth) {
+                    // second Gram-Schmidt pass on W and H W (H X = AS[:, :bb]), then
+                    // their CholQR again
+                    L.project_out(bb, L.S.p, L.AS.p, na, W, AW);
+                    L.normalize(na, W, AW);
+                    orth = L.cholqr(na, W, AW, piv_w, loose);
+                }
+            }
             int m = bb + na;
             if (haveP && useP) {
                 // P orthogonal to X and W; H P follows the same combinations

@@ -110,6 +110,7 @@ struct SolverWs {
     HBuf<double> hd;                // pinned: diagonal, residual norms, Ritz values
     HBuf<int> hi;                   // pinned: active column indices
     HBuf<int> hflag;                // pinned: status words of the CholQR factor
+    HBuf<double> hkept;             // pinned: column norms of the projected search block
 };
 
 struct Model {
