@@ -46,52 +46,69 @@ __device__ __forceinline__ double lane_get(double v, int i) {
 // v with lane i replaced by x
 __device__ __forceinline__ double lane_set(double v, int i, double x) { return (int)threadIdx.x == i ? x : v; }
 
-// 1 + 2: H(k) and its reduction T = Q^H H(k) Q.  Q -> qout[k] ([o*n + j]);
-// the diagonal and the off-diagonal of T -> tri[k] at 0 and kMaxOrb.
+// 1: H(k) = sum_R e^{ik.R} H(R) / deg(R) for kKF k-points per workgroup, one
+// thread per matrix entry: every entry accumulates over R in the reference's
+// order (the sums and their order of a per-k loop), and each H(R) element is
+// read once for the kKF k-points.  hk[k][r*n + s].
+constexpr int kKF = 16, kFT = 256, kFR = 64;
+__global__ void __launch_bounds__(kFT)
+hk_fourier_kernel(int nk, int n, int nvec, int flags, const double *__restrict__ kpt, const cd *__restrict__ hop,
+                  const double *__restrict__ rcart, const double *__restrict__ invdeg, cd *__restrict__ hk) {
+    __shared__ cd ph[kKF][kFR];
+    const int nn = n * n, tid = threadIdx.x;
+    const int k0 = blockIdx.x * kKF, idx = blockIdx.y * kFT + tid;
+    const bool compat = flags & WTB_FLAG_COMPAT_F32;
+    cd acc[kKF];
+#pragma unroll
+    for (int kk = 0; kk < kKF; ++kk) acc[kk] = make_double2(0.0, 0.0);
+    for (int r0 = 0; r0 < nvec; r0 += kFR) {
+        const int rend = min(kFR, nvec - r0);
+        __syncthreads();      // the phases of the previous chunk are consumed
+        for (int t = tid; t < kKF * kFR; t += kFT) {
+            const int kk = t / kFR, rr = t % kFR, k = k0 + kk;
+            if (k < nk && rr < rend) {
+                double x = kdotr(kpt[3 * k], kpt[3 * k + 1], kpt[3 * k + 2], rcart + 3 * (r0 + rr));
+                if (compat) x = (double)(float)x;
+                double sn, cs;
+                sincos(x, &sn, &cs);
+                ph[kk][rr] = make_double2(cs * invdeg[r0 + rr], sn * invdeg[r0 + rr]);
+            }
+        }
+        __syncthreads();
+        if (idx < nn) {
+            const cd *hb = hop + (size_t)r0 * nn + idx;
+            for (int rr = 0; rr < rend; ++rr) {
+                const cd h = hb[(size_t)rr * nn];
+#pragma unroll
+                for (int kk = 0; kk < kKF; ++kk) acc[kk] = cadd(acc[kk], cmul(ph[kk][rr], h));
+            }
+        }
+    }
+    if (idx < nn) {
+#pragma unroll
+        for (int kk = 0; kk < kKF; ++kk)
+            if (k0 + kk < nk) hk[(size_t)(k0 + kk) * nn + idx] = acc[kk];
+    }
+}
+
+// 2: the reduction T = Q^H H(k) Q of H(k) from hk[k].  Q -> qout[k]
+// ([o*n + j]); the diagonal and the off-diagonal of T -> tri[k] at 0 and
+// kMaxOrb.
 __global__ void __launch_bounds__(kLanes)
-hk_house_kernel(int n, int nvec, int flags, double efermi, const double *__restrict__ kpt,
-                const cd *__restrict__ hop, const double *__restrict__ rcart, const double *__restrict__ invdeg,
-                cd *__restrict__ qout, double *__restrict__ tri) {
+hk_house_kernel(int n, int flags, double efermi, const cd *__restrict__ hk, cd *__restrict__ qout,
+                double *__restrict__ tri) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int k = blockIdx.x;
     const int tid = threadIdx.x;
     const int nn = n * n;
     cd *A = reinterpret_cast<cd *>(smem);            // [r*n + s]
     cd *V = A + nn;                                  // Q, [o*n + j]
-    cd *ph = V + nn;                                 // kLanes phases
-    cd *vb = ph + kLanes;                            // 32: Householder vector
+    cd *vb = V + nn;                                 // 32: Householder vector
     cd *yb = vb + kMaxOrb;                           // 32: A v
     cd *wb = yb + kMaxOrb;                           // 32: w = tau y - |tau|^2 (v^H y) / 2 v
     const bool compat = flags & WTB_FLAG_COMPAT_F32;
-    const double kx = kpt[3 * k], ky = kpt[3 * k + 1], kz = kpt[3 * k + 2];
-
-    // ---- 1. Fourier sum ----
-    // Entry idx = tid + 64 s accumulates over R in the reference's order; the
-    // partial sums wait in A between chunks of 64 R, and within a chunk the
-    // loop over R is unrolled so that several H(R) loads are in flight.
-    const int ns = (nn + kLanes - 1) / kLanes;
-    for (int idx = tid; idx < nn; idx += kLanes) A[idx] = make_double2(0.0, 0.0);
-    for (int r0 = 0; r0 < nvec; r0 += kLanes) {
-        int r = r0 + tid;
-        if (r < nvec) {
-            double x = kdotr(kx, ky, kz, rcart + 3 * r);
-            if (compat) x = (double)(float)x;
-            double sn, cs;
-            sincos(x, &sn, &cs);
-            ph[tid] = make_double2(cs * invdeg[r], sn * invdeg[r]);
-        }
-        __syncthreads();
-        const int rend = min(kLanes, nvec - r0);
-        for (int s = 0; s < ns; ++s) {
-            const int idx = tid + s * kLanes, ic = min(idx, nn - 1);
-            const cd *hb = hop + (size_t)r0 * nn + ic;
-            cd acc = A[ic];
-#pragma unroll 8
-            for (int rr = 0; rr < rend; ++rr) acc = cadd(acc, cmul(ph[rr], hb[(size_t)rr * nn]));
-            if (idx < nn) A[idx] = acc;
-        }
-        __syncthreads();
-    }
+    for (int idx = tid; idx < nn; idx += kLanes) A[idx] = hk[(size_t)k * nn + idx];
+    __syncthreads();
     const double ef = compat ? (double)(float)efermi : efermi;
     if (tid < n) A[tid * n + tid].x -= ef;
     __syncthreads();
@@ -521,13 +538,18 @@ void hk_eig_on(Ctx &c, const double *kpt, int nk, int nv, int nc, Bands &b) {
     b.rix.alloc((size_t)nk * cap);
     b.nrot.alloc(nk);
     b.zrot.alloc((size_t)nk * n * n);
-    const size_t sh_house = 2 * (size_t)n * n * sizeof(cd) + (kLanes + 3 * kMaxOrb) * sizeof(cd);
+    const size_t sh_house = 2 * (size_t)n * n * sizeof(cd) + 3 * kMaxOrb * sizeof(cd);
+    b.hk.alloc((size_t)nk * n * n);
     const size_t sh_vec = (size_t)n * n * (sizeof(cd) + sizeof(double)) + kMaxOrb * (sizeof(double) + sizeof(int));
     // lane_ql: the QL on one lane per k with a rotation log replayed by
     // hk_rot_kernel; otherwise one wavefront per k rotates Z in place
     auto run = [&](bool lane_ql) {
-        hipLaunchKernelGGL(hk_house_kernel, dim3(nk), dim3(kLanes), sh_house, c.stream, n, m.nvec, m.flags,
-                           m.efermi, kpt, m.hop.p, m.rcart.p, m.invdeg.p, b.uall.p, b.tri.p);
+        hipLaunchKernelGGL(hk_fourier_kernel, dim3((unsigned)((nk + kKF - 1) / kKF), (unsigned)((n * n + kFT - 1) / kFT)),
+                           dim3(kFT), 0, c.stream, nk, n, m.nvec, m.flags, kpt, m.hop.p, m.rcart.p, m.invdeg.p,
+                           b.hk.p);
+        WTB_LAUNCH_CHECK();
+        hipLaunchKernelGGL(hk_house_kernel, dim3(nk), dim3(kLanes), sh_house, c.stream, n, m.flags, m.efermi, b.hk.p,
+                           b.uall.p, b.tri.p);
         WTB_LAUNCH_CHECK();
         if (lane_ql) {
             hipLaunchKernelGGL(hk_ql_lane_kernel, dim3((unsigned)((nk + kQLLanes - 1) / kQLLanes)), dim3(kQLLanes), 0,

@@ -128,6 +128,7 @@ struct Bands {
     int nk = 0, nv = 0, nc = 0, nb = 0;
     DBuf<double> eall;   // [k][n] all bands, scissor applied
     DBuf<cd> uall;       // [k][o][n] component o of band n
+    DBuf<cd> hk;         // [k][r*n + s] H(k) of the eigensolve (before the Fermi shift)
     DBuf<int> nocp;      // [k]
     DBuf<double> esel;   // [k][j] selected bands
     DBuf<cd> usel;       // [(o*nb + j)*nk + k]  (k fastest, as the reference's vector(k,j,o))
