@@ -50,7 +50,7 @@ __device__ __forceinline__ double lane_set(double v, int i, double x) { return (
 // thread per matrix entry: every entry accumulates over R in the reference's
 // order (the sums and their order of a per-k loop), and each H(R) element is
 // read once for the kKF k-points.  hk[k][r*n + s].
-constexpr int kKF = 16, kFT = 256, kFR = 64;
+constexpr int kKF = 8, kFT = 256, kFR = 64;
 __global__ void __launch_bounds__(kFT)
 hk_fourier_kernel(int nk, int n, int nvec, int flags, const double *__restrict__ kpt, const cd *__restrict__ hop,
                   const double *__restrict__ rcart, const double *__restrict__ invdeg, cd *__restrict__ hk) {
