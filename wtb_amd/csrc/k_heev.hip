@@ -762,7 +762,12 @@ bisect_kernel(int n, int nvec, const double *__restrict__ d, const double *__res
     if (tid == 0 && kk < nvec) W[kk] = 0.5 * (lo + hi);
 }
 
-// Inverse iteration (three steps from a fixed start vector) with partial
+// Inverse iteration steps: with the eigenvalue from bisection (to ~2 eps |l|)
+// the first step leaves an error of ~eps |T| / gap times the start vector's
+// weight ratio, the second squares it; vectors of a cluster are made
+// orthogonal afterwards (cluster_gs_kernel).
+constexpr int kInvIter = 2;
+// Inverse iteration (kInvIter steps from a fixed start vector) with partial
 // pivoting, one thread per eigenvector and `vpw` vectors per workgroup.  The
 // LU factors and the iterate of a vector live in LDS, interleaved across the
 // workgroup's vectors (element j of local vector k at [j vpw + k]), so each
@@ -823,7 +828,7 @@ inviter_kernel(int n, int nvec, int vpw, const double *__restrict__ d, const dou
         z[(size_t)j * vpw + kl] = 1.0 + 1e-4 * (double)(((unsigned)j * 7919u + (unsigned)k * 104729u) & 1023u);
     constexpr int kB = 8;
     double scale = 1.0;     // the iterate is z * scale
-    for (int iter = 0; iter < 3; ++iter) {
+    for (int iter = 0; iter < kInvIter; ++iter) {
         // forward: apply the row operations (L^-1 P) to z, carrying z[j] in a register
         double zj = z[kl] * scale;
         for (int j0 = 0; j0 < n - 1; j0 += kB) {
