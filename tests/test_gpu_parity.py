@@ -115,6 +115,30 @@ def test_bse_matrix(precision, conv):
     ctx.close()
 
 
+def test_bse_matrix_irregular_kpoint_order():
+    """A regular mesh in a shuffled order is not recognised as one: the
+    assembly then evaluates V(k - k') from the k-points instead of reading
+    its table over mesh offsets.  Both must match the oracle on the same
+    k-point list."""
+    say("case bse matrix shuffled k")
+    mesh = (6, 6, 1)
+    m = hex_model(mesh)
+    flags = hip.FLAG_COMPAT_F32 | hip.FLAG_GAUGE_LAPACK
+    k = T.mp_mesh(*mesh, m.rlat)
+    perm = np.random.default_rng(7).permutation(len(k))
+    for kl in (k, k[perm]):
+        ctx = hip.Context(0)
+        ctx.set_tb_model(m, flags)
+        ctx.set_kpoints(kl, mesh)
+        ctx.hk_eig(2, 2)
+        ref = O.hk_eig(O.Model(m, flags=flags), kl, 2, 2, full=True)
+        ctx.bse_build(COUL, hip.FP64)
+        H = ctx.get_block(0, ctx.N, 0, ctx.N)
+        Hr = O.bse_matrix(kl, ref["eig"], ref["vec"], O.coulomb(**COUL), mesh, m.rlat, 2, 2)
+        assert np.abs(H - Hr).max() < 1e-12
+        ctx.close()
+
+
 @pytest.mark.parametrize("kind", ["V2DK", "V3D", "V2DT", "V2DT2", "V2DOH", "V0DT", "V1DT"])
 def test_bse_matrix_potentials(kind):
     say("case potential", kind)

@@ -248,6 +248,31 @@ __device__ __forceinline__ void pair_products_mfma(int norb, const float2 *uk, c
 template <int TK>
 constexpr int pair_slots() { return TK * (TK + 1) + 1; }
 
+// V(k - k') of a pair: from the table over mesh offsets on a regular mesh
+// (vtab non-null), else from the k-points.
+__device__ __forceinline__ double pair_potential(const CoulombDev &coul, const double *kpt, const double *vtab,
+                                                 const int *koff, long vcenter, int k, int kp) {
+    if (vtab) return vtab[vcenter + koff[k] - koff[kp]];
+    return coulomb_v(coul, kpt[3 * k] - kpt[3 * kp], kpt[3 * k + 1] - kpt[3 * kp + 1], kpt[3 * k + 2] - kpt[3 * kp + 2]);
+}
+
+// V over the offsets of a regular mesh: entry t = (d_1 + n_1 - 1) S_1 +
+// (d_2 + n_2 - 1) S_2 + d_3 + n_3 - 1 holds V(d_1 g_1 + d_2 g_2 + d_3 g_3).
+struct KGen {
+    double g[9];
+};
+__global__ void vtab_kernel(CoulombDev coul, int n1, int n2, int n3, KGen kg, double *__restrict__ vtab) {
+    const long S2 = 2L * n3 - 1, S1 = (2L * n2 - 1) * S2, T = (2L * n1 - 1) * S1;
+    const long t = blockIdx.x * (long)blockDim.x + threadIdx.x;
+    if (t >= T) return;
+    const long d1 = t / S1 - (n1 - 1), r = t % S1, d2 = r / S2 - (n2 - 1), d3 = r % S2 - (n3 - 1);
+    const double *g = kg.g;
+    const double dx = (double)d1 * g[0] + (double)d2 * g[3] + (double)d3 * g[6];
+    const double dy = (double)d1 * g[1] + (double)d2 * g[4] + (double)d3 * g[7];
+    const double dz = (double)d1 * g[2] + (double)d2 * g[5] + (double)d3 * g[8];
+    vtab[t] = coulomb_v(coul, dx, dy, dz);
+}
+
 // R: arithmetic/store precision of the overlaps and the matrix; NC, NV:
 // bands; TK: k-points per side of the square tile (TK nc nv rows and columns).
 // Workgroup t covers the tile (bk, bkp) of the lower block triangle,
@@ -264,7 +289,8 @@ constexpr int pair_slots() { return TK * (TK + 1) + 1; }
 template <class R, int NC, int NV, int TK, bool EXT>
 __global__ void __launch_bounds__(kThreads)
 bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
-                 const double *__restrict__ kpt, const double *__restrict__ esel,
+                 const double *__restrict__ kpt, const double *__restrict__ vtab, const int *__restrict__ koff,
+                 long vcenter, const double *__restrict__ esel,
                  const typename CT<R>::C *__restrict__ usel, typename CT<R>::C *__restrict__ H,
                  unsigned *__restrict__ hmax, const double *__restrict__ fdeh, const cd *__restrict__ xa,
                  const double *__restrict__ vqx) {
@@ -332,8 +358,7 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
             const int kp = kp0 + kpl;
             double vq = 0.0;
             if (k < nk && kp < nk)
-                vq = coulomb_v(coul, kpt[3 * k] - kpt[3 * kp], kpt[3 * k + 1] - kpt[3 * kp + 1],
-                               kpt[3 * k + 2] - kpt[3 * kp + 2]);
+                vq = pair_potential(coul, kpt, vtab, koff, vcenter, k, kp);
             pv[kpl * (TK + 1) + kl] = (R)vq;
         }
     } else {
@@ -361,8 +386,7 @@ bse_build_kernel(int nk, int norb, long ld, int textbook, CoulombDev coul,
             const int kp = kp0 + kpl;
             double vq = 0.0;
             if (k < nk && kp < nk)
-                vq = coulomb_v(coul, kpt[3 * k] - kpt[3 * kp], kpt[3 * k + 1] - kpt[3 * kp + 1],
-                               kpt[3 * k + 2] - kpt[3 * kp + 2]);
+                vq = pair_potential(coul, kpt, vtab, koff, vcenter, k, kp);
             pv[p] = (R)vq;
 #pragma unroll
             for (int e = 0; e < NC2; ++e) pmc[(size_t)e * NPP + p] = mc[s][e];
@@ -495,14 +519,20 @@ void launch_tile(Ctx &c, const CoulombDev &cd_, int textbook, const double *esel
     const long nblk = (nk + TK - 1) / TK;
     const long ntile = nblk * (nblk + 1) / 2;
     unsigned *hm = sizeof(R) == sizeof(float) ? c.bse.hmax.p : nullptr;
+    // the potential table of a regular mesh (filled by launch_bse_build)
+    const double *vt = c.kmesh ? c.vtab.p : nullptr;
+    const long vc = (long)(c.ngrid[0] - 1) * (2L * c.ngrid[1] - 1) * (2L * c.ngrid[2] - 1) +
+                    (long)(c.ngrid[1] - 1) * (2L * c.ngrid[2] - 1) + (c.ngrid[2] - 1);
     const double *fd = c.bse.temperature >= 0.0 ? c.bse.fdeh.p : nullptr;
     const cd *xa = c.bse.xa.n ? c.bse.xa.p : nullptr;
     if (fd || xa)
         hipLaunchKernelGGL((bse_build_kernel<R, NC, NV, TK, true>), dim3((unsigned)ntile), dim3(kThreads), shmem,
-                           c.stream, nk, norb, ld, textbook, cd_, c.kpt.p, esel, usel, H, hm, fd, xa, c.bse.vqx.p);
+                           c.stream, nk, norb, ld, textbook, cd_, c.kpt.p, vt, c.kmesh_off.p, vc, esel, usel, H, hm, fd, xa,
+                           c.bse.vqx.p);
     else
         hipLaunchKernelGGL((bse_build_kernel<R, NC, NV, TK, false>), dim3((unsigned)ntile), dim3(kThreads), shmem,
-                           c.stream, nk, norb, ld, textbook, cd_, c.kpt.p, esel, usel, H, hm, nullptr, nullptr,
+                           c.stream, nk, norb, ld, textbook, cd_, c.kpt.p, vt, c.kmesh_off.p, vc, esel, usel, H, hm, nullptr,
+                           nullptr,
                            nullptr);
     WTB_LAUNCH_CHECK();
     if (ld > N) {
@@ -599,6 +629,51 @@ __global__ void fdeh_kernel(int nk, int nc, int nv, double temp, const double *_
 
 }  // namespace
 
+void detect_kmesh(Ctx &c, const std::vector<double> &k) {
+    c.kmesh = false;
+    const long n1 = c.ngrid[0], n2 = c.ngrid[1], n3 = c.ngrid[2], nk = c.nk;
+    if (n1 * n2 * n3 != nk) return;
+    const long T = (2 * n1 - 1) * (2 * n2 - 1) * (2 * n3 - 1);
+    if (T > (1L << 24)) return;
+    // the last coordinate running fastest, or the first
+    for (int order = 0; order < 2; ++order) {
+        const auto idx = [&](long c1, long c2, long c3) {
+            return order == 0 ? (c1 * n2 + c2) * n3 + c3 : (c3 * n2 + c2) * n1 + c1;
+        };
+        double g[9] = {0};
+        double gmax = 0.0;
+        const long unit[3] = {n1 > 1 ? idx(1, 0, 0) : -1, n2 > 1 ? idx(0, 1, 0) : -1, n3 > 1 ? idx(0, 0, 1) : -1};
+        for (int j = 0; j < 3; ++j)
+            if (unit[j] >= 0)
+                for (int d = 0; d < 3; ++d) {
+                    g[3 * j + d] = k[3 * unit[j] + d] - k[d];
+                    gmax = std::max(gmax, std::fabs(g[3 * j + d]));
+                }
+        const double tol = 1e-9 * std::max(gmax, 1e-300);
+        bool ok = gmax > 0.0;
+        for (long c1 = 0; c1 < n1 && ok; ++c1)
+            for (long c2 = 0; c2 < n2 && ok; ++c2)
+                for (long c3 = 0; c3 < n3 && ok; ++c3) {
+                    const long kk = idx(c1, c2, c3);
+                    for (int d = 0; d < 3; ++d) {
+                        const double p = k[d] + (double)c1 * g[d] + (double)c2 * g[3 + d] + (double)c3 * g[6 + d];
+                        if (!(std::fabs(p - k[3 * kk + d]) <= tol)) ok = false;
+                    }
+                }
+        if (!ok) continue;
+        const long S2 = 2 * n3 - 1, S1 = (2 * n2 - 1) * S2;
+        std::vector<int> off(nk);
+        for (long c1 = 0; c1 < n1; ++c1)
+            for (long c2 = 0; c2 < n2; ++c2)
+                for (long c3 = 0; c3 < n3; ++c3) off[idx(c1, c2, c3)] = (int)(c1 * S1 + c2 * S2 + c3);
+        c.kmesh_off.alloc(nk);
+        WTB_HIP_CHECK(hipMemcpy(c.kmesh_off.p, off.data(), sizeof(int) * nk, hipMemcpyHostToDevice));
+        for (int i = 0; i < 9; ++i) c.kgen[i] = g[i];
+        c.kmesh = true;
+        return;
+    }
+}
+
 void launch_bse_build(Ctx &c, const wtb_coulomb &coul, int precision, int convention) {
     Bands &b = c.bands;
     if (!b.valid) throw Error{WTB_ERR_STATE, "wtb_hk_eig must run before wtb_bse_build"};
@@ -609,6 +684,15 @@ void launch_bse_build(Ctx &c, const wtb_coulomb &coul, int precision, int conven
     const int nk = b.nk, nc = b.nc, nv = b.nv;
     const long N = (long)nk * nc * nv;
     CoulombDev cd_ = make_coulomb(coul, c.model.rlat, c.ngrid);
+    if (c.kmesh) {
+        const long T = (2L * c.ngrid[0] - 1) * (2L * c.ngrid[1] - 1) * (2L * c.ngrid[2] - 1);
+        c.vtab.alloc(T);
+        KGen kg;
+        for (int i = 0; i < 9; ++i) kg.g[i] = c.kgen[i];
+        hipLaunchKernelGGL(vtab_kernel, dim3((unsigned)((T + 255) / 256)), dim3(256), 0, c.stream, cd_, c.ngrid[0],
+                           c.ngrid[1], c.ngrid[2], kg, c.vtab.p);
+        WTB_LAUNCH_CHECK();
+    }
     Bse &s = c.bse;
     s.valid = false;
     c.sol.valid = false;

@@ -183,6 +183,7 @@ int wtb_set_kpoints(wtb_ctx *ctx, int nk, const double *kpt, const int *ngrid) {
         copy(c, c.kpt.p, k.data(), sizeof(double) * k.size(), hipMemcpyHostToDevice);
         c.nk = nk;
         for (int i = 0; i < 3; ++i) c.ngrid[i] = ngrid[i];
+        detect_kmesh(c, k);
         c.bands.valid = false;
         c.fq.on = false;
         c.fq.bq.valid = false;

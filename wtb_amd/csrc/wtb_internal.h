@@ -243,6 +243,15 @@ struct Ctx {
     int nk = 0;
     int ngrid[3] = {1, 1, 1};
     DBuf<double> kpt;    // [k][3]
+    // Regular mesh: kpt[k] = kpt[0] + c_1 g_1 + c_2 g_2 + c_3 g_3 with integer
+    // c_j in [0, ngrid_j) for every k (checked when the k-points are set).
+    // Then V(k - k') depends on c(k) - c(k') only: the assembly reads it from
+    // a table over the offsets, kmesh_off[k] = c_1 S_1 + c_2 S_2 + c_3 with
+    // S_2 = 2 n_3 - 1, S_1 = (2 n_2 - 1) S_2.
+    bool kmesh = false;
+    double kgen[9] = {0};       // g_1, g_2, g_3
+    DBuf<int> kmesh_off;        // [k]
+    DBuf<double> vtab;          // [offset] V of the last assembly's potential
     Bands bands;
     FiniteQ fq;
     Bse bse;
@@ -388,6 +397,9 @@ void heev_lowest(Ctx &c, int n, int nvec, cd *A, int lda, double *W, cd *Z, int 
 // 1 + the first column whose pivot falls below rel * its original diagonal.
 int cholesky_upper(Ctx &c, int n, cd *G, int ldg, double rel, bool check = true, double *min_ratio = nullptr);
 const int *cholesky_upper_async(Ctx &c, int n, cd *G, int ldg, double rel, int *hflag, hipEvent_t ev);
+// Regular-mesh check of the k-points k ([k][3], host) against c.ngrid: sets
+// c.kmesh, c.kgen and c.kmesh_off.
+void detect_kmesh(Ctx &c, const std::vector<double> &k);
 
 // Copy H(i0:i0+ni, j0:j0+nj) of the stored matrix into out (ni x nj, ldo) as FP64.
 // gradients of sum_n w_n E_n (k_grad.hip)
